@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Benchmark: GN semi-Lagrangian Hessian matvecs per second at 256³ (FP32).
+
+Workload (one "step" = one CLAIRE::HessianMatVec, src/CLAIRE.cpp:1311-1379,
+as timed by the reference harness apps/benchmark.cpp:234-283):
+  grid 256³ periodic on [0,2π)³, nt = 4, GN, semi-Lagrangian, nc = 1;
+  velocity band-limited (|w|∞ <= 8, N(0,1) coefficients decayed by
+  1/(1+|w|²)) scaled to CFL 2; template = 24 seeded periodic Gaussian blobs;
+  H1SN regularization β = 1e-2 (the harness default, src/RegOpt.cpp:1071-1073);
+  incremental velocity ṽ = v as in the reference harness
+  (SetIncControlVariable(v), apps/benchmark.cpp:252).
+Inputs are resident in HBM before the timed region; the Newton-invariant
+data (trajectories, state, ∇m^j) are built once before it, like the
+reference's warm-up (forward solve + gradient + one matvec,
+apps/benchmark.cpp:257-260).
+
+Prints one JSON line.  Only N = 1 GPU is supported.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Hessian matvec/s + interp GB/s @256^3, mixed vs plain FP32, relerr≤1e-5"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--n", type=int, default=256, help="grid size per axis")
+    p.add_argument("--nt", type=int, default=4)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-reps", type=int, default=1)
+    p.add_argument("--no-parity", action="store_true")
+    p.add_argument("--fp64", action="store_true", help="also score GPU and CPU FP32 against the FP64 oracle")
+    return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import claire_amd
+    from claire_amd import Claire, DeviceArray, synth
+
+    n, nt = args.n, args.nt
+    shape = (n, n, n)
+    beta = (1e-2, 1e-4, 0.0)
+    regnorm = "h1sn"
+    t0 = time.time()
+    v = synth.bandlimited_velocity(shape, kmax=8, cfl=2.0, nt=nt, seed=0)
+    mT = synth.gaussian_blobs(shape, count=24, seed=1)
+    log(f"synthetic inputs {shape} in {time.time() - t0:.1f}s")
+
+    c = Claire(shape, nt=nt, regnorm=regnorm, beta=beta)
+    c.set_velocity(v)
+    c.set_template_image(mT)
+    c.prepare_hessian()
+    vt = DeviceArray.from_host(v)
+    hv = DeviceArray((3,) + shape)
+    c.sync()
+
+    for _ in range(args.warmup):
+        c.hessian_matvec_device(hv, vt)
+    c.sync()
+
+    # timed region: exactly K matvecs, synchronized on both sides
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        c.hessian_matvec_device(hv, vt)
+    c.sync()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = args.steps / elapsed
+    log(f"{args.steps} matvecs in {elapsed * 1e3:.1f} ms -> {value:.1f} matvec/s")
+
+    # per-kernel times: the same K matvecs with HIP events around every launch
+    c.profile_reset()
+    c.profile_enable(True)
+    for _ in range(args.steps):
+        c.hessian_matvec_device(hv, vt)
+    c.profile_enable(False)
+    prof = c.profile()
+    total_ms = sum(p[0] for p in prof.values())
+    kernels = {}
+    for name, (ms, cnt, by) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        avg_us = 1e3 * ms / cnt
+        kernels[name] = {"avg_us": round(avg_us, 2), "launches_per_step": cnt / args.steps,
+                         "share": round(ms / total_ms, 4), "GBps": round(by / (ms * 1e-3) / 1e9, 1)}
+    dom = max(prof.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dcnt, dbytes) = dom
+    avg_s = dms * 1e-3 / dcnt
+    achieved = (dbytes / dcnt) / avg_s / 1e9
+    roofline = {"kernel": dname, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": dbytes / dcnt, "avg_us": round(avg_s * 1e6, 2)}
+
+    # interpolation throughput: scalar gather along the state trajectory (20 B/point)
+    f = DeviceArray.from_host(mT)
+    fo = DeviceArray(shape)
+    c.profile_reset()
+    c.profile_enable(True)
+    for _ in range(args.steps):
+        claire_amd.lib().clr_interp_scalar(c.ctx, 0, claire_amd._vp(f.ptr), claire_amd._vp(fo.ptr))
+    c.profile_enable(False)
+    g = c.profile()["gather_scalar"]
+    interp = {"kernel": "gather_scalar", "GBps": round(g[2] / (g[0] * 1e-3) / 1e9, 1),
+              "frac": round(g[2] / (g[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "avg_us": round(1e3 * g[0] / g[1], 2),
+              "bytes_per_point": 20}
+
+    out = {"metric": METRIC, "value": round(value, 3), "unit": "matvec/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"GN-SL Hessian matvec, {n}^3 FP32, nt={nt}, H1SN beta=1e-2, band-limited v "
+                                  f"|k|<=8 CFL 2, 24 Gaussian blobs",
+                      "grid": list(shape), "nt": nt, "regnorm": regnorm, "beta": list(beta[:2]), "precision": "fp32"},
+           "roofline": roofline, "interp": interp, "kernels": kernels,
+           "device_bytes": c.device_bytes()}
+
+    Hgpu = hv.download()
+
+    if not args.no_cpu_baseline:
+        from oracle import oracle_py
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        threads = max(1, min(threads, 16))
+        oracle_py.set_num_threads(threads)
+        log(f"CPU reference path: FP32 port, {threads} threads")
+        m32 = oracle_py.solve_state(v, mT, nt, prec="f32")
+        Hcpu, secs = oracle_py.hessian_bench(v, m32, v, nt, regnorm, beta, reps=args.cpu_reps, prec="f32")
+        out["cpu_baseline"] = {"value": round(args.cpu_reps / secs, 5), "unit": "matvec/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{args.cpu_reps} Hessian matvec(s) of the same {n}^3 workload, FP32 "
+                                         f"OpenMP restatement, trajectories precomputed ({secs:.1f}s)"}
+        if not args.no_parity:
+            def rel(a, b):
+                a = a.astype(np.float64)
+                b = b.astype(np.float64)
+                return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+            out["relerr_vs_cpu_fp32"] = rel(Hgpu, Hcpu)
+            if args.fp64:
+                log("FP64 oracle for the same matvec")
+                m64 = oracle_py.solve_state(v, mT, nt, prec="f64")
+                H64 = oracle_py.hessian_matvec(v, m64, v, nt, regnorm, beta, prec="f64")
+                out["relerr_vs_fp64"] = {"gpu_fp32": rel(Hgpu, H64), "cpu_fp32": rel(Hcpu, H64)}
+    c.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,325 @@
+"""claire_amd -- MI355X-native CLAIRE registration hot path (Python binding).
+
+A thin ctypes layer over the C ABI in include/claire_amd.h, implemented by
+the in-tree HIP library claire_amd/libclaire_amd.so (hand-written gfx950
+kernels + C++ host classes mirroring the reference's CLAIRE,
+SemiLagrangian, Regularization and VecField).  There is no CPU fallback:
+if the library is missing or no GPU is usable the calls raise.
+
+Array conventions (host numpy arrays, FP32):
+  scalar field   (n1, n2, n3), x3 fastest
+  vector field   (3, n1, n2, n3), the flat [v1|v2|v3] solver vector
+  time series    (nt+1, n1, n2, n3)
+  query points   (n1, n2, n3, 3), normalized by 2π
+  half spectrum  (n1, n2, n3//2+1) complex64
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libclaire_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "claire_amd.h")
+
+REG = {"h1": 0, "h2": 1, "h1sn": 2, "h2sn": 3}
+STATE, ADJOINT = 0, 1
+
+_lib = None
+
+
+class ClaireError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libclaire_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ClaireError(-1, f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.clr_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def declared_symbols():
+    """Names of the functions declared in include/claire_amd.h."""
+    with open(HEADER_PATH) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(clr_\w+)\s*\(", text, flags=re.M)))
+
+
+def _check(rc):
+    if rc != 0:
+        raise ClaireError(rc, lib().clr_last_error().decode())
+
+
+def _vp(p):
+    return ctypes.c_void_p(p)
+
+
+class DeviceArray:
+    """Device allocation owned by Python (clr_device_malloc/free)."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().clr_device_malloc(ctypes.byref(p), ctypes.c_size_t(max(self.nbytes, 4))))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, dtype=np.float32):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.shape, dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        _check(lib().clr_memcpy_htod(_vp(self.ptr), a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(self.nbytes)))
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(lib().clr_memcpy_dtoh(out.ctypes.data_as(ctypes.c_void_p), _vp(self.ptr), ctypes.c_size_t(self.nbytes)))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().clr_device_free(_vp(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Claire:
+    """One registration problem (GN, semi-Lagrangian, nc = 1) on the GPU."""
+
+    def __init__(self, nx, nt=4, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0), stream=None):
+        self.nx = tuple(int(n) for n in nx)
+        self.nt = int(nt)
+        self.nl = int(np.prod(self.nx))
+        self.nh = self.nx[2] // 2 + 1
+        b = list(beta) + [0.0] * (3 - len(beta))
+        ctx = ctypes.c_void_p()
+        _check(lib().clr_create(ctypes.byref(ctx), (ctypes.c_int * 3)(*self.nx), ctypes.c_int(self.nt),
+                                ctypes.c_int(REG[regnorm]), (ctypes.c_double * 3)(*b[:3]), _vp(stream or 0)))
+        self.ctx = ctx
+        self._bufs = {}
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().clr_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- scratch device buffers keyed by role -----------------------------
+    def buf(self, key, shape, dtype=np.float32):
+        d = self._bufs.get(key)
+        if d is None or d.shape != tuple(shape) or d.dtype != np.dtype(dtype):
+            d = DeviceArray(shape, dtype)
+            self._bufs[key] = d
+        return d
+
+    def _in(self, key, a, shape):
+        if isinstance(a, DeviceArray):
+            return a
+        d = self.buf(key, shape)
+        d.upload(np.asarray(a, dtype=np.float32).reshape(shape))
+        return d
+
+    def sync(self):
+        _check(lib().clr_synchronize(self.ctx))
+
+    def device_bytes(self):
+        n = ctypes.c_size_t()
+        _check(lib().clr_device_bytes(self.ctx, ctypes.byref(n)))
+        return n.value
+
+    # -- problem data -------------------------------------------------------
+    def set_velocity(self, v):
+        d = self._in("v", v, (3,) + self.nx)
+        _check(lib().clr_set_velocity(self.ctx, _vp(d.ptr)))
+
+    def set_inc_velocity(self, vt):
+        d = self._in("vt", vt, (3,) + self.nx)
+        _check(lib().clr_set_inc_velocity(self.ctx, _vp(d.ptr)))
+
+    def set_template_image(self, mT):
+        d = self._in("mT", mT, self.nx)
+        _check(lib().clr_set_template_image(self.ctx, _vp(d.ptr)))
+
+    def set_reference_image(self, mR):
+        d = self._in("mR", mR, self.nx)
+        _check(lib().clr_set_reference_image(self.ctx, _vp(d.ptr)))
+
+    # -- solves -----------------------------------------------------------------
+    def solve_state(self):
+        out = self.buf("m", (self.nt + 1,) + self.nx)
+        _check(lib().clr_solve_state(self.ctx, _vp(out.ptr)))
+        return out.download()
+
+    def solve_adjoint(self):
+        lam = self.buf("lam", self.nx)
+        b = self.buf("b", (3,) + self.nx)
+        _check(lib().clr_solve_adjoint(self.ctx, _vp(lam.ptr), _vp(b.ptr)))
+        return lam.download(), b.download()
+
+    def solve_inc_state(self, vt):
+        d = self._in("vt", vt, (3,) + self.nx)
+        out = self.buf("mt", self.nx)
+        _check(lib().clr_solve_inc_state(self.ctx, _vp(d.ptr), _vp(out.ptr)))
+        return out.download()
+
+    def solve_inc_adjoint(self, mt):
+        d = self._in("mt_in", mt, self.nx)
+        lt = self.buf("lt", self.nx)
+        bt = self.buf("bt", (3,) + self.nx)
+        _check(lib().clr_solve_inc_adjoint(self.ctx, _vp(d.ptr), _vp(lt.ptr), _vp(bt.ptr)))
+        return lt.download(), bt.download()
+
+    def prepare_hessian(self):
+        _check(lib().clr_prepare_hessian(self.ctx))
+
+    def hessian_matvec_device(self, out, vt, scale=True):
+        _check(lib().clr_hessian_matvec(self.ctx, _vp(out.ptr), _vp(vt.ptr), ctypes.c_int(int(scale))))
+
+    def hessian_matvec(self, vt, scale=True):
+        d = self._in("hv_in", vt, (3,) + self.nx)
+        out = self.buf("hv", (3,) + self.nx)
+        self.hessian_matvec_device(out, d, scale)
+        return out.download()
+
+    def evaluate_objective(self, v=None):
+        vp = _vp(self._in("v", v, (3,) + self.nx).ptr) if v is not None else _vp(0)
+        J, D, R = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().clr_evaluate_objective(self.ctx, vp, ctypes.byref(J), ctypes.byref(D), ctypes.byref(R)))
+        return J.value, D.value, R.value
+
+    def evaluate_gradient(self, v=None):
+        vp = _vp(self._in("v", v, (3,) + self.nx).ptr) if v is not None else _vp(0)
+        g = self.buf("g", (3,) + self.nx)
+        _check(lib().clr_evaluate_gradient(self.ctx, vp, _vp(g.ptr)))
+        return g.download()
+
+    # -- regularization -----------------------------------------------------
+    def apply_inv_reg(self, x, applysqrt=False):
+        d = self._in("x", x, (3,) + self.nx)
+        out = self.buf("ax", (3,) + self.nx)
+        _check(lib().clr_apply_inv_reg(self.ctx, _vp(d.ptr), _vp(out.ptr), ctypes.c_int(int(applysqrt))))
+        return out.download()
+
+    def reg_apply(self, v):
+        d = self._in("x", v, (3,) + self.nx)
+        out = self.buf("ax", (3,) + self.nx)
+        _check(lib().clr_reg_apply(self.ctx, _vp(d.ptr), _vp(out.ptr)))
+        return out.download()
+
+    def reg_functional(self, v):
+        d = self._in("x", v, (3,) + self.nx)
+        r = ctypes.c_double()
+        _check(lib().clr_reg_functional(self.ctx, _vp(d.ptr), ctypes.byref(r)))
+        return r.value
+
+    def reg_eigvals_inv(self):
+        lo, hi = ctypes.c_double(), ctypes.c_double()
+        _check(lib().clr_reg_eigvals_inv(self.ctx, ctypes.byref(lo), ctypes.byref(hi)))
+        return lo.value, hi.value
+
+    # -- semi-Lagrangian building blocks --------------------------------------
+    def trajectory(self, flag=STATE):
+        out = self.buf("X", self.nx + (3,))
+        _check(lib().clr_trajectory(self.ctx, ctypes.c_int(flag), _vp(out.ptr)))
+        return out.download()
+
+    def interp_scalar(self, f, flag=STATE):
+        d = self._in("f", f, self.nx)
+        out = self.buf("fo", self.nx)
+        _check(lib().clr_interp_scalar(self.ctx, ctypes.c_int(flag), _vp(d.ptr), _vp(out.ptr)))
+        return out.download()
+
+    def interp_vector(self, v, flag=STATE):
+        d = self._in("f3", v, (3,) + self.nx)
+        out = self.buf("fo3", (3,) + self.nx)
+        _check(lib().clr_interp_vector(self.ctx, ctypes.c_int(flag), _vp(d.ptr), _vp(out.ptr)))
+        return out.download()
+
+    def interp_points(self, f, X):
+        d = self._in("f", f, self.nx)
+        x = self._in("Xq", X, self.nx + (3,))
+        out = self.buf("fo", self.nx)
+        _check(lib().clr_interp_points(self.ctx, _vp(d.ptr), _vp(x.ptr), _vp(out.ptr)))
+        return out.download()
+
+    # -- spectral ---------------------------------------------------------------
+    def grad(self, f):
+        d = self._in("f", f, self.nx)
+        out = self.buf("g3", (3,) + self.nx)
+        _check(lib().clr_grad(self.ctx, _vp(d.ptr), _vp(out.ptr)))
+        return out.download()
+
+    def div(self, v):
+        d = self._in("f3", v, (3,) + self.nx)
+        out = self.buf("fo", self.nx)
+        _check(lib().clr_div(self.ctx, _vp(d.ptr), _vp(out.ptr)))
+        return out.download()
+
+    def fft_r2c(self, f):
+        d = self._in("f", f, self.nx)
+        out = self.buf("spec", (self.nx[0], self.nx[1], self.nh), np.complex64)
+        _check(lib().clr_fft_r2c(self.ctx, _vp(d.ptr), _vp(out.ptr)))
+        return out.download()
+
+    def fft_c2r(self, fh):
+        shape = (self.nx[0], self.nx[1], self.nh)
+        d = self.buf("spec_in", shape, np.complex64)
+        d.upload(np.asarray(fh, dtype=np.complex64).reshape(shape))
+        out = self.buf("fo", self.nx)
+        _check(lib().clr_fft_c2r(self.ctx, _vp(d.ptr), _vp(out.ptr)))
+        return out.download()
+
+    def dot(self, a, b):
+        da = DeviceArray.from_host(a)
+        db = DeviceArray.from_host(b)
+        r = ctypes.c_double()
+        _check(lib().clr_dot(self.ctx, _vp(da.ptr), _vp(db.ptr), ctypes.c_int64(da.nbytes // 4), ctypes.byref(r)))
+        return r.value
+
+    # -- kernel timing ---------------------------------------------------------
+    def profile_enable(self, on=True):
+        _check(lib().clr_profile_enable(self.ctx, ctypes.c_int(int(on))))
+
+    def profile_reset(self):
+        _check(lib().clr_profile_reset(self.ctx))
+
+    def profile(self):
+        """{kernel class: (ms, count, algorithmic bytes)} accumulated so far."""
+        out = {}
+        i = 0
+        name = ctypes.create_string_buffer(128)
+        while True:
+            ms, cnt, by = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+            rc = lib().clr_profile_get(self.ctx, ctypes.c_int(i), name, ctypes.c_int(128), ctypes.byref(ms),
+                                       ctypes.byref(cnt), ctypes.byref(by))
+            if rc != 0:
+                break
+            out[name.value.decode()] = (ms.value, cnt.value, by.value)
+            i += 1
+        return out

@@ -1,0 +1,435 @@
+// capi.cpp -- the extern "C" boundary (include/claire_amd.h).
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "../../include/claire_amd.h"
+#include "claire.hpp"
+
+using claire_amd::Error;
+
+struct clr_ctx {
+    std::unique_ptr<reg::Profiler> prof;
+    std::unique_ptr<reg::CLAIRE> claire;
+    std::unique_ptr<claire_amd::FFTPlan3D> plan;  // for the raw FFT entry points
+    reg::DeviceArray<claire_amd::c32> spec;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return CLR_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return CLR_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return CLR_ERR_INVALID_ARGUMENT;
+    }
+}
+
+void need(const void* p, const char* what) {
+    if (!p) throw Error(claire_amd::kErrInvalidArgument, std::string("null pointer: ") + what);
+}
+
+reg::CLAIRE& C(clr_ctx* ctx) {
+    need(ctx, "context");
+    return *ctx->claire;
+}
+
+const reg::RegOpt& O(clr_ctx* ctx) { return C(ctx).Opt(); }
+
+void check_hip_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(claire_amd::kErrHip, std::string("kernel launch failed: ") + hipGetErrorString(e));
+}
+}  // namespace
+
+extern "C" {
+
+const char* clr_last_error(void) { return g_err.c_str(); }
+
+int clr_device_malloc(void** ptr, size_t bytes) {
+    return guard([&] {
+        need(ptr, "ptr");
+        *ptr = nullptr;
+        CLR_HIP_CHECK(hipMalloc(ptr, bytes));
+    });
+}
+int clr_device_free(void* ptr) {
+    return guard([&] {
+        if (ptr) CLR_HIP_CHECK(hipFree(ptr));
+    });
+}
+int clr_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    return guard([&] { CLR_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int clr_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    return guard([&] { CLR_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int clr_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    return guard([&] { CLR_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); });
+}
+int clr_device_synchronize(void) {
+    return guard([&] { CLR_HIP_CHECK(hipDeviceSynchronize()); });
+}
+
+int clr_create(clr_ctx** out, const int nx[3], int nt, int regnorm, const double beta[3], void* stream) {
+    return guard([&] {
+        need(out, "ctx");
+        need(nx, "nx");
+        need(beta, "beta");
+        *out = nullptr;
+        CLR_REQUIRE(regnorm >= 0 && regnorm <= 3, claire_amd::kErrInvalidArgument, "unknown regularization norm");
+        CLR_REQUIRE(nt >= 1, claire_amd::kErrInvalidArgument, "nt must be >= 1");
+        for (int a = 0; a < 3; ++a)
+            CLR_REQUIRE(claire_amd::fft_size_supported(nx[a]), claire_amd::kErrUnsupportedSize,
+                        "grid size " + std::to_string(nx[a]) + " not supported (powers of two 8..512)");
+        std::unique_ptr<clr_ctx> c(new clr_ctx());
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        c->prof.reset(new reg::Profiler(st));
+        reg::RegOpt opt;
+        opt.domain = claire_amd::make_dims(nx);
+        opt.nt = nt;
+        opt.nc = 1;
+        opt.regnorm = regnorm;
+        for (int i = 0; i < 3; ++i) opt.beta[i] = static_cast<float>(beta[i]);
+        opt.stream = st;
+        opt.prof = c->prof.get();
+        c->claire.reset(new reg::CLAIRE(opt));
+        check_hip_launch();
+        *out = c.release();
+    });
+}
+
+int clr_destroy(clr_ctx* ctx) {
+    return guard([&] {
+        if (!ctx) return;
+        (void)hipStreamSynchronize(ctx->claire ? ctx->claire->Opt().stream : nullptr);
+        delete ctx;
+    });
+}
+
+int clr_synchronize(clr_ctx* ctx) {
+    return guard([&] { CLR_HIP_CHECK(hipStreamSynchronize(O(ctx).stream)); });
+}
+
+int clr_device_bytes(clr_ctx* ctx, size_t* bytes) {
+    return guard([&] {
+        need(bytes, "bytes");
+        *bytes = C(ctx).DeviceBytes();
+    });
+}
+
+int clr_set_velocity(clr_ctx* ctx, const float* v) {
+    return guard([&] {
+        need(v, "v");
+        C(ctx).SetControlVariable(v);
+        check_hip_launch();
+    });
+}
+int clr_set_inc_velocity(clr_ctx* ctx, const float* vt) {
+    return guard([&] {
+        need(vt, "vtilde");
+        C(ctx).SetIncControlVariable(vt);
+        check_hip_launch();
+    });
+}
+int clr_set_template_image(clr_ctx* ctx, const float* mT) {
+    return guard([&] { C(ctx).SetTemplateImage(mT); });
+}
+int clr_set_reference_image(clr_ctx* ctx, const float* mR) {
+    return guard([&] { C(ctx).SetReferenceImage(mR); });
+}
+
+int clr_solve_state(clr_ctx* ctx, float* m_out) {
+    return guard([&] {
+        reg::CLAIRE& c = C(ctx);
+        c.SolveStateEquation();
+        check_hip_launch();
+        if (m_out) {
+            const size_t n = static_cast<size_t>(c.Opt().nt + 1) * c.Opt().domain.nl;
+            CLR_HIP_CHECK(hipMemcpyAsync(m_out, c.StateVariable(), n * sizeof(float), hipMemcpyDeviceToDevice,
+                                         c.Opt().stream));
+        }
+    });
+}
+
+int clr_solve_adjoint(clr_ctx* ctx, float* lambda_out, float* b_out) {
+    return guard([&] {
+        reg::CLAIRE& c = C(ctx);
+        c.SolveAdjointEquation();
+        check_hip_launch();
+        const size_t nl = static_cast<size_t>(c.Opt().domain.nl);
+        if (lambda_out)
+            CLR_HIP_CHECK(hipMemcpyAsync(lambda_out, c.AdjointVariable(), nl * sizeof(float), hipMemcpyDeviceToDevice,
+                                         c.Opt().stream));
+        if (b_out) c.BodyForce()->GetComponents(b_out);
+    });
+}
+
+int clr_solve_inc_state(clr_ctx* ctx, const float* vtilde, float* mtilde_out) {
+    return guard([&] {
+        reg::CLAIRE& c = C(ctx);
+        if (vtilde) c.SetIncControlVariable(vtilde);
+        c.SolveIncStateEquation();
+        check_hip_launch();
+        if (mtilde_out)
+            CLR_HIP_CHECK(hipMemcpyAsync(mtilde_out, c.IncStateVariable(), c.Opt().domain.nl * sizeof(float),
+                                         hipMemcpyDeviceToDevice, c.Opt().stream));
+    });
+}
+
+int clr_solve_inc_adjoint(clr_ctx* ctx, const float* mtilde, float* ltilde_out, float* btilde_out) {
+    return guard([&] {
+        reg::CLAIRE& c = C(ctx);
+        if (mtilde)
+            c.SolveIncAdjointFrom(mtilde);
+        else
+            c.SolveIncAdjointEquation();
+        check_hip_launch();
+        if (ltilde_out)
+            CLR_HIP_CHECK(hipMemcpyAsync(ltilde_out, c.IncAdjointVariable(), c.Opt().domain.nl * sizeof(float),
+                                         hipMemcpyDeviceToDevice, c.Opt().stream));
+        if (btilde_out) c.IncBodyForce()->GetComponents(btilde_out);
+    });
+}
+
+int clr_hessian_matvec(clr_ctx* ctx, float* Hv, const float* vtilde, int scale) {
+    return guard([&] {
+        C(ctx).HessianMatVec(Hv, vtilde, scale != 0);
+        check_hip_launch();
+    });
+}
+
+int clr_prepare_hessian(clr_ctx* ctx) {
+    return guard([&] {
+        C(ctx).PrepareHessian();
+        check_hip_launch();
+    });
+}
+
+int clr_evaluate_objective(clr_ctx* ctx, const float* v, double* J, double* D, double* R) {
+    return guard([&] {
+        need(J, "J");
+        double d = 0, r = 0, j = 0;
+        C(ctx).EvaluateObjective(&j, &d, &r, v);
+        *J = j;
+        if (D) *D = d;
+        if (R) *R = r;
+    });
+}
+
+int clr_evaluate_gradient(clr_ctx* ctx, const float* v, float* g) {
+    return guard([&] {
+        need(g, "g");
+        C(ctx).EvaluateGradient(g, v);
+        check_hip_launch();
+    });
+}
+
+int clr_apply_inv_reg(clr_ctx* ctx, const float* in, float* out, int applysqrt) {
+    return guard([&] {
+        need(in, "in");
+        need(out, "out");
+        C(ctx).ApplyInvRegularizationOperator(out, in, applysqrt != 0);
+        check_hip_launch();
+    });
+}
+
+int clr_reg_apply(clr_ctx* ctx, const float* v, float* out) {
+    return guard([&] {
+        need(v, "v");
+        need(out, "out");
+        reg::CLAIRE& c = C(ctx);
+        reg::VecField o(&c.Opt(), out), in(&c.Opt(), const_cast<float*>(v));
+        c.Reg()->HessianMatVec(&o, &in);
+        check_hip_launch();
+    });
+}
+
+int clr_reg_functional(clr_ctx* ctx, const float* v, double* R) {
+    return guard([&] {
+        need(v, "v");
+        need(R, "R");
+        reg::CLAIRE& c = C(ctx);
+        reg::VecField in(&c.Opt(), const_cast<float*>(v));
+        c.Reg()->EvaluateFunctional(R, &in);
+    });
+}
+
+int clr_reg_eigvals_inv(clr_ctx* ctx, double* emin, double* emax) {
+    return guard([&] {
+        need(emin, "emin");
+        need(emax, "emax");
+        float lo = 0, hi = 0;
+        C(ctx).Reg()->GetExtremeEigValsInvOp(lo, hi);
+        *emin = lo;
+        *emax = hi;
+    });
+}
+
+int clr_trajectory(clr_ctx* ctx, int flag, float* X_out) {
+    return guard([&] {
+        need(X_out, "X");
+        reg::CLAIRE& c = C(ctx);
+        const std::string f = flag == CLR_STATE ? "state" : (flag == CLR_ADJOINT ? "adjoint" : "?");
+        if (f == "?") throw Error(claire_amd::kErrInvalidArgument, "flag wrong");
+        c.EnsureTrajectories();
+        CLR_HIP_CHECK(hipMemcpyAsync(X_out, c.SL()->Trajectory(f), 3 * c.Opt().domain.nl * sizeof(float),
+                                     hipMemcpyDeviceToDevice, c.Opt().stream));
+    });
+}
+
+namespace {
+void ensure_traj(reg::CLAIRE& c, int flag) {
+    const std::string f = flag == CLR_STATE ? "state" : (flag == CLR_ADJOINT ? "adjoint" : "?");
+    if (f == "?") throw Error(claire_amd::kErrInvalidArgument, "flag wrong");
+    c.EnsureTrajectories();
+}
+}  // namespace
+
+int clr_interp_scalar(clr_ctx* ctx, int flag, const float* in, float* out) {
+    return guard([&] {
+        need(in, "in");
+        need(out, "out");
+        reg::CLAIRE& c = C(ctx);
+        ensure_traj(c, flag);
+        c.SL()->Interpolate(out, in, flag == CLR_STATE ? "state" : "adjoint");
+        check_hip_launch();
+    });
+}
+
+int clr_interp_vector(clr_ctx* ctx, int flag, const float* in, float* out) {
+    return guard([&] {
+        need(in, "in");
+        need(out, "out");
+        reg::CLAIRE& c = C(ctx);
+        ensure_traj(c, flag);
+        reg::VecField o(&c.Opt(), out), i(&c.Opt(), const_cast<float*>(in));
+        c.SL()->Interpolate(&o, &i, flag == CLR_STATE ? "state" : "adjoint");
+        check_hip_launch();
+    });
+}
+
+int clr_interp_points(clr_ctx* ctx, const float* f, const float* X, float* out) {
+    return guard([&] {
+        need(f, "f");
+        need(X, "X");
+        need(out, "out");
+        CLR_REQUIRE(out != f, claire_amd::kErrInvalidArgument, "interpolation output must not alias the field");
+        reg::CLAIRE& c = C(ctx);
+        reg::KernelScope k(c.Opt().prof, "gather_scalar", 20.0 * c.Opt().domain.nl);
+        claire_amd::launch_gather1(claire_amd::gather_dims(c.Opt().domain), f, X, out, c.Opt().stream);
+        check_hip_launch();
+    });
+}
+
+int clr_grad(clr_ctx* ctx, const float* f, float* g) {
+    return guard([&] {
+        need(f, "f");
+        need(g, "g");
+        reg::CLAIRE& c = C(ctx);
+        const int64_t nl = c.Opt().domain.nl;
+        float* o[3] = {g, g + nl, g + 2 * nl};
+        c.Spectral()->Grad(o, f);
+        check_hip_launch();
+    });
+}
+
+int clr_div(clr_ctx* ctx, const float* v, float* d) {
+    return guard([&] {
+        need(v, "v");
+        need(d, "d");
+        reg::CLAIRE& c = C(ctx);
+        const int64_t nl = c.Opt().domain.nl;
+        const float* in[3] = {v, v + nl, v + 2 * nl};
+        c.Spectral()->Div(d, in);
+        check_hip_launch();
+    });
+}
+
+int clr_fft_r2c(clr_ctx* ctx, const float* in, float* out_complex) {
+    return guard([&] {
+        need(in, "in");
+        need(out_complex, "out");
+        reg::CLAIRE& c = C(ctx);
+        c.Spectral()->R2C(in, reinterpret_cast<claire_amd::c32*>(out_complex));
+        check_hip_launch();
+    });
+}
+
+int clr_fft_c2r(clr_ctx* ctx, const float* in_complex, float* out) {
+    return guard([&] {
+        need(in_complex, "in");
+        need(out, "out");
+        reg::CLAIRE& c = C(ctx);
+        const int64_t nlc = c.Opt().domain.nlc;
+        ctx->spec.resize(static_cast<size_t>(nlc));
+        CLR_HIP_CHECK(hipMemcpyAsync(ctx->spec.get(), in_complex, nlc * sizeof(claire_amd::c32), hipMemcpyDeviceToDevice,
+                                     c.Opt().stream));
+        c.Spectral()->C2R(ctx->spec.get(), out);
+        check_hip_launch();
+    });
+}
+
+int clr_dot(clr_ctx* ctx, const float* a, const float* b, int64_t n, double* out) {
+    return guard([&] {
+        need(a, "a");
+        need(b, "b");
+        need(out, "out");
+        *out = C(ctx).Red()->Dot(a, b, n);
+    });
+}
+
+int clr_norm2(clr_ctx* ctx, const float* a, int64_t n, double* out) {
+    return guard([&] {
+        need(a, "a");
+        need(out, "out");
+        const double s = C(ctx).Red()->Dot(a, a, n);
+        *out = s > 0 ? std::sqrt(s) : 0.0;
+    });
+}
+
+int clr_profile_enable(clr_ctx* ctx, int on) {
+    return guard([&] {
+        need(ctx, "context");
+        if (!on) ctx->prof->collect();
+        ctx->prof->enable(on != 0);
+    });
+}
+
+int clr_profile_reset(clr_ctx* ctx) {
+    return guard([&] {
+        need(ctx, "context");
+        ctx->prof->reset();
+    });
+}
+
+int clr_profile_get(clr_ctx* ctx, int i, char* name, int name_len, double* ms, long* count, double* bytes) {
+    return guard([&] {
+        need(ctx, "context");
+        ctx->prof->collect();
+        const auto& e = ctx->prof->entries();
+        CLR_REQUIRE(i >= 0 && i < static_cast<int>(e.size()), claire_amd::kErrInvalidArgument, "no such profile entry");
+        auto it = e.begin();
+        std::advance(it, i);
+        if (name && name_len > 0) {
+            std::strncpy(name, it->first.c_str(), static_cast<size_t>(name_len - 1));
+            name[name_len - 1] = '\0';
+        }
+        if (ms) *ms = it->second.ms;
+        if (count) *count = it->second.count;
+        if (bytes) *bytes = it->second.bytes;
+    });
+}
+
+}  // extern "C"

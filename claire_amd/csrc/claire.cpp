@@ -1,0 +1,771 @@
+// claire.cpp -- host side of the MI355X registration hot path.
+#include "claire.hpp"
+
+#include <cstring>
+
+namespace claire_amd {
+
+Dims make_dims(const int nx[3]) {
+    Dims d{};
+    for (int a = 0; a < 3; ++a) {
+        CLR_REQUIRE(nx[a] >= 4, kErrInvalidArgument, "grid size must be >= 4 per axis");
+        d.n[a] = nx[a];
+        // hx = 2π/n with the division in double, stored in FP32 (src/RegOpt.cpp:935)
+        d.hx[a] = static_cast<float>(3.14159265358979323846 * 2.0 / static_cast<double>(static_cast<float>(nx[a])));
+    }
+    d.nh = nx[2] / 2 + 1;
+    d.nl = static_cast<int64_t>(nx[0]) * nx[1] * nx[2];
+    d.nlc = static_cast<int64_t>(nx[0]) * nx[1] * d.nh;
+    d.hd = d.hx[0] * d.hx[1] * d.hx[2];
+    float s = 1.0f;
+    for (int a = 0; a < 3; ++a) s *= static_cast<float>(nx[a]);
+    d.scale = static_cast<float>(1.0 / static_cast<double>(s));
+    return d;
+}
+
+}  // namespace claire_amd
+
+namespace reg {
+
+using namespace claire_amd;
+
+// ---------------------------------------------------------------------------
+// Profiler
+// ---------------------------------------------------------------------------
+Profiler::~Profiler() {
+    for (auto& p : pending_) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool_) (void)hipEventDestroy(e);
+}
+
+void Profiler::enable(bool on) { enabled_ = on; }
+
+hipEvent_t Profiler::get_event() {
+    if (!pool_.empty()) {
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    CLR_HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+void Profiler::begin(const char* name, double bytes) {
+    Pending p{name, bytes, get_event(), get_event()};
+    CLR_HIP_CHECK(hipEventRecord(p.a, stream_));
+    pending_.push_back(p);
+}
+
+void Profiler::end() { CLR_HIP_CHECK(hipEventRecord(pending_.back().b, stream_)); }
+
+void Profiler::collect() {
+    CLR_HIP_CHECK(hipStreamSynchronize(stream_));
+    for (auto& p : pending_) {
+        float ms = 0.f;
+        CLR_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+        Entry& e = totals_[p.name];
+        e.ms += ms;
+        e.bytes += p.bytes;
+        e.count += 1;
+        pool_.push_back(p.a);
+        pool_.push_back(p.b);
+    }
+    pending_.clear();
+}
+
+void Profiler::reset() {
+    collect();
+    totals_.clear();
+}
+
+// ---------------------------------------------------------------------------
+// Reducer (FP64)
+// ---------------------------------------------------------------------------
+Reducer::Reducer(const RegOpt* opt) : m_Opt(opt), m_Partial(kReduceBlocks), m_Result(1) {}
+
+double Reducer::fetch() {
+    double h = 0.0;
+    CLR_HIP_CHECK(hipMemcpyAsync(&h, m_Result.get(), sizeof(double), hipMemcpyDeviceToHost, m_Opt->stream));
+    CLR_HIP_CHECK(hipStreamSynchronize(m_Opt->stream));
+    return h;
+}
+
+double Reducer::Dot(const float* a, const float* b, IntType n) {
+    {
+        KernelScope k(m_Opt->prof, "reduce_dot", 8.0 * n);
+        launch_dot(a, b, n, m_Partial.get(), m_Result.get(), m_Opt->stream);
+    }
+    return fetch();
+}
+
+double Reducer::AbsMax(const float* a, IntType n) {
+    {
+        KernelScope k(m_Opt->prof, "reduce_absmax", 4.0 * n);
+        launch_absmax(a, n, m_Partial.get(), m_Result.get(), m_Opt->stream);
+    }
+    return fetch();
+}
+
+double Reducer::SqDist(const float* a, const float* b, IntType n) {
+    {
+        KernelScope k(m_Opt->prof, "reduce_sqdist", 8.0 * n);
+        launch_sqdist(a, b, n, m_Partial.get(), m_Result.get(), m_Opt->stream);
+    }
+    return fetch();
+}
+
+// ---------------------------------------------------------------------------
+// VecField (src/VecField.cpp)
+// ---------------------------------------------------------------------------
+VecField::VecField(const RegOpt* opt) : m_Opt(opt), m_Storage(3 * static_cast<size_t>(opt->domain.nl)) {
+    const int64_t nl = opt->domain.nl;
+    m_X1 = m_Storage.get();
+    m_X2 = m_X1 + nl;
+    m_X3 = m_X2 + nl;
+}
+
+VecField::VecField(const RegOpt* opt, float* flat) : m_Opt(opt) {
+    CLR_REQUIRE(flat != nullptr, kErrInvalidArgument, "null vector field");
+    const int64_t nl = opt->domain.nl;
+    m_X1 = flat;
+    m_X2 = flat + nl;
+    m_X3 = flat + 2 * nl;
+}
+
+int VecField::SetComponents(const float* flat) {
+    CLR_REQUIRE(flat != nullptr, kErrInvalidArgument, "null input vector");
+    if (flat != m_X1)
+        CLR_HIP_CHECK(hipMemcpyAsync(m_X1, flat, sizeof(float) * 3 * m_Opt->domain.nl, hipMemcpyDeviceToDevice,
+                                     m_Opt->stream));
+    return 0;
+}
+
+int VecField::GetComponents(float* flat) const {
+    CLR_REQUIRE(flat != nullptr, kErrInvalidArgument, "null output vector");
+    if (flat != m_X1)
+        CLR_HIP_CHECK(hipMemcpyAsync(flat, m_X1, sizeof(float) * 3 * m_Opt->domain.nl, hipMemcpyDeviceToDevice,
+                                     m_Opt->stream));
+    return 0;
+}
+
+int VecField::SetValue(ScalarType v) {
+    launch_fill(m_X1, v, 3 * m_Opt->domain.nl, m_Opt->stream);
+    return 0;
+}
+
+int VecField::Copy(const VecField* x) { return SetComponents(x->m_X1); }
+
+int VecField::Scale(ScalarType a) {
+    launch_scale(m_X1, a, 3 * m_Opt->domain.nl, m_Opt->stream);
+    return 0;
+}
+
+int VecField::AXPY(ScalarType a, const VecField* x) {
+    launch_axpy(a, x->m_X1, m_X1, 3 * m_Opt->domain.nl, m_Opt->stream);
+    return 0;
+}
+
+int VecField::WAXPY(ScalarType a, const VecField* x, const VecField* y) {
+    launch_waxpy(a, x->m_X1, y->m_X1, m_X1, 3 * m_Opt->domain.nl, m_Opt->stream);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// SpectralOps (AccFFT replacement)
+// ---------------------------------------------------------------------------
+SpectralOps::SpectralOps(const RegOpt* opt) : m_Opt(opt), m_Plan(new FFTPlan3D(opt->domain)) {
+    for (auto& s : m_Spec) s.resize(static_cast<size_t>(opt->domain.nlc));
+}
+
+namespace {
+double bytes_x3_r2c(const Dims& d) { return 4.0 * d.nl + 8.0 * d.nlc; }
+double bytes_col(const Dims& d) { return 16.0 * d.nlc; }
+double bytes_x3_c2r(const Dims& d, bool add) { return 8.0 * d.nlc + 4.0 * d.nl + (add ? 4.0 * d.nl : 0.0); }
+}  // namespace
+
+void SpectralOps::R2C(const float* in, c32* out) {
+    const Dims& d = m_Opt->domain;
+    hipStream_t st = m_Opt->stream;
+    { KernelScope k(m_Opt->prof, "fft_x3_r2c", bytes_x3_r2c(d)); m_Plan->x3_r2c(in, out, st); }
+    { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(out, -1, st); }
+    { KernelScope k(m_Opt->prof, "fft_x1", bytes_col(d)); m_Plan->x1(out, -1, st); }
+}
+
+void SpectralOps::C2R(c32* in, float* out, float alpha, const float* add, float beta) {
+    const Dims& d = m_Opt->domain;
+    hipStream_t st = m_Opt->stream;
+    { KernelScope k(m_Opt->prof, "fft_x1", bytes_col(d)); m_Plan->x1(in, +1, st); }
+    { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(in, +1, st); }
+    { KernelScope k(m_Opt->prof, "fft_x3_c2r", bytes_x3_c2r(d, add)); m_Plan->x3_c2r(in, out, st, alpha, add, beta); }
+}
+
+void SpectralOps::Grad(float* const g[3], const float* f) {
+    const Dims& d = m_Opt->domain;
+    hipStream_t st = m_Opt->stream;
+    c32* s[3] = {m_Spec[0].get(), m_Spec[1].get(), m_Spec[2].get()};
+    { KernelScope k(m_Opt->prof, "fft_x3_r2c", bytes_x3_r2c(d)); m_Plan->x3_r2c(f, s[0], st); }
+    { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[0], -1, st); }
+    { KernelScope k(m_Opt->prof, "fft_x1_grad", 32.0 * d.nlc); m_Plan->x1_grad(s[0], s, st); }
+    for (int c = 0; c < 3; ++c) {
+        { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[c], +1, st); }
+        { KernelScope k(m_Opt->prof, "fft_x3_c2r", bytes_x3_c2r(d, false)); m_Plan->x3_c2r(s[c], g[c], st); }
+    }
+}
+
+void SpectralOps::Div(float* out, const float* const v[3]) {
+    const Dims& d = m_Opt->domain;
+    hipStream_t st = m_Opt->stream;
+    c32* s[3] = {m_Spec[0].get(), m_Spec[1].get(), m_Spec[2].get()};
+    for (int c = 0; c < 3; ++c) {
+        { KernelScope k(m_Opt->prof, "fft_x3_r2c", bytes_x3_r2c(d)); m_Plan->x3_r2c(v[c], s[c], st); }
+        { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[c], -1, st); }
+    }
+    { KernelScope k(m_Opt->prof, "fft_x1_div", 32.0 * d.nlc); m_Plan->x1_div(s, s[0], st); }
+    { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[0], +1, st); }
+    { KernelScope k(m_Opt->prof, "fft_x3_c2r", bytes_x3_c2r(d, false)); m_Plan->x3_c2r(s[0], out, st); }
+}
+
+void SpectralOps::Multiplier3(const SpecParams& sp, const float* const in[3], float* const out[3], float alpha,
+                              const float* const add[3], float beta) {
+    const Dims& d = m_Opt->domain;
+    hipStream_t st = m_Opt->stream;
+    c32* s[3] = {m_Spec[0].get(), m_Spec[1].get(), m_Spec[2].get()};
+    for (int c = 0; c < 3; ++c) {
+        { KernelScope k(m_Opt->prof, "fft_x3_r2c", bytes_x3_r2c(d)); m_Plan->x3_r2c(in[c], s[c], st); }
+        { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[c], -1, st); }
+    }
+    { KernelScope k(m_Opt->prof, "fft_x1_mult", 3.0 * bytes_col(d)); m_Plan->x1_multiplier(s, s, 3, sp, st); }
+    for (int c = 0; c < 3; ++c) {
+        const float* a = add ? add[c] : nullptr;
+        { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[c], +1, st); }
+        { KernelScope k(m_Opt->prof, "fft_x3_c2r", bytes_x3_c2r(d, a != nullptr)); m_Plan->x3_c2r(s[c], out[c], st, alpha, a, beta); }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Regularization (src/Regularization*.cpp)
+// ---------------------------------------------------------------------------
+SpecParams Regularization::Params(int op) const {
+    SpecParams sp{};
+    sp.op = op;
+    sp.norm = Norm();
+    sp.beta0 = m_Opt->beta[0];
+    sp.beta1 = m_Opt->beta[1];
+    sp.hd = m_Opt->GetLebesgueMeasure();
+    sp.scale = m_Opt->ComputeFFTScale();
+    return sp;
+}
+
+int Regularization::EvaluateGradient(VecField* dvR, const VecField* v) {
+    if (m_Opt->beta[0] == 0.0f) return dvR->SetValue(0.0f);  // src/RegularizationH1.cpp:188-190
+    const float* in[3] = {v->m_X1, v->m_X2, v->m_X3};
+    float* out[3] = {dvR->m_X1, dvR->m_X2, dvR->m_X3};
+    m_Spec->Multiplier3(Params(kOpRegApply), in, out, 1.0f, nullptr, 0.0f);
+    return 0;
+}
+
+void Regularization::ApplyFused(VecField* out, const VecField* v, float alpha, const VecField* add, float beta) {
+    if (m_Opt->beta[0] == 0.0f) {
+        if (add) {
+            launch_copy(add->m_X1, out->m_X1, 3 * m_Opt->domain.nl, m_Opt->stream);
+            if (beta != 1.0f) launch_scale(out->m_X1, beta, 3 * m_Opt->domain.nl, m_Opt->stream);
+        } else {
+            out->SetValue(0.0f);
+        }
+        return;
+    }
+    const float* in[3] = {v->m_X1, v->m_X2, v->m_X3};
+    float* o[3] = {out->m_X1, out->m_X2, out->m_X3};
+    const float* a[3] = {add ? add->m_X1 : nullptr, add ? add->m_X2 : nullptr, add ? add->m_X3 : nullptr};
+    m_Spec->Multiplier3(Params(kOpRegApply), in, o, alpha, add ? a : nullptr, beta);
+}
+
+int Regularization::ApplyInverse(VecField* Ainvx, const VecField* x, bool applysqrt) {
+    if (m_Opt->beta[0] == 0.0f) return Ainvx->Copy(x);  // src/RegularizationH1.cpp:336-340
+    const float* in[3] = {x->m_X1, x->m_X2, x->m_X3};
+    float* out[3] = {Ainvx->m_X1, Ainvx->m_X2, Ainvx->m_X3};
+    m_Spec->Multiplier3(Params(applysqrt ? kOpRegInvSqrt : kOpRegInverse), in, out, 1.0f, nullptr, 0.0f);
+    return 0;
+}
+
+int Regularization::EvaluateFunctional(double* R, const VecField* v) {
+    *R = 0.0;
+    if (m_Opt->beta[0] == 0.0f) return 0;
+    const int64_t nl = m_Opt->domain.nl;
+    const double hd = m_Opt->GetLebesgueMeasure();
+    VecField work(m_Opt);
+    float* w[3] = {work.m_X1, work.m_X2, work.m_X3};
+    const int norm = Norm();
+    double acc = 0.0;
+    if (norm == kRegH1 || norm == kRegH1SN) {
+        // Σ_i ‖∇v_i‖² (src/RegularizationH1SN.cpp:84-141)
+        for (int c = 0; c < 3; ++c) {
+            const float* vc = c == 0 ? v->m_X1 : (c == 1 ? v->m_X2 : v->m_X3);
+            m_Spec->Grad(w, vc);
+            for (int k = 0; k < 3; ++k) acc += m_Red->Dot(w[k], w[k], nl);
+        }
+        if (norm == kRegH1) {
+            // ½·hd·β0·(H1v + β1·L2v), src/RegularizationH1.cpp:140-147
+            const double l2 = m_Red->Dot(v->m_X1, v->m_X1, 3 * nl);
+            *R = 0.5 * hd * m_Opt->beta[0] * (acc + m_Opt->beta[1] * l2);
+        } else {
+            *R = 0.5 * hd * m_Opt->beta[0] * acc;
+        }
+        return 0;
+    }
+    const float* in[3] = {v->m_X1, v->m_X2, v->m_X3};
+    SpecParams sp = Params(norm == kRegH2 ? kOpH2Functional : kOpScaledLaplace);
+    if (norm == kRegH2) {
+        // src/RegularizationH2.cpp:80-81,128: multiplier scale·√β0·(lapik + √β1)
+        sp.beta0 = std::sqrt(m_Opt->beta[0]);
+        sp.beta1 = std::sqrt(m_Opt->beta[1]);
+    }
+    m_Spec->Multiplier3(sp, in, w, 1.0f, nullptr, 0.0f);
+    for (int k = 0; k < 3; ++k) acc += m_Red->Dot(w[k], w[k], nl);
+    if (norm == kRegH2)
+        *R = 0.5 * hd * acc;                         // src/RegularizationH2.cpp:164-166
+    else
+        *R = 0.5 * m_Opt->beta[0] * hd * acc;        // src/RegularizationH2SN.cpp:163
+    return 0;
+}
+
+int Regularization::GetExtremeEigValsInvOp(ScalarType& emin, ScalarType& emax) {
+    float w[3];
+    for (int i = 0; i < 3; ++i) w[i] = static_cast<float>(m_Opt->domain.n[i]) / 2.0f;
+    float regop = -(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    const float b1 = m_Opt->beta[0], b2 = m_Opt->beta[1];
+    switch (Norm()) {
+        case kRegH1:  // src/RegularizationH1.cpp:440-448
+            regop = b1 * (-regop + b2);
+            emin = 1.0f / regop;
+            emax = 1.0f / b2;
+            break;
+        case kRegH2:  // src/RegularizationH2.cpp:439-446
+            regop = b1 * (regop * regop) + b2;
+            emin = 1.0f / regop;
+            emax = 1.0f / b2;
+            break;
+        case kRegH1SN:  // src/RegularizationH1SN.cpp:418-424
+            regop = -b1 * regop + b2;
+            emin = 1.0f / regop;
+            emax = 1.0f / b2;
+            break;
+        default:  // src/RegularizationH2SN.cpp:447-455
+            regop = b1 * (regop * regop);
+            emin = 1.0f / regop;
+            emax = 1.0f / b1;
+            break;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// SemiLagrangian (src/SemiLagrangian.cpp)
+// ---------------------------------------------------------------------------
+SemiLagrangian::SemiLagrangian(const RegOpt* opt) : m_Opt(opt), m_Scratch(1) {}
+
+int SemiLagrangian::Which(const std::string& flag) const {
+    if (flag == "state") return 0;
+    if (flag == "adjoint") return 1;
+    throw Error(kErrInvalidArgument, "flag wrong");  // src/SemiLagrangian.cpp:221-223
+}
+
+int SemiLagrangian::ComputeTrajectory(const VecField* v, const std::string& flag) {
+    const int w = Which(flag);
+    m_X[w].resize(3 * static_cast<size_t>(m_Opt->domain.nl));
+    const float* vc[3] = {v->m_X1, v->m_X2, v->m_X3};
+    {
+        // bytes: v read (12) + v gathered (12) + X written (12) per point
+        KernelScope k(m_Opt->prof, "gather_traj_rk2", 36.0 * m_Opt->domain.nl);
+        launch_trajectory_rk2(m_Opt->domain, vc, w == 0 ? 1.0f : -1.0f, m_Opt->GetTimeStepSize(), m_X[w].get(),
+                              m_Opt->stream);
+    }
+    m_D[w] = max_displacement(gather_dims(m_Opt->domain), m_X[w].get(), 3, m_Scratch.get(), m_Opt->stream);
+    m_Valid[w] = true;
+    return 0;
+}
+
+bool SemiLagrangian::HasTrajectory(const std::string& flag) const { return m_Valid[Which(flag)]; }
+
+const float* SemiLagrangian::Trajectory(const std::string& flag) const {
+    const int w = Which(flag);
+    CLR_REQUIRE(m_Valid[w], kErrState, "trajectory '" + flag + "' has not been computed");
+    return m_X[w].get();
+}
+
+void SemiLagrangian::Gather(float* xo, const float* xi, const std::string& flag) {
+    const float* X = Trajectory(flag);
+    const GatherDims g = gather_dims(m_Opt->domain);
+    KernelScope k(m_Opt->prof, "gather_scalar", 20.0 * g.nl);
+    if (UseLds(flag)) {
+        const float* in[1] = {xi};
+        float* out[1] = {xo};
+        launch_lds_gather(DisplacementBound(flag), g, in, 1, X, out, m_Opt->stream);
+    } else {
+        launch_gather1(g, xi, X, xo, m_Opt->stream);
+    }
+}
+
+void SemiLagrangian::Gather3(const float* const in[3], float* const out[3], const std::string& flag) {
+    const float* X = Trajectory(flag);
+    const GatherDims g = gather_dims(m_Opt->domain);
+    KernelScope k(m_Opt->prof, "gather_vector", 36.0 * g.nl);
+    const bool spaced = (in[1] - in[0] == in[2] - in[1]) && (out[1] - out[0] == out[2] - out[1]);
+    if (UseLds(flag) && spaced)
+        launch_lds_gather(DisplacementBound(flag), g, in, 3, X, out, m_Opt->stream);
+    else
+        launch_gather3(g, in, X, out, m_Opt->stream);
+}
+
+int SemiLagrangian::Interpolate(float* xo, const float* xi, const std::string& flag) {
+    const GatherDims g = gather_dims(m_Opt->domain);
+    if (xo == xi) {
+        DeviceArray<float> tmp(static_cast<size_t>(g.nl));
+        Gather(tmp.get(), xi, flag);
+        launch_copy(tmp.get(), xo, g.nl, m_Opt->stream);
+        CLR_HIP_CHECK(hipStreamSynchronize(m_Opt->stream));
+        return 0;
+    }
+    Gather(xo, xi, flag);
+    return 0;
+}
+
+int SemiLagrangian::Interpolate(VecField* vo, const VecField* vi, const std::string& flag) {
+    const float* in[3] = {vi->m_X1, vi->m_X2, vi->m_X3};
+    if (vo->m_X1 == vi->m_X1) {
+        VecField tmp(m_Opt);
+        float* out[3] = {tmp.m_X1, tmp.m_X2, tmp.m_X3};
+        Gather3(in, out, flag);
+        vo->Copy(&tmp);
+        CLR_HIP_CHECK(hipStreamSynchronize(m_Opt->stream));
+        return 0;
+    }
+    float* out[3] = {vo->m_X1, vo->m_X2, vo->m_X3};
+    Gather3(in, out, flag);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// CLAIRE (src/CLAIRE.cpp, GN + semi-Lagrangian + nc = 1)
+// ---------------------------------------------------------------------------
+CLAIRE::CLAIRE(const RegOpt& opt) : m_Opt(opt) {
+    CLR_REQUIRE(m_Opt.nt >= 1, kErrInvalidArgument, "nt must be >= 1");
+    CLR_REQUIRE(m_Opt.nc == 1, kErrInvalidArgument, "only nc = 1 is supported");
+    const size_t nl = static_cast<size_t>(m_Opt.domain.nl);
+    const size_t nt = static_cast<size_t>(m_Opt.nt);
+    m_Red.reset(new Reducer(&m_Opt));
+    m_Spectral.reset(new SpectralOps(&m_Opt));
+    switch (m_Opt.regnorm) {  // CLAIREBase::SetupRegularization, src/CLAIREBase.cpp:701-790
+        case kRegH1: m_Regularization.reset(new RegularizationH1(&m_Opt, m_Spectral.get(), m_Red.get())); break;
+        case kRegH2: m_Regularization.reset(new RegularizationH2(&m_Opt, m_Spectral.get(), m_Red.get())); break;
+        case kRegH1SN: m_Regularization.reset(new RegularizationH1SN(&m_Opt, m_Spectral.get(), m_Red.get())); break;
+        case kRegH2SN: m_Regularization.reset(new RegularizationH2SN(&m_Opt, m_Spectral.get(), m_Red.get())); break;
+        default: throw Error(kErrInvalidArgument, "regularization norm not implemented");
+    }
+    m_SL.reset(new SemiLagrangian(&m_Opt));
+    m_VelocityField.reset(new VecField(&m_Opt));
+    m_IncVelocityField.reset(new VecField(&m_Opt));
+    m_BodyForce.reset(new VecField(&m_Opt));
+    m_IncBodyForce.reset(new VecField(&m_Opt));
+    m_WorkVecField1.reset(new VecField(&m_Opt));
+    m_TemplateImage.resize(nl);
+    m_ReferenceImage.resize(nl);
+    m_State.resize((nt + 1) * nl);
+    m_Adjoint.resize(nl);
+    m_IncState[0].resize(nl);
+    m_IncState[1].resize(nl);
+    m_IncAdjointT0.resize(nl);
+    m_X4.resize(4 * nl);
+    m_GradM.resize((nt + 1) * 3 * nl);
+    m_GradMX.resize(nt * 3 * nl);
+    m_S.resize(nt * nl);
+    m_Lam.resize(nt * nl);
+    m_Scalar.resize(nl);
+    m_VtX.resize(3 * nl);
+}
+
+CLAIRE::~CLAIRE() = default;
+
+size_t CLAIRE::DeviceBytes() const {
+    const size_t nl = static_cast<size_t>(m_Opt.domain.nl), nt = static_cast<size_t>(m_Opt.nt);
+    const size_t floats = 5 * 3 * nl + 2 * nl + (nt + 1) * nl + nl + 3 * nl + 4 * nl + (nt + 1) * 3 * nl +
+                          nt * 3 * nl + nt * nl + nt * nl + nl + 2 * 3 * nl;
+    return floats * 4 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
+}
+
+void CLAIRE::RequireVelocity() const { CLR_REQUIRE(m_HaveVelocity, kErrState, "velocity field not set"); }
+
+int CLAIRE::SetControlVariable(const float* v) {
+    m_VelocityField->SetComponents(v);
+    m_HaveVelocity = true;
+    m_TrajValid = m_StateValid = m_GradMValid = m_GradMXValid = false;
+    // CLAIREBase::IsVelocityZero (src/CLAIREBase.cpp:578-605): ∞-norm == 0
+    m_VelocityIsZero = m_Red->AbsMax(m_VelocityField->m_X1, 3 * m_Opt.domain.nl) == 0.0;
+    return 0;
+}
+
+int CLAIRE::SetIncControlVariable(const float* vtilde) {
+    m_IncVelocityField->SetComponents(vtilde);
+    m_HaveIncVelocity = true;
+    return 0;
+}
+
+int CLAIRE::SetTemplateImage(const float* mT) {
+    CLR_REQUIRE(mT != nullptr, kErrInvalidArgument, "null template image");
+    CLR_HIP_CHECK(hipMemcpyAsync(m_TemplateImage.get(), mT, sizeof(float) * m_Opt.domain.nl, hipMemcpyDeviceToDevice,
+                                 m_Opt.stream));
+    m_HaveTemplate = true;
+    m_StateValid = m_GradMValid = m_GradMXValid = false;
+    return 0;
+}
+
+int CLAIRE::SetReferenceImage(const float* mR) {
+    CLR_REQUIRE(mR != nullptr, kErrInvalidArgument, "null reference image");
+    CLR_HIP_CHECK(hipMemcpyAsync(m_ReferenceImage.get(), mR, sizeof(float) * m_Opt.domain.nl, hipMemcpyDeviceToDevice,
+                                 m_Opt.stream));
+    m_HaveReference = true;
+    return 0;
+}
+
+void CLAIRE::EnsureTrajectories() {
+    RequireVelocity();
+    if (m_TrajValid) return;
+    m_SL->ComputeTrajectory(m_VelocityField.get(), "state");
+    m_SL->ComputeTrajectory(m_VelocityField.get(), "adjoint");
+    // div v and the adjoint update coefficient at every point
+    // (src/CLAIRE.cpp:2614-2635 / 3907-3927 compute ∇·v and (∇·v)(X) per solve)
+    const float* vc[3] = {m_VelocityField->m_X1, m_VelocityField->m_X2, m_VelocityField->m_X3};
+    m_Spectral->Div(m_Scalar.get(), vc);
+    {
+        KernelScope k(m_Opt.prof, "gather_adjoint_coeff", 36.0 * m_Opt.domain.nl);
+        launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Trajectory("adjoint"), m_Scalar.get(),
+                             m_Opt.GetTimeStepSize(), m_X4.get(), m_Opt.stream);
+    }
+    m_TrajValid = true;
+}
+
+int CLAIRE::SolveStateEquation() {
+    RequireVelocity();
+    CLR_REQUIRE(m_HaveTemplate, kErrState, "template image not set");
+    const int64_t nl = m_Opt.domain.nl;
+    const int nt = m_Opt.nt;
+    EnsureTrajectories();
+    float* m = m_State.get();
+    launch_copy(m_TemplateImage.get(), m, nl, m_Opt.stream);  // m^0 = mT (src/CLAIRE.cpp:1896)
+    if (m_VelocityIsZero) {
+        for (int j = 1; j <= nt; ++j) launch_copy(m, m + j * nl, nl, m_Opt.stream);  // src/CLAIRE.cpp:1909-1920
+    } else {
+        for (int j = 0; j < nt; ++j)  // src/CLAIRE.cpp:2153-2163
+            m_SL->Gather(m + (j + 1) * nl, m + j * nl, "state");
+    }
+    m_StateValid = true;
+    m_GradMValid = m_GradMXValid = false;
+    return 0;
+}
+
+void CLAIRE::EnsureStateGradients() {
+    if (!m_StateValid) SolveStateEquation();
+    if (m_GradMValid) return;
+    const int64_t nl = m_Opt.domain.nl;
+    for (int j = 0; j <= m_Opt.nt; ++j) {
+        float* gj = m_GradM.get() + 3 * j * nl;
+        float* g[3] = {gj, gj + nl, gj + 2 * nl};
+        m_Spectral->Grad(g, m_State.get() + j * nl);
+    }
+    m_GradMValid = true;
+    m_GradMXValid = false;
+}
+
+void CLAIRE::EnsureGradientsAtStateTrajectory() {
+    EnsureStateGradients();
+    EnsureTrajectories();
+    if (m_GradMXValid) return;
+    const int64_t nl = m_Opt.domain.nl;
+    for (int j = 0; j < m_Opt.nt; ++j) {  // (∇m^j)(X_s), src/CLAIRE.cpp:3243-3247
+        const float* gj = m_GradM.get() + 3 * j * nl;
+        float* gx = m_GradMX.get() + 3 * j * nl;
+        const float* in[3] = {gj, gj + nl, gj + 2 * nl};
+        float* out[3] = {gx, gx + nl, gx + 2 * nl};
+        m_SL->Gather3(in, out, "state");
+    }
+    m_GradMXValid = true;
+}
+
+int CLAIRE::PrepareHessian() {
+    EnsureTrajectories();
+    EnsureGradientsAtStateTrajectory();
+    CLR_HIP_CHECK(hipStreamSynchronize(m_Opt.stream));
+    return 0;
+}
+
+// λ^0 given in m_Lam slot 0; writes b (×hd) and, when lam_out is set, λ at t = 0.
+void CLAIRE::IncAdjointCore(const float* lt0, VecField* bt) {
+    const int64_t nl = m_Opt.domain.nl;
+    const int nt = m_Opt.nt;
+    const GatherDims g = gather_dims(m_Opt.domain);
+    float* b[3] = {bt->m_X1, bt->m_X2, bt->m_X3};
+    (void)lt0;
+    if (m_VelocityIsZero) {
+        KernelScope k(m_Opt.prof, "body_force_static", 20.0 * nl);
+        launch_body_force_static(nl, m_Opt.GetLebesgueMeasure(), m_Lam.get(), m_GradM.get(), b, m_Opt.stream);
+        return;
+    }
+    const float* X4 = m_X4.get();
+    const bool lds = m_SL->UseLds("adjoint");
+    const int db = m_SL->DisplacementBound("adjoint");
+    for (int j = 0; j + 1 < nt; ++j) {
+        KernelScope k(m_Opt.prof, "adjoint_step", 24.0 * nl);
+        if (lds)
+            launch_lds_adjoint_step(db, g, X4, m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl, m_Opt.stream);
+        else
+            launch_adjoint_step(g, X4, m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl, m_Opt.stream);
+    }
+    // bytes: X4 16 + gathered λ 4 + λ^0..λ^{nt−1} 4nt + ∇m 12(nt+1) + b 12 + λ(t=0) 4
+    KernelScope k(m_Opt.prof, "adjoint_final", (16.0 + 4.0 + 4.0 * nt + 12.0 * (nt + 1) + 12.0 + 4.0) * nl);
+    if (lds)
+        launch_lds_adjoint_final(db, g, nt, m_Opt.GetTimeStepSize(), m_Opt.GetLebesgueMeasure(), X4, m_Lam.get(),
+                                 m_GradM.get(), b, m_Scalar.get(), m_Opt.stream);
+    else
+        launch_adjoint_final(g, nt, m_Opt.GetTimeStepSize(), m_Opt.GetLebesgueMeasure(), X4, m_Lam.get(),
+                             m_GradM.get(), b, m_Scalar.get(), m_Opt.stream);
+}
+
+// Incremental state: writes λ̃^0 = −m̃^nt into out (or m̃^nt when !negate_last).
+void CLAIRE::IncStateCore(const VecField* vt, float* out) {
+    const int64_t nl = m_Opt.domain.nl;
+    const int nt = m_Opt.nt;
+    EnsureGradientsAtStateTrajectory();
+    const GatherDims g = gather_dims(m_Opt.domain);
+    const float* Xs = m_SL->Trajectory("state");
+    const float* v[3] = {vt->m_X1, vt->m_X2, vt->m_X3};
+    float* first = (nt == 1) ? out : m_IncState[0].get();
+    float* vx[3] = {m_VtX.get(), m_VtX.get() + nl, m_VtX.get() + 2 * nl};
+    // ṽ(X_s) (src/CLAIRE.cpp:3221)
+    m_SL->Gather3(v, vx, "state");
+    {
+        // bytes: ṽX 12 + ṽ 12 + (∇m^j)(X_s) 12nt + ∇m^{j+1} 12nt + s_j 4nt
+        KernelScope k(m_Opt.prof, "inc_state_rhs", (24.0 + 28.0 * nt) * nl);
+        launch_inc_state_rhs(nl, nt, m_Opt.GetTimeStepSize(), vx, v, m_GradMX.get(), m_GradM.get(), m_S.get(), first,
+                             m_Opt.stream);
+    }
+    const bool lds = m_SL->UseLds("state");
+    const int db = m_SL->DisplacementBound("state");
+    for (int j = 1; j < nt; ++j) {
+        const float* src = m_IncState[(j - 1) % 2].get();
+        const bool last = (j == nt - 1);
+        float* dst = last ? out : m_IncState[j % 2].get();
+        // bytes: X 12 + gathered m̃ 4 + s_j 4 + out 4
+        KernelScope k(m_Opt.prof, "inc_state_step", 24.0 * nl);
+        if (lds)
+            launch_lds_inc_state_step(db, g, Xs, src, m_S.get() + j * nl, dst, last ? 1 : 0, m_Opt.stream);
+        else
+            launch_inc_state_step(g, Xs, src, m_S.get() + j * nl, dst, last ? 1 : 0, m_Opt.stream);
+    }
+}
+
+int CLAIRE::SolveIncStateEquation() {
+    CLR_REQUIRE(m_HaveIncVelocity, kErrState, "incremental velocity not set");
+    if (!m_StateValid) SolveStateEquation();
+    // λ̃⁰ = −m̃^nt lands in m_Lam; m̃^nt = −λ̃⁰ is kept for the API.
+    IncStateCore(m_IncVelocityField.get(), m_Lam.get());
+    const int64_t nl = m_Opt.domain.nl;
+    float* mt = m_IncState[0].get();
+    launch_copy(m_Lam.get(), mt, nl, m_Opt.stream);
+    launch_scale(mt, -1.0f, nl, m_Opt.stream);
+    m_IncStateSlot = 0;
+    return 0;
+}
+
+int CLAIRE::SolveIncAdjointFrom(const float* mtilde) {
+    RequireVelocity();
+    if (!m_StateValid) SolveStateEquation();
+    EnsureGradientsAtStateTrajectory();
+    const int64_t nl = m_Opt.domain.nl;
+    // λ̃⁰ = −m̃ (src/DistanceMeasureSL2.cpp:266-272)
+    launch_copy(mtilde, m_Lam.get(), nl, m_Opt.stream);
+    launch_scale(m_Lam.get(), -1.0f, nl, m_Opt.stream);
+    IncAdjointCore(m_Lam.get(), m_IncBodyForce.get());
+    if (m_VelocityIsZero)
+        launch_copy(m_Lam.get(), m_IncAdjointT0.get(), nl, m_Opt.stream);
+    else
+        launch_copy(m_Scalar.get(), m_IncAdjointT0.get(), nl, m_Opt.stream);
+    return 0;
+}
+
+int CLAIRE::SolveIncAdjointEquation() { return SolveIncAdjointFrom(m_IncState[m_IncStateSlot].get()); }
+
+int CLAIRE::HessianMatVec(float* Hvtilde, const float* vtilde, bool scale) {
+    RequireVelocity();
+    if (!m_StateValid) SolveStateEquation();
+    const VecField* vt = m_IncVelocityField.get();
+    std::unique_ptr<VecField> view;
+    if (vtilde) {
+        view.reset(new VecField(&m_Opt, const_cast<float*>(vtilde)));
+        vt = view.get();
+    } else {
+        CLR_REQUIRE(m_HaveIncVelocity, kErrState, "incremental velocity not set");
+    }
+    CLR_REQUIRE(Hvtilde != nullptr, kErrInvalidArgument, "null output vector");
+    CLR_REQUIRE(Hvtilde != vt->m_X1, kErrInvalidArgument, "HessianMatVec cannot run in place");
+    VecField hv(&m_Opt, Hvtilde);
+    // \tilde{m} (src/CLAIRE.cpp:1345) with λ̃⁰ = −m̃^nt fused into the last step
+    IncStateCore(vt, m_Lam.get());
+    // \tilde{λ} and \tilde{b}·hd (src/CLAIRE.cpp:1348, 3307-3491)
+    IncAdjointCore(m_Lam.get(), m_IncBodyForce.get());
+    // Hṽ = β·A[ṽ] + b̃ (src/CLAIRE.cpp:1356-1361); 1/hd when !scale (src/CLAIRE.cpp:1283-1286)
+    const float a = scale ? 1.0f : 1.0f / m_Opt.GetLebesgueMeasure();
+    m_Regularization->ApplyFused(&hv, vt, a, m_IncBodyForce.get(), a);
+    return 0;
+}
+
+int CLAIRE::SolveAdjointEquation() {
+    RequireVelocity();
+    CLR_REQUIRE(m_HaveReference, kErrState, "reference image not set");
+    if (!m_StateValid) SolveStateEquation();
+    EnsureStateGradients();
+    const int64_t nl = m_Opt.domain.nl;
+    // λ¹ = mR − m^nt (src/DistanceMeasureSL2.cpp:198-205)
+    launch_sub(m_ReferenceImage.get(), m_State.get() + m_Opt.nt * nl, m_Lam.get(), nl, m_Opt.stream);
+    IncAdjointCore(m_Lam.get(), m_BodyForce.get());
+    if (m_VelocityIsZero)
+        launch_copy(m_Lam.get(), m_Adjoint.get(), nl, m_Opt.stream);
+    else
+        launch_copy(m_Scalar.get(), m_Adjoint.get(), nl, m_Opt.stream);
+    return 0;
+}
+
+int CLAIRE::EvaluateObjective(double* J, double* D, double* R, const float* v) {
+    if (v) SetControlVariable(v);
+    CLR_REQUIRE(m_HaveReference, kErrState, "reference image not set");
+    SolveStateEquation();
+    const int64_t nl = m_Opt.domain.nl;
+    const double hd = m_Opt.GetLebesgueMeasure();
+    // ½·hd·Σ(mR − m^nt)²/nc (src/DistanceMeasureSL2.cpp:124-147)
+    *D = 0.5 * hd * m_Red->SqDist(m_ReferenceImage.get(), m_State.get() + m_Opt.nt * nl, nl);
+    *R = 0.0;
+    if (!m_VelocityIsZero) m_Regularization->EvaluateFunctional(R, m_VelocityField.get());
+    *J = *D + *R;
+    return 0;
+}
+
+int CLAIRE::EvaluateGradient(float* g, const float* v) {
+    if (v) SetControlVariable(v);
+    if (!m_StateValid) SolveStateEquation();
+    SolveAdjointEquation();
+    VecField gv(&m_Opt, g);
+    if (m_VelocityIsZero) return m_BodyForce->GetComponents(g);  // src/CLAIRE.cpp:960-965
+    // g = β·A[v] + b (src/CLAIRE.cpp:1021-1026)
+    m_Regularization->ApplyFused(&gv, m_VelocityField.get(), 1.0f, m_BodyForce.get(), 1.0f);
+    return 0;
+}
+
+int CLAIRE::ApplyInvRegularizationOperator(float* Ainvx, const float* x, bool applysqrt) {
+    // CLAIREBase::ApplyInvRegularizationOperator (src/CLAIREBase.cpp:987-1021)
+    VecField out(&m_Opt, Ainvx), in(&m_Opt, const_cast<float*>(x));
+    return m_Regularization->ApplyInverse(&out, &in, applysqrt);
+}
+
+}  // namespace reg

@@ -1,0 +1,326 @@
+// claire.hpp -- host classes of the MI355X registration hot path.
+//
+// The class and method names mirror the reference's C++ API so that the
+// code above the solver boundary reads the same:
+//   RegOpt            include/RegOpt.hpp:475-638 (the hot-path subset)
+//   VecField          include/VecField.hpp:35-106
+//   SemiLagrangian    include/SemiLagrangian.hpp:37-94
+//   Regularization*   include/Regularization.hpp:35-62 and the H1/H2/H1SN/H2SN subclasses
+//   CLAIRE            include/CLAIRE.hpp:37-188 (GN, semi-Lagrangian, nc = 1)
+// PETSc Vec arguments become device pointers to flat FP32 vectors; a vector
+// field is [v1|v2|v3] with each component x3-fastest (VecField::
+// SetComponents/GetComponents, src/VecField.cpp:396-476).  Methods return an
+// int error code like PetscErrorCode (0 = OK); failures throw
+// claire_amd::Error internally and are converted at the C-ABI boundary.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace reg {
+
+using claire_amd::c32;
+using claire_amd::Dims;
+using ScalarType = float;
+using IntType = int64_t;
+
+// Owning device allocation.
+template <class T>
+class DeviceArray {
+  public:
+    DeviceArray() = default;
+    explicit DeviceArray(size_t n) { resize(n); }
+    ~DeviceArray() { release(); }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    void resize(size_t n) {
+        if (n == n_) return;
+        release();
+        if (n) {
+            hipError_t e = hipMalloc(&p_, n * sizeof(T));
+            if (e != hipSuccess) {
+                p_ = nullptr;
+                throw claire_amd::Error(claire_amd::kErrOutOfMemory,
+                                        "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
+                                            hipGetErrorString(e));
+            }
+        }
+        n_ = n;
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+
+  private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Per-kernel-class timing with HIP events recorded on the context stream.
+class Profiler {
+  public:
+    struct Entry {
+        double ms = 0.0;
+        double bytes = 0.0;
+        long count = 0;
+    };
+    explicit Profiler(hipStream_t s) : stream_(s) {}
+    ~Profiler();
+    bool enabled() const { return enabled_; }
+    void enable(bool on);
+    void begin(const char* name, double bytes);
+    void end();
+    // Synchronize, fold the recorded events into the totals.
+    void collect();
+    void reset();
+    const std::map<std::string, Entry>& entries() const { return totals_; }
+
+  private:
+    struct Pending {
+        std::string name;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    hipEvent_t get_event();
+    hipStream_t stream_;
+    bool enabled_ = false;
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    std::map<std::string, Entry> totals_;
+};
+
+class KernelScope {
+  public:
+    KernelScope(Profiler* p, const char* name, double bytes) : p_(p && p->enabled() ? p : nullptr) {
+        if (p_) p_->begin(name, bytes);
+    }
+    ~KernelScope() {
+        if (p_) p_->end();
+    }
+
+  private:
+    Profiler* p_;
+};
+
+// The hot-path subset of the reference's RegOpt.
+struct RegOpt {
+    Dims domain{};
+    int nt = 4;
+    int nc = 1;
+    int regnorm = claire_amd::kRegH1SN;
+    float beta[3] = {1e-2f, 1e-4f, 0.0f};
+    hipStream_t stream = nullptr;
+    Profiler* prof = nullptr;
+    // include/RegOpt.hpp:485-500, src/RegOpt.cpp:2505-2512
+    float GetLebesgueMeasure() const { return domain.hd; }
+    float GetTimeStepSize() const { return 1.0f / static_cast<float>(nt); }
+    float ComputeFFTScale() const { return domain.scale; }
+};
+
+// FP64 reduction scratch.
+class Reducer {
+  public:
+    explicit Reducer(const RegOpt* opt);
+    double Dot(const float* a, const float* b, IntType n);
+    double AbsMax(const float* a, IntType n);
+    double SqDist(const float* a, const float* b, IntType n);
+
+  private:
+    double fetch();
+    const RegOpt* m_Opt;
+    DeviceArray<double> m_Partial, m_Result;
+};
+
+class VecField {
+  public:
+    // Owning field (three contiguous components).
+    explicit VecField(const RegOpt* opt);
+    // Non-owning view of a flat [v1|v2|v3] device vector.
+    VecField(const RegOpt* opt, float* flat);
+    float* m_X1 = nullptr;
+    float* m_X2 = nullptr;
+    float* m_X3 = nullptr;
+    float* Flat() const { return m_X1; }
+    float* Comp(int d) const { return d == 0 ? m_X1 : (d == 1 ? m_X2 : m_X3); }
+    int SetComponents(const float* flat);
+    int GetComponents(float* flat) const;
+    int SetValue(ScalarType v);
+    int Copy(const VecField* x);
+    int Scale(ScalarType a);
+    int AXPY(ScalarType a, const VecField* x);
+    int WAXPY(ScalarType a, const VecField* x, const VecField* y);
+
+  private:
+    const RegOpt* m_Opt;
+    DeviceArray<float> m_Storage;
+};
+
+// Replaces AccFFT's plan and spectral operators (single GPU).
+class SpectralOps {
+  public:
+    explicit SpectralOps(const RegOpt* opt);
+    void R2C(const float* in, c32* out);
+    void C2R(c32* in, float* out, float alpha = 1.f, const float* add = nullptr, float beta = 0.f);
+    void Grad(float* const g[3], const float* f);
+    void Div(float* d, const float* const v[3]);
+    // out_d = alpha·F⁻¹(m(k)·F(in_d)) + beta·add_d, d = 0..2
+    void Multiplier3(const claire_amd::SpecParams& sp, const float* const in[3], float* const out[3], float alpha,
+                     const float* const add[3], float beta);
+    c32* Spectrum(int i) const { return m_Spec[i].get(); }
+
+  private:
+    const RegOpt* m_Opt;
+    std::unique_ptr<claire_amd::FFTPlan3D> m_Plan;
+    DeviceArray<c32> m_Spec[3];
+};
+
+class Regularization {
+  public:
+    Regularization(const RegOpt* opt, SpectralOps* spec, Reducer* red) : m_Opt(opt), m_Spec(spec), m_Red(red) {}
+    virtual ~Regularization() = default;
+    virtual int EvaluateFunctional(double* R, const VecField* v);
+    virtual int EvaluateGradient(VecField* dvR, const VecField* v);
+    virtual int HessianMatVec(VecField* dvvR, const VecField* vtilde) { return EvaluateGradient(dvvR, vtilde); }
+    virtual int ApplyInverse(VecField* Ainvx, const VecField* x, bool applysqrt);
+    virtual int GetExtremeEigValsInvOp(ScalarType& emin, ScalarType& emax);
+    // Fused form used by HessianMatVec: out = alpha·R''v + beta·add.
+    void ApplyFused(VecField* out, const VecField* v, float alpha, const VecField* add, float beta);
+    virtual int Norm() const = 0;
+
+  protected:
+    claire_amd::SpecParams Params(int op) const;
+    const RegOpt* m_Opt;
+    SpectralOps* m_Spec;
+    Reducer* m_Red;
+};
+
+class RegularizationH1 : public Regularization {
+  public:
+    using Regularization::Regularization;
+    int Norm() const override { return claire_amd::kRegH1; }
+};
+class RegularizationH2 : public Regularization {
+  public:
+    using Regularization::Regularization;
+    int Norm() const override { return claire_amd::kRegH2; }
+};
+class RegularizationH1SN : public Regularization {
+  public:
+    using Regularization::Regularization;
+    int Norm() const override { return claire_amd::kRegH1SN; }
+};
+class RegularizationH2SN : public Regularization {
+  public:
+    using Regularization::Regularization;
+    int Norm() const override { return claire_amd::kRegH2SN; }
+};
+
+class SemiLagrangian {
+  public:
+    explicit SemiLagrangian(const RegOpt* opt);
+    // flag "state" (σ = +1) or "adjoint" (σ = −1)
+    int ComputeTrajectory(const VecField* v, const std::string& flag);
+    int Interpolate(float* xo, const float* xi, const std::string& flag);
+    // Out-of-place gathers along a trajectory (no aliasing allowed).
+    void Gather(float* xo, const float* xi, const std::string& flag);
+    void Gather3(const float* const in[3], float* const out[3], const std::string& flag);
+    int Interpolate(VecField* vo, const VecField* vi, const std::string& flag);
+    const float* Trajectory(const std::string& flag) const;
+    bool HasTrajectory(const std::string& flag) const;
+    // Per-axis displacement bound (cells) of the trajectory; gathers along it
+    // use the LDS-staged kernels when it is at most lds_gather_max_displacement().
+    int DisplacementBound(const std::string& flag) const { return m_D[Which(flag)]; }
+    bool UseLds(const std::string& flag) const {
+        return m_D[Which(flag)] <= claire_amd::lds_gather_max_displacement();
+    }
+
+  private:
+    int Which(const std::string& flag) const;
+    const RegOpt* m_Opt;
+    DeviceArray<float> m_X[2];
+    DeviceArray<int> m_Scratch;
+    bool m_Valid[2] = {false, false};
+    int m_D[2] = {0, 0};
+};
+
+class CLAIRE {
+  public:
+    explicit CLAIRE(const RegOpt& opt);
+    ~CLAIRE();
+
+    int SetControlVariable(const float* v);
+    int SetIncControlVariable(const float* vtilde);
+    int SetTemplateImage(const float* mT);
+    int SetReferenceImage(const float* mR);
+
+    int SolveStateEquation();
+    int SolveAdjointEquation();
+    int SolveIncStateEquation();
+    int SolveIncAdjointEquation();
+    // Hṽ = R''ṽ + b̃ (scale = false: divided by hd)
+    int HessianMatVec(float* Hvtilde, const float* vtilde, bool scale = true);
+    int EvaluateObjective(double* J, double* D, double* R, const float* v);
+    int EvaluateGradient(float* g, const float* v);
+    int ApplyInvRegularizationOperator(float* Ainvx, const float* x, bool applysqrt);
+
+    // Accessors for the C-ABI building blocks and tests.
+    const float* StateVariable() const { return m_State.get(); }
+    const float* AdjointVariable() const { return m_Adjoint.get(); }
+    const VecField* BodyForce() const { return m_BodyForce.get(); }
+    const float* IncStateVariable() const { return m_IncState[m_IncStateSlot].get(); }
+    const float* IncAdjointVariable() const { return m_IncAdjointT0.get(); }
+    const VecField* IncBodyForce() const { return m_IncBodyForce.get(); }
+    SemiLagrangian* SL() { return m_SL.get(); }
+    SpectralOps* Spectral() { return m_Spectral.get(); }
+    Regularization* Reg() { return m_Regularization.get(); }
+    Reducer* Red() { return m_Red.get(); }
+    const RegOpt& Opt() const { return m_Opt; }
+    bool VelocityIsZero() const { return m_VelocityIsZero; }
+    size_t DeviceBytes() const;
+    // Newton-invariant data used by the Hessian (computed on demand).
+    int PrepareHessian();
+    void EnsureTrajectories();
+    int SolveIncAdjointFrom(const float* mtilde);
+
+  private:
+    void RequireVelocity() const;
+    void EnsureStateGradients();
+    void EnsureGradientsAtStateTrajectory();
+    void IncStateCore(const VecField* vt, float* lt0_out);
+    void IncAdjointCore(const float* lt0, VecField* bt);
+
+    RegOpt m_Opt;
+    std::unique_ptr<Reducer> m_Red;
+    std::unique_ptr<SpectralOps> m_Spectral;
+    std::unique_ptr<Regularization> m_Regularization;
+    std::unique_ptr<SemiLagrangian> m_SL;
+
+    std::unique_ptr<VecField> m_VelocityField, m_IncVelocityField, m_BodyForce, m_IncBodyForce, m_WorkVecField1;
+    DeviceArray<float> m_TemplateImage, m_ReferenceImage;
+    DeviceArray<float> m_State;          // (nt+1)·nl
+    DeviceArray<float> m_Adjoint;        // λ at t = 0
+    DeviceArray<float> m_IncState[2];    // ping-pong m̃
+    int m_IncStateSlot = 0;
+    DeviceArray<float> m_IncAdjointT0;   // λ̃ at t = 0
+    DeviceArray<float> m_X4;             // adjoint trajectory + update coefficient, 4·nl
+    DeviceArray<float> m_GradM;          // ∇m^j, (nt+1)·3·nl
+    DeviceArray<float> m_GradMX;         // (∇m^j)(X_s), nt·3·nl
+    DeviceArray<float> m_S;              // inc-state decrements, nt·nl
+    DeviceArray<float> m_Lam;            // adjoint time series λ^0..λ^{nt−1}, nt·nl
+    DeviceArray<float> m_Scalar;         // nl work
+    DeviceArray<float> m_VtX;            // ṽ(X_s), 3·nl
+
+    bool m_HaveVelocity = false, m_HaveIncVelocity = false, m_HaveTemplate = false, m_HaveReference = false;
+    bool m_VelocityIsZero = false;
+    bool m_TrajValid = false, m_StateValid = false, m_GradMValid = false, m_GradMXValid = false;
+};
+
+}  // namespace reg

@@ -1,0 +1,127 @@
+// kernels.hpp -- host-side launch interface of the claire_amd HIP kernels.
+#pragma once
+
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+namespace claire_amd {
+
+struct __align__(8) c32 {
+    float x, y;
+};
+
+// ---------------------------------------------------------------------------
+// FFT (fft.hip)
+// ---------------------------------------------------------------------------
+bool fft_size_supported(int n);
+
+class FFTPlan3D {
+  public:
+    explicit FFTPlan3D(const Dims& d);
+    ~FFTPlan3D();
+    FFTPlan3D(const FFTPlan3D&) = delete;
+    FFTPlan3D& operator=(const FFTPlan3D&) = delete;
+
+    // Full transforms (unnormalized).  c2r overwrites its input.
+    // c2r epilogue: out = alpha·F⁻¹(in) + beta·add (add may be null).
+    void r2c(const float* in, c32* out, hipStream_t st) const;
+    void c2r(c32* in, float* out, hipStream_t st, float alpha = 1.f, const float* add = nullptr, float beta = 0.f) const;
+
+    // Individual passes.
+    void x3_r2c(const float* in, c32* out, hipStream_t st) const;
+    void x3_c2r(const c32* in, float* out, hipStream_t st, float alpha = 1.f, const float* add = nullptr,
+                float beta = 0.f) const;
+    void x2(c32* data, int dir, hipStream_t st) const;
+    void x1(c32* data, int dir, hipStream_t st) const;
+    // Fused x1 passes (forward x1, k-space operator, inverse x1).
+    void x1_multiplier(const c32* const in[3], c32* const out[3], int ncomp, const SpecParams& sp, hipStream_t st) const;
+    void x1_grad(const c32* in, c32* const out[3], hipStream_t st) const;
+    void x1_div(const c32* const in[3], c32* out, hipStream_t st) const;
+
+    Dims dims;
+
+  private:
+    c32* tw[3] = {nullptr, nullptr, nullptr};
+};
+
+// ---------------------------------------------------------------------------
+// Semi-Lagrangian gathers (interp.hip).  X: normalized query points,
+// interleaved xyz (3 floats per grid point).  X4: xyz + one per-point
+// coefficient (4 floats per point).
+// ---------------------------------------------------------------------------
+struct GatherDims {
+    int n1, n2, n3;
+    int64_t nl;
+};
+
+GatherDims gather_dims(const Dims& d);
+
+// out = f(X)
+void launch_gather1(const GatherDims& g, const float* f, const float* X, float* out, hipStream_t st);
+// out_d = f_d(X), d = 0..2 (shared stencil weights)
+void launch_gather3(const GatherDims& g, const float* const f[3], const float* X, float* const out[3], hipStream_t st);
+// RK2 characteristic, Euler predictor and velocity gather fused:
+//   X̃ = (x − σ·ht·v)/2π,  X = (x − σ·ht/2·(v(X̃) + v))/2π
+void launch_trajectory_rk2(const Dims& d, const float* const v[3], float sigma, float ht, float* X, hipStream_t st);
+// Adjoint-side per-point data: X4 = (X_a, c) with d = ∇·v, dX = d(X_a),
+// c = 1 + ht/2·(dX + (1 + ht·dX)·d) (the SL adjoint update
+// λ⁺ = λX + ht/2·(λX·dX + (λX + ht·λX·dX)·d) = c·λX).
+void launch_adjoint_coeff(const GatherDims& g, const float* Xa, const float* divv, float ht, float* X4,
+                          hipStream_t st);
+// m̃^{j+1} = m̃^j(X_s) − s_j, negated when `negate` (λ̃⁰ = −m̃^nt).
+void launch_inc_state_step(const GatherDims& g, const float* Xs, const float* mt, const float* sj, float* out,
+                           int negate, hipStream_t st);
+// λ^{j+1} = c·λ^j(X_a)
+void launch_adjoint_step(const GatherDims& g, const float* X4, const float* lam, float* out, hipStream_t st);
+// Last adjoint step fused with the body force:
+//   λ^nt = c·λ^{nt−1}(X_a),  b_d = hd·Σ_{j=0..nt} w_j·λ^j·∂_d m^{nt−j},
+//   w_0 = w_nt = ht/2, w_j = ht otherwise.  lam holds λ^0..λ^{nt−1};
+//   λ^nt is written to lam_out when it is not null.
+void launch_adjoint_final(const GatherDims& g, int nt, float ht, float hd, const float* X4, const float* lam,
+                          const float* gm, float* const b[3], float* lam_out, hipStream_t st);
+// v = 0 body force: b_d = hd·λ·∂_d m
+void launch_body_force_static(int64_t nl, float hd, const float* lam, const float* gm, float* const b[3],
+                              hipStream_t st);
+
+// LDS-staged gathers (gather_lds.hip).  `dbound` is the per-axis
+// characteristic displacement bound in cells (max_displacement); points
+// beyond it fall back to the direct gather.  Vector fields are passed as nf
+// equally spaced component pointers.
+int lds_gather_max_displacement();
+int max_displacement(const GatherDims& g, const float* X, int stride, int* scratch, hipStream_t st);
+void launch_lds_gather(int dbound, const GatherDims& g, const float* const f[], int nf, const float* X,
+                       float* const out[], hipStream_t st);
+void launch_lds_inc_state_step(int dbound, const GatherDims& g, const float* Xs, const float* mt, const float* sj,
+                               float* out, int negate, hipStream_t st);
+void launch_lds_adjoint_step(int dbound, const GatherDims& g, const float* X4, const float* lam, float* out,
+                             hipStream_t st);
+void launch_lds_adjoint_final(int dbound, const GatherDims& g, int nt, float ht, float hd, const float* X4,
+                              const float* lam, const float* gm, float* const b[3], float* lam_out, hipStream_t st);
+// s_j = ht/2·[(∇m^j)(X_s)·ṽX + ∇m^{j+1}·ṽ] for j = 0..nt−1 from a precomputed
+// ṽX = ṽ(X_s); writes m̃¹ = −s_0 (λ̃⁰ = s_0 when nt = 1) and s_1..s_{nt−1}.
+void launch_inc_state_rhs(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
+                          const float* gX, const float* gm, float* s, float* mt1, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Element-wise kernels and reductions (pointwise.hip)
+// ---------------------------------------------------------------------------
+void launch_fill(float* x, float value, int64_t n, hipStream_t st);
+void launch_copy(const float* x, float* y, int64_t n, hipStream_t st);
+void launch_scale(float* x, float alpha, int64_t n, hipStream_t st);
+// y = alpha·x + y
+void launch_axpy(float alpha, const float* x, float* y, int64_t n, hipStream_t st);
+// w = alpha·x + y
+void launch_waxpy(float alpha, const float* x, const float* y, float* w, int64_t n, hipStream_t st);
+// z = x − y
+void launch_sub(const float* x, const float* y, float* z, int64_t n, hipStream_t st);
+
+// FP64 reductions.  `partial` must hold kReduceBlocks doubles and `result`
+// one double (device memory).
+constexpr int kReduceBlocks = 1024;
+void launch_dot(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st);
+void launch_absmax(const float* a, int64_t n, double* partial, double* result, hipStream_t st);
+void launch_sqdist(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st);
+
+}  // namespace claire_amd

@@ -1,0 +1,137 @@
+// pointwise.hip -- element-wise updates and FP64 reductions.
+//
+// Replaces the PETSc Vec operations on the path (VecAXPY, VecWAXPY, VecScale,
+// VecSet, VecCopy, VecTDot, VecNorm; e.g. src/VecField.cpp:577-692,
+// src/RegularizationH1.cpp:105-143) and the SL2 mismatch reduction
+// (src/DistanceMeasureSL2.cpp:124-147).  Reductions accumulate in FP64 with a
+// fixed block count and a fixed combination order, so results do not depend
+// on scheduling.
+#include "kernels.hpp"
+
+namespace claire_amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+int blocks_for(int64_t n, int per_thread = 1) {
+    const int64_t b = (n + static_cast<int64_t>(kThreads) * per_thread - 1) / (static_cast<int64_t>(kThreads) * per_thread);
+    return static_cast<int>(b < 1 ? 1 : b);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill(float* __restrict__ x, float v, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+__global__ __launch_bounds__(kThreads) void k_copy(const float* __restrict__ x, float* __restrict__ y, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+__global__ __launch_bounds__(kThreads) void k_scale(float* __restrict__ x, float a, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) x[i] *= a;
+}
+__global__ __launch_bounds__(kThreads) void k_axpy(float a, const float* __restrict__ x, float* __restrict__ y, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + y[i];
+}
+__global__ __launch_bounds__(kThreads) void k_waxpy(float a, const float* __restrict__ x, const float* __restrict__ y,
+                                                    float* __restrict__ w, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) w[i] = a * x[i] + y[i];
+}
+__global__ __launch_bounds__(kThreads) void k_sub(const float* __restrict__ x, const float* __restrict__ y,
+                                                  float* __restrict__ z, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) z[i] = x[i] - y[i];
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+enum RedOp { kRedDot = 0, kRedAbsMax = 1, kRedSqDist = 2 };
+
+template <int OP>
+__global__ __launch_bounds__(kThreads) void k_reduce_partial(const float* __restrict__ a, const float* __restrict__ b,
+                                                             int64_t n, double* __restrict__ partial) {
+    __shared__ double sh[kThreads];
+    double acc = 0.0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
+        if constexpr (OP == kRedDot) {
+            acc += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+        } else if constexpr (OP == kRedAbsMax) {
+            acc = fmax(acc, fabs(static_cast<double>(a[i])));
+        } else {
+            const double d = static_cast<double>(a[i] - b[i]);
+            acc += d * d;
+        }
+    }
+    const double r = (OP == kRedAbsMax) ? block_max(acc, sh) : block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+template <int OP>
+__global__ __launch_bounds__(kThreads) void k_reduce_final(const double* __restrict__ partial, int np,
+                                                           double* __restrict__ result) {
+    __shared__ double sh[kThreads];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < np; i += kThreads) acc = (OP == kRedAbsMax) ? fmax(acc, partial[i]) : acc + partial[i];
+    const double r = (OP == kRedAbsMax) ? block_max(acc, sh) : block_sum(acc, sh);
+    if (threadIdx.x == 0) result[0] = r;
+}
+
+template <int OP>
+void reduce(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_partial<OP>, dim3(kReduceBlocks), dim3(kThreads), 0, st, a, b, n, partial);
+    hipLaunchKernelGGL(k_reduce_final<OP>, dim3(1), dim3(kThreads), 0, st, partial, kReduceBlocks, result);
+}
+
+}  // namespace
+
+void launch_fill(float* x, float value, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kThreads), 0, st, x, value, n);
+}
+void launch_copy(const float* x, float* y, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_copy, dim3(blocks_for(n)), dim3(kThreads), 0, st, x, y, n);
+}
+void launch_scale(float* x, float alpha, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_scale, dim3(blocks_for(n)), dim3(kThreads), 0, st, x, alpha, n);
+}
+void launch_axpy(float alpha, const float* x, float* y, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n)), dim3(kThreads), 0, st, alpha, x, y, n);
+}
+void launch_waxpy(float alpha, const float* x, const float* y, float* w, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_waxpy, dim3(blocks_for(n)), dim3(kThreads), 0, st, alpha, x, y, w, n);
+}
+void launch_sub(const float* x, const float* y, float* z, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_sub, dim3(blocks_for(n)), dim3(kThreads), 0, st, x, y, z, n);
+}
+void launch_dot(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st) {
+    reduce<kRedDot>(a, b, n, partial, result, st);
+}
+void launch_absmax(const float* a, int64_t n, double* partial, double* result, hipStream_t st) {
+    reduce<kRedAbsMax>(a, nullptr, n, partial, result, st);
+}
+void launch_sqdist(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st) {
+    reduce<kRedSqDist>(a, b, n, partial, result, st);
+}
+
+}  // namespace claire_amd

@@ -1,0 +1,165 @@
+/*
+ * claire_amd.h -- C ABI of the MI355X-native CLAIRE registration hot path.
+ *
+ * Drop-in boundary for the reference's Gauss–Newton–Krylov path
+ * (GN, semi-Lagrangian transport, nc = 1, FP32 data).  All field pointers
+ * are DEVICE pointers to FP32 data; vector fields are flat [v1|v2|v3] with
+ * each component stored x3-fastest (the layout of the PETSc vectors TAO/KSP
+ * hand to CLAIRE, VecField::SetComponents/GetComponents,
+ * src/VecField.cpp:396-476).  Work is ordered on the stream given to
+ * clr_create (0 = the null stream).  Every function returns 0 on success or
+ * a nonzero status (CLR_ERR_*) with a message from clr_last_error(); this
+ * maps one-to-one onto PetscErrorCode at the binding.  A context is not
+ * re-entrant (like the reference's problem object): calls on one context
+ * must not overlap.
+ */
+#ifndef CLAIRE_AMD_H_
+#define CLAIRE_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLR_OK 0
+#define CLR_ERR_INVALID_ARGUMENT 1
+#define CLR_ERR_UNSUPPORTED_SIZE 2
+#define CLR_ERR_HIP 3
+#define CLR_ERR_STATE 4
+#define CLR_ERR_OUT_OF_MEMORY 5
+
+/* Regularization norms: RegularizationH1 / H2 / H1SN / H2SN
+ * (src/RegularizationH1.cpp, src/RegularizationH2.cpp,
+ *  src/RegularizationH1SN.cpp, src/RegularizationH2SN.cpp). */
+#define CLR_REG_H1 0
+#define CLR_REG_H2 1
+#define CLR_REG_H1SN 2
+#define CLR_REG_H2SN 3
+
+/* Trajectory selector: "state" (σ = +1) / "adjoint" (σ = −1),
+ * SemiLagrangian::ComputeTrajectory flag (src/SemiLagrangian.cpp:217-223). */
+#define CLR_STATE 0
+#define CLR_ADJOINT 1
+
+typedef struct clr_ctx clr_ctx;
+
+/* Last error message of the calling thread. */
+const char* clr_last_error(void);
+
+/* --- device memory helpers (plumbing for bindings without a GPU array type) */
+int clr_device_malloc(void** ptr, size_t bytes);
+int clr_device_free(void* ptr);
+int clr_memcpy_htod(void* dst, const void* src, size_t bytes);
+int clr_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int clr_memcpy_dtod(void* dst, const void* src, size_t bytes);
+int clr_device_synchronize(void);
+
+/* Create a problem object for an nx[0]×nx[1]×nx[2] periodic grid on
+ * [0,2π)^3 with nt time steps, the given regularization norm and weights
+ * beta[0..2] (RegOpt m_RegNorm.beta), on `stream` (a hipStream_t or NULL).
+ * Replaces `new CLAIRE(opt)` + RegOpt::DoSetup/InitializeFFT
+ * (src/RegOpt.cpp:906-1030; hx = 2π/nx src/RegOpt.cpp:935).
+ * Supported grid sizes: powers of two from 8 to 512 per axis. */
+int clr_create(clr_ctx** ctx, const int nx[3], int nt, int regnorm, const double beta[3], void* stream);
+int clr_destroy(clr_ctx* ctx);
+int clr_synchronize(clr_ctx* ctx);
+/* Device memory held by the context (bytes). */
+int clr_device_bytes(clr_ctx* ctx, size_t* bytes);
+
+/* CLAIRE::SetControlVariable (include/CLAIRE.hpp; src/CLAIREBase.cpp) --
+ * sets v (3·nl) and invalidates trajectories and Newton-invariant caches. */
+int clr_set_velocity(clr_ctx* ctx, const float* v);
+/* CLAIRE::SetIncControlVariable */
+int clr_set_inc_velocity(clr_ctx* ctx, const float* vtilde);
+/* CLAIREBase::SetTemplateImage / SetReferenceImage */
+int clr_set_template_image(clr_ctx* ctx, const float* mT);
+int clr_set_reference_image(clr_ctx* ctx, const float* mR);
+
+/* CLAIRE::SolveStateEquation (src/CLAIRE.cpp:1861-1984, SL 2116-2170).
+ * m_out: (nt+1)·nl time series, or NULL to keep it on the context only. */
+int clr_solve_state(clr_ctx* ctx, float* m_out);
+/* CLAIRE::SolveAdjointEquation (src/CLAIRE.cpp:2181-2356, SL 2550-2725):
+ * λ at t = 0 (nl, may be NULL) and the body force b scaled by hd (3·nl, may
+ * be NULL).  Runs the state solve first when it is stale. */
+int clr_solve_adjoint(clr_ctx* ctx, float* lambda_out, float* b_out);
+/* CLAIRE::SolveIncStateEquation (src/CLAIRE.cpp:2866-2950, SL 3147-3294):
+ * m̃ at t = 1 for the incremental velocity vtilde (3·nl; NULL = the one set
+ * with clr_set_inc_velocity). */
+int clr_solve_inc_state(clr_ctx* ctx, const float* vtilde, float* mtilde_out);
+/* CLAIRE::SolveIncAdjointEquation (src/CLAIRE.cpp:3307-3491, GN SL
+ * 3849-4015) with final condition λ̃ = −m̃ (src/DistanceMeasureSL2.cpp:
+ * 223-276): λ̃ at t = 0 (may be NULL) and b̃ scaled by hd (3·nl, may be NULL). */
+int clr_solve_inc_adjoint(clr_ctx* ctx, const float* mtilde, float* ltilde_out, float* btilde_out);
+
+/* CLAIRE::HessianMatVec(Vec Hvtilde, Vec vtilde, bool scale)
+ * (src/CLAIRE.cpp:1218-1303 → HessMatVec 1311-1379), the GN Hessian behind
+ * the TAO/KSP shell matrix callback reg::HessianMatVec(Mat,Vec,Vec)
+ * (src/TaoInterface.cpp:137-155).  Hv must not alias vtilde (PETSc MatMult
+ * rule).  scale = 0 divides the result by hd. */
+int clr_hessian_matvec(clr_ctx* ctx, float* Hv, const float* vtilde, int scale);
+/* Precompute the Newton-invariant data the Hessian uses (trajectories,
+ * state, ∇m^j, (∇m^j)(X_s), adjoint update coefficient).  Optional: the
+ * first clr_hessian_matvec after clr_set_velocity does it otherwise. */
+int clr_prepare_hessian(clr_ctx* ctx);
+
+/* CLAIRE::EvaluateObjective (src/CLAIRE.cpp:793-865) behind
+ * reg::EvaluateObjective(Tao,Vec,PetscReal*,void*) (src/TaoInterface.cpp:40-61).
+ * v may be NULL (use the current velocity). */
+int clr_evaluate_objective(clr_ctx* ctx, const float* v, double* J, double* D, double* R);
+/* CLAIRE::EvaluateGradient (src/CLAIRE.cpp:871-1008, L2 gradient 1015-1042)
+ * behind reg::EvaluateGradient(Tao,Vec,Vec,void*) (src/TaoInterface.cpp:67-87). */
+int clr_evaluate_gradient(clr_ctx* ctx, const float* v, float* g);
+/* CLAIREBase::ApplyInvRegularizationOperator (src/CLAIREBase.cpp:987-1021),
+ * the INVREG preconditioner behind reg::PrecondMatVec(PC,Vec,Vec)
+ * (src/TaoInterface.cpp:162-180, src/Preconditioner.cpp:549-575).
+ * out may alias in. */
+int clr_apply_inv_reg(clr_ctx* ctx, const float* in, float* out, int applysqrt);
+
+/* Regularization::EvaluateGradient == HessianMatVec (3·nl → 3·nl). */
+int clr_reg_apply(clr_ctx* ctx, const float* v, float* out);
+/* Regularization::EvaluateFunctional */
+int clr_reg_functional(clr_ctx* ctx, const float* v, double* R);
+/* Regularization::GetExtremeEigValsInvOp */
+int clr_reg_eigvals_inv(clr_ctx* ctx, double* emin, double* emax);
+
+/* SemiLagrangian::ComputeTrajectory (RK2, src/SemiLagrangian.cpp:192-288)
+ * for the current velocity; X_out: nl×3 interleaved, normalized by 2π. */
+int clr_trajectory(clr_ctx* ctx, int flag, float* X_out);
+/* SemiLagrangian::Interpolate scalar (src/SemiLagrangian.cpp:536-594) and
+ * vector (:632-728) along the current trajectory.  out may alias in. */
+int clr_interp_scalar(clr_ctx* ctx, int flag, const float* in, float* out);
+int clr_interp_vector(clr_ctx* ctx, int flag, const float* in, float* out);
+/* Tricubic interpolation of f at arbitrary query points X (nl×3 normalized),
+ * Interp3_Plan::interpolate + optimized_interp3_ghost_xyz_p
+ * (src/Interp3_Plan.cpp:699-845, src/interp3.cpp:2492-2719). */
+int clr_interp_points(clr_ctx* ctx, const float* f, const float* X, float* out);
+
+/* accfft_grad_t / accfft_divergence_t (AccFFT, not vendored; e.g.
+ * src/CLAIRE.cpp:2622,2457) and accfft_execute_r2c_t / c2r_t.  The complex
+ * arrays are nx0·nx1·(nx2/2+1) interleaved (re,im) FP32; c2r is
+ * unnormalized and leaves its input intact. */
+int clr_grad(clr_ctx* ctx, const float* f, float* g);
+int clr_div(clr_ctx* ctx, const float* v, float* d);
+int clr_fft_r2c(clr_ctx* ctx, const float* in, float* out_complex);
+int clr_fft_c2r(clr_ctx* ctx, const float* in_complex, float* out);
+
+/* FP64 inner product / 2-norm of n FP32 values (VecTDot / VecNorm). */
+int clr_dot(clr_ctx* ctx, const float* a, const float* b, int64_t n, double* out);
+int clr_norm2(clr_ctx* ctx, const float* a, int64_t n, double* out);
+
+/* Kernel timing with HIP events on the context stream.  While enabled every
+ * kernel launch is bracketed by events; clr_profile_get returns, for entry
+ * i (0-based, sorted by name), the kernel class name, accumulated time in
+ * ms, launch count and the algorithmic bytes of those launches.  Returns
+ * CLR_ERR_INVALID_ARGUMENT past the last entry. */
+int clr_profile_enable(clr_ctx* ctx, int on);
+int clr_profile_reset(clr_ctx* ctx);
+int clr_profile_get(clr_ctx* ctx, int i, char* name, int name_len, double* ms, long* count, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLAIRE_AMD_H_ */

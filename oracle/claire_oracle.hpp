@@ -1,0 +1,806 @@
+// claire_oracle.hpp -- CPU oracle for the CLAIRE registration hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This is a plain C++17 restatement of the
+// reference's algorithm, templated on the real type: instantiated with
+// double it is the accuracy oracle, instantiated with float it is the
+// single-precision "reference CPU path" (the reference's FP32 PETSc build
+// restated, OpenMP over grid points, with the reference's OpenMP races on
+// shared temporaries privatized).  Only tests, the smoke check and the CPU
+// baseline leg of the benchmark may load it, and only as the checker.  The
+// product library never links it.
+//
+// Every function cites the reference lines it restates (paths relative to
+// the reference repository root).
+//
+// Parity pins (see tests/test_oracle_*.py):
+//   fft / r2c / c2r ..... numpy.fft (pocketfft, an independent library) and
+//                         brute-force DFT on tiny grids
+//   grad / div .......... closed forms on trigonometric polynomials,
+//                         Nyquist mode mapped to zero
+//   reg operators ....... Fourier-mode eigenpairs, inverse∘apply = hd·I / I
+//   interpolation ....... partition of unity, node reproduction, exactness
+//                         for per-axis cubics, integer shifts, brute force
+//   trajectory .......... constant velocity closed form
+//   transport solves .... v = 0 closed forms, constant-v translation
+//   Hessian matvec ...... v = 0 closed form, symmetry, linearity, Taylor
+//                         test against the gradient
+#pragma once
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+#include <algorithm>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace oracle {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// ---------------------------------------------------------------------------
+// Grid conventions.
+//   hx[i] = 2π/nx[i]                         src/RegOpt.cpp:935
+//   hd    = hx0·hx1·hx2                      include/RegOpt.hpp:485-489
+//   ht    = 1/nt (horizon [0,1])             include/RegOpt.hpp:497-500
+//   scale = 1/(n0·n1·n2)                     src/RegOpt.cpp:2505-2512
+//   flat index l = i1·n2·n3 + i2·n3 + i3     include/CLAIREUtils.hpp:198-212
+// ---------------------------------------------------------------------------
+template <class T>
+struct Grid {
+    int n[3];
+    int64_t nl;      // number of real grid points
+    int nh;          // n[2]/2 + 1 (half-spectrum length along x3)
+    int64_t nlc;     // number of complex coefficients n0·n1·nh
+    T hx[3];
+    T hd;
+    T scale;
+    explicit Grid(const int nx[3]) {
+        for (int i = 0; i < 3; ++i) {
+            n[i] = nx[i];
+            if (nx[i] < 2) throw std::invalid_argument("grid size must be >= 2");
+            hx[i] = static_cast<T>(kPi * 2.0 / static_cast<double>(static_cast<T>(nx[i])));
+        }
+        nl = static_cast<int64_t>(n[0]) * n[1] * n[2];
+        nh = n[2] / 2 + 1;
+        nlc = static_cast<int64_t>(n[0]) * n[1] * nh;
+        hd = hx[0] * hx[1] * hx[2];
+        T s = T(1);
+        for (int i = 0; i < 3; ++i) s *= static_cast<T>(n[i]);
+        scale = T(1) / s;
+    }
+};
+
+// Wave number with the Nyquist mode mapped to zero.
+// include/CLAIREUtils.hpp:248-255 (ComputeWaveNumber)
+inline long wavenumber(long k, long n) {
+    if (k > n / 2) return k - n;
+    if (k == n / 2) return 0;
+    return k;
+}
+
+// ---------------------------------------------------------------------------
+// FFT.  The reference calls AccFFT (github.com/amirgholami/accfft, master,
+// not vendored; deps/get_libs.sh:8) on top of FFTW 3.3.6-pl2
+// (deps/get_libs.sh:16).  Both compute the unnormalized DFT
+//   X[k] = Σ_n x[n]·exp(-2πi·k·n/N)  (forward),  exp(+2πi…) (backward).
+// This is an own recursive mixed-radix decimation-in-time Cooley–Tukey
+// transform with twiddles generated in double precision; it is pinned against
+// numpy.fft in the tests.
+// ---------------------------------------------------------------------------
+template <class T>
+class FFT1D {
+  public:
+    using C = std::complex<T>;
+    explicit FFT1D(int n) : n_(n) {
+        int m = n;
+        for (int p : {4, 2, 3, 5}) {
+            while (m % p == 0) {
+                factors_.push_back(p);
+                m /= p;
+            }
+        }
+        for (int p = 7; m > 1; p += 2) {
+            while (m % p == 0) { factors_.push_back(p); m /= p; }
+        }
+        tw_.resize(n);
+        for (int k = 0; k < n; ++k) {
+            double a = -2.0 * kPi * static_cast<double>(k) / static_cast<double>(n);
+            tw_[k] = C(static_cast<T>(std::cos(a)), static_cast<T>(std::sin(a)));
+        }
+    }
+    int size() const { return n_; }
+    // out[k] = Σ in[j·is]·w^{jk}; sign -1 forward, +1 backward.
+    void exec(const C* in, int64_t is, C* out, int sign) const {
+        rec(in, is, out, n_, 0, 1, sign);
+    }
+
+  private:
+    C twid(int64_t idx, int sign) const {
+        C w = tw_[static_cast<size_t>(idx % n_)];
+        return sign < 0 ? w : std::conj(w);
+    }
+    void rec(const C* in, int64_t is, C* out, int n, size_t fi, int64_t twstep, int sign) const {
+        if (n == 1) { out[0] = in[0]; return; }
+        const int p = factors_[fi];
+        const int m = n / p;
+        for (int q = 0; q < p; ++q) rec(in + q * is, is * p, out + q * m, m, fi + 1, twstep * p, sign);
+        C t[16];
+        C* tp = t;
+        std::vector<C> big;
+        if (p > 16) { big.resize(p); tp = big.data(); }
+        for (int k = 0; k < m; ++k) {
+            for (int q = 0; q < p; ++q) {
+                C y = out[q * m + k];
+                tp[q] = (q == 0 || k == 0) ? y : y * twid(static_cast<int64_t>(q) * k * twstep, sign);
+            }
+            const int64_t pstep = static_cast<int64_t>(n_) / p;
+            for (int s = 0; s < p; ++s) {
+                C acc = tp[0];
+                for (int q = 1; q < p; ++q) acc += tp[q] * twid(static_cast<int64_t>((q * s) % p) * pstep, sign);
+                out[k + m * s] = acc;
+            }
+        }
+    }
+    int n_;
+    std::vector<int> factors_;
+    std::vector<C> tw_;
+};
+
+// 3D real-to-complex transform, unnormalized; output layout [k1][k2][k3],
+// k3 in [0, n3/2].  (AccFFT single-rank layout, src/RegularizationH1.cpp:212-233)
+template <class T>
+void r2c3d(const Grid<T>& g, const T* in, std::complex<T>* out) {
+    using C = std::complex<T>;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2], nh = g.nh;
+    FFT1D<T> f0(n0), f1(n1), f2(n2);
+#pragma omp parallel
+    {
+        std::vector<C> a(std::max({n0, n1, n2})), b(std::max({n0, n1, n2}));
+#pragma omp for collapse(2) schedule(static)
+        for (int i0 = 0; i0 < n0; ++i0)
+            for (int i1 = 0; i1 < n1; ++i1) {
+                const T* row = in + (static_cast<int64_t>(i0) * n1 + i1) * n2;
+                for (int i = 0; i < n2; ++i) a[i] = C(row[i], T(0));
+                f2.exec(a.data(), 1, b.data(), -1);
+                C* o = out + (static_cast<int64_t>(i0) * n1 + i1) * nh;
+                for (int k = 0; k < nh; ++k) o[k] = b[k];
+            }
+#pragma omp for collapse(2) schedule(static)
+        for (int i0 = 0; i0 < n0; ++i0)
+            for (int k = 0; k < nh; ++k) {
+                C* base = out + static_cast<int64_t>(i0) * n1 * nh + k;
+                for (int i = 0; i < n1; ++i) a[i] = base[static_cast<int64_t>(i) * nh];
+                f1.exec(a.data(), 1, b.data(), -1);
+                for (int i = 0; i < n1; ++i) base[static_cast<int64_t>(i) * nh] = b[i];
+            }
+#pragma omp for collapse(2) schedule(static)
+        for (int i1 = 0; i1 < n1; ++i1)
+            for (int k = 0; k < nh; ++k) {
+                C* base = out + static_cast<int64_t>(i1) * nh + k;
+                const int64_t st = static_cast<int64_t>(n1) * nh;
+                for (int i = 0; i < n0; ++i) a[i] = base[i * st];
+                f0.exec(a.data(), 1, b.data(), -1);
+                for (int i = 0; i < n0; ++i) base[i * st] = b[i];
+            }
+    }
+}
+
+// 3D complex-to-real transform, unnormalized.  Like FFTW's c2r, the
+// imaginary parts of the k3 = 0 and k3 = n3/2 coefficients are discarded in
+// the last (x3) stage.  `in` is not modified.
+template <class T>
+void c2r3d(const Grid<T>& g, const std::complex<T>* in, T* out) {
+    using C = std::complex<T>;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2], nh = g.nh;
+    FFT1D<T> f0(n0), f1(n1), f2(n2);
+    std::vector<C> w(in, in + g.nlc);
+#pragma omp parallel
+    {
+        std::vector<C> a(std::max({n0, n1, n2})), b(std::max({n0, n1, n2}));
+#pragma omp for collapse(2) schedule(static)
+        for (int i1 = 0; i1 < n1; ++i1)
+            for (int k = 0; k < nh; ++k) {
+                C* base = w.data() + static_cast<int64_t>(i1) * nh + k;
+                const int64_t st = static_cast<int64_t>(n1) * nh;
+                for (int i = 0; i < n0; ++i) a[i] = base[i * st];
+                f0.exec(a.data(), 1, b.data(), +1);
+                for (int i = 0; i < n0; ++i) base[i * st] = b[i];
+            }
+#pragma omp for collapse(2) schedule(static)
+        for (int i0 = 0; i0 < n0; ++i0)
+            for (int k = 0; k < nh; ++k) {
+                C* base = w.data() + static_cast<int64_t>(i0) * n1 * nh + k;
+                for (int i = 0; i < n1; ++i) a[i] = base[static_cast<int64_t>(i) * nh];
+                f1.exec(a.data(), 1, b.data(), +1);
+                for (int i = 0; i < n1; ++i) base[static_cast<int64_t>(i) * nh] = b[i];
+            }
+#pragma omp for collapse(2) schedule(static)
+        for (int i0 = 0; i0 < n0; ++i0)
+            for (int i1 = 0; i1 < n1; ++i1) {
+                const C* h = w.data() + (static_cast<int64_t>(i0) * n1 + i1) * nh;
+                for (int k = 0; k < nh; ++k) a[k] = h[k];
+                a[0] = C(a[0].real(), T(0));
+                if (n2 % 2 == 0) a[n2 / 2] = C(a[n2 / 2].real(), T(0));
+                for (int k = nh; k < n2; ++k) a[k] = std::conj(a[n2 - k]);
+                f2.exec(a.data(), 1, b.data(), +1);
+                T* row = out + (static_cast<int64_t>(i0) * n1 + i1) * n2;
+                for (int i = 0; i < n2; ++i) row[i] = b[i].real();
+            }
+    }
+}
+
+// Visit every half-spectrum coefficient with its (Nyquist-zeroed) wave numbers.
+template <class T, class F>
+void for_each_mode(const Grid<T>& g, F&& fn) {
+    const int n0 = g.n[0], n1 = g.n[1], nh = g.nh;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k0 = 0; k0 < n0; ++k0)
+        for (int k1 = 0; k1 < n1; ++k1)
+            for (int k2 = 0; k2 < nh; ++k2) {
+                long w[3] = {wavenumber(k0, g.n[0]), wavenumber(k1, g.n[1]), wavenumber(k2, g.n[2])};
+                int64_t i = (static_cast<int64_t>(k0) * n1 + k1) * nh + k2;
+                fn(i, w);
+            }
+}
+
+// Spectral gradient ∇f = F⁻¹(i·w·f̂)·scale (AccFFT accfft_grad_t; call sites
+// e.g. src/CLAIRE.cpp:2622,3243; one r2c and three c2r, include/RegOpt.hpp:183-186).
+template <class T>
+void grad(const Grid<T>& g, const T* f, T* g1, T* g2, T* g3) {
+    using C = std::complex<T>;
+    std::vector<C> fh(g.nlc), gh(g.nlc);
+    r2c3d(g, f, fh.data());
+    T* outs[3] = {g1, g2, g3};
+    for (int d = 0; d < 3; ++d) {
+        const T s = g.scale;
+        for_each_mode(g, [&](int64_t i, const long* w) {
+            const T wd = static_cast<T>(w[d]);
+            const C v = fh[i];
+            gh[i] = C(-wd * v.imag() * s, wd * v.real() * s);
+        });
+        c2r3d(g, gh.data(), outs[d]);
+    }
+}
+
+// Spectral divergence ∇·v = F⁻¹(Σ_j i·w_j·v̂_j)·scale (AccFFT
+// accfft_divergence_t; e.g. src/CLAIRE.cpp:2622,3911; three r2c and one c2r).
+template <class T>
+void div(const Grid<T>& g, const T* v1, const T* v2, const T* v3, T* d) {
+    using C = std::complex<T>;
+    std::vector<C> a(g.nlc), b(g.nlc), c(g.nlc), acc(g.nlc);
+    r2c3d(g, v1, a.data());
+    r2c3d(g, v2, b.data());
+    r2c3d(g, v3, c.data());
+    const T s = g.scale;
+    for_each_mode(g, [&](int64_t i, const long* w) {
+        const T w0 = static_cast<T>(w[0]), w1 = static_cast<T>(w[1]), w2 = static_cast<T>(w[2]);
+        const T re = -w0 * a[i].imag() - w1 * b[i].imag() - w2 * c[i].imag();
+        const T im = w0 * a[i].real() + w1 * b[i].real() + w2 * c[i].real();
+        acc[i] = C(re * s, im * s);
+    });
+    c2r3d(g, acc.data(), d);
+}
+
+// ---------------------------------------------------------------------------
+// Regularization operators (per-mode multipliers on each velocity component).
+// ---------------------------------------------------------------------------
+enum RegNorm { kH1 = 0, kH2 = 1, kH1SN = 2, kH2SN = 3 };
+
+// First variation / second variation (identical; HessianMatVec calls
+// EvaluateGradient, src/RegularizationH1.cpp:277-303).
+//   H1   src/RegularizationH1.cpp:222-229     regop = hd·scale·β0·(−lapik + β1)
+//   H2   src/RegularizationH2.cpp:236-239     regop = scale·β0·(lapik² + β1)   (no hd)
+//   H1SN src/RegularizationH1SN.cpp:210-217   regop = −hd·scale·β·lapik
+//   H2SN src/RegularizationH2SN.cpp:241-244   regop = hd·scale·β·lapik²
+template <class T>
+T reg_apply_op(int norm, const long* w, T hd, T scale, const T* beta) {
+    const T lapik = -static_cast<T>(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    switch (norm) {
+        case kH1:   return hd * scale * beta[0] * (-lapik + beta[1]);
+        case kH2:   return scale * beta[0] * (lapik * lapik + beta[1]);
+        case kH1SN: return -hd * scale * beta[0] * lapik;
+        case kH2SN: return hd * scale * beta[0] * (lapik * lapik);
+    }
+    throw std::invalid_argument("unknown regularization norm");
+}
+
+// Inverse (sqrt = true: inverse square root).
+//   H1   src/RegularizationH1.cpp:373-382
+//   H2   src/RegularizationH2.cpp:375-381
+//   H1SN src/RegularizationH1SN.cpp:352-359   (β at w = 0)
+//   H2SN src/RegularizationH2SN.cpp:385-390   (β at w = 0)
+template <class T>
+T reg_inverse_op(int norm, const long* w, T scale, const T* beta, bool applysqrt) {
+    const T lapik = -static_cast<T>(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    T regop;
+    switch (norm) {
+        case kH1:   regop = beta[0] * (-lapik + beta[1]); break;
+        case kH2:   regop = beta[0] * (lapik * lapik + beta[1]); break;
+        case kH1SN: regop = (std::fabs(lapik) == T(0)) ? beta[0] : -beta[0] * lapik; break;
+        case kH2SN: regop = (std::fabs(lapik) == T(0)) ? beta[0] : beta[0] * (lapik * lapik); break;
+        default: throw std::invalid_argument("unknown regularization norm");
+    }
+    if (applysqrt) regop = std::sqrt(regop);
+    return scale / regop;
+}
+
+// Apply a per-mode real multiplier to the three components of a vector field.
+template <class T, class Op>
+void apply_multiplier3(const Grid<T>& g, const T* const in[3], T* const out[3], Op&& op) {
+    using C = std::complex<T>;
+    std::vector<C> h(g.nlc);
+    std::vector<T> mult(g.nlc);
+    for_each_mode(g, [&](int64_t i, const long* w) { mult[i] = op(w); });
+    for (int d = 0; d < 3; ++d) {
+        r2c3d(g, in[d], h.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < g.nlc; ++i) h[i] *= mult[i];
+        c2r3d(g, h.data(), out[d]);
+    }
+}
+
+// Regularization::EvaluateGradient / HessianMatVec.  β0 == 0 gives zero
+// (src/RegularizationH1.cpp:188-190).
+template <class T>
+void reg_apply(const Grid<T>& g, int norm, const T* beta, const T* const in[3], T* const out[3]) {
+    if (beta[0] == T(0)) {
+        for (int d = 0; d < 3; ++d) std::fill(out[d], out[d] + g.nl, T(0));
+        return;
+    }
+    apply_multiplier3(g, in, out, [&](const long* w) { return reg_apply_op<T>(norm, w, g.hd, g.scale, beta); });
+}
+
+// Regularization::ApplyInverse.  β0 == 0 copies (src/RegularizationH1.cpp:336-340).
+template <class T>
+void reg_inverse(const Grid<T>& g, int norm, const T* beta, const T* const in[3], T* const out[3], bool applysqrt) {
+    if (beta[0] == T(0)) {
+        for (int d = 0; d < 3; ++d) std::copy(in[d], in[d] + g.nl, out[d]);
+        return;
+    }
+    apply_multiplier3(g, in, out, [&](const long* w) { return reg_inverse_op<T>(norm, w, g.scale, beta, applysqrt); });
+}
+
+template <class T>
+double dot(const T* a, const T* b, int64_t n) {
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int64_t i = 0; i < n; ++i) s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+    return s;
+}
+
+// Regularization::EvaluateFunctional.
+//   H1   src/RegularizationH1.cpp:64-156   ½·hd·β0·(Σ‖∂_j v_i‖² + β1·‖v‖²)
+//   H2   src/RegularizationH2.cpp:64-170   ½·hd·Σ‖F⁻¹(scale·√β0·(lapik+√β1)·v̂)‖²
+//   H1SN src/RegularizationH1SN.cpp:64-150 ½·hd·β·Σ‖∂_j v_i‖²
+//   H2SN src/RegularizationH2SN.cpp:66-170 ½·β·hd·‖Δv‖²
+template <class T>
+double reg_functional(const Grid<T>& g, int norm, const T* beta, const T* const v[3]) {
+    if (beta[0] == T(0)) return 0.0;
+    std::vector<T> a(g.nl), b(g.nl), c(g.nl);
+    double acc = 0.0;
+    if (norm == kH1 || norm == kH1SN) {
+        for (int d = 0; d < 3; ++d) {
+            grad(g, v[d], a.data(), b.data(), c.data());
+            acc += dot(a.data(), a.data(), g.nl) + dot(b.data(), b.data(), g.nl) + dot(c.data(), c.data(), g.nl);
+        }
+        if (norm == kH1) {
+            double l2 = 0.0;
+            for (int d = 0; d < 3; ++d) l2 += dot(v[d], v[d], g.nl);
+            return 0.5 * g.hd * beta[0] * (acc + beta[1] * l2);
+        }
+        return 0.5 * g.hd * beta[0] * acc;
+    }
+    T* outs[3] = {a.data(), b.data(), c.data()};
+    if (norm == kH2) {
+        const T sb0 = std::sqrt(beta[0]), sb1 = std::sqrt(beta[1]);
+        apply_multiplier3(g, v, outs, [&](const long* w) {
+            const T lapik = -static_cast<T>(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+            return g.scale * sb0 * (lapik + sb1);
+        });
+        for (int d = 0; d < 3; ++d) acc += dot(outs[d], outs[d], g.nl);
+        return 0.5 * g.hd * acc;
+    }
+    apply_multiplier3(g, v, outs, [&](const long* w) {
+        const T lapik = -static_cast<T>(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        return g.scale * lapik;
+    });
+    for (int d = 0; d < 3; ++d) acc += dot(outs[d], outs[d], g.nl);
+    return 0.5 * beta[0] * g.hd * acc;
+}
+
+// Regularization::GetExtremeEigValsInvOp.
+//   H1   src/RegularizationH1.cpp:427-453, H2 src/RegularizationH2.cpp:426-450,
+//   H1SN src/RegularizationH1SN.cpp:406-430, H2SN src/RegularizationH2SN.cpp:435-460
+template <class T>
+void reg_eigvals_inv(const Grid<T>& g, int norm, const T* beta, T* emin, T* emax) {
+    T w[3];
+    for (int i = 0; i < 3; ++i) w[i] = static_cast<T>(g.n[i]) / T(2);
+    T regop = -(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    switch (norm) {
+        case kH1:   regop = beta[0] * (-regop + beta[1]); *emin = T(1) / regop; *emax = T(1) / beta[1]; break;
+        case kH2:   regop = beta[0] * (regop * regop) + beta[1]; *emin = T(1) / regop; *emax = T(1) / beta[1]; break;
+        case kH1SN: regop = -beta[0] * regop + beta[1]; *emin = T(1) / regop; *emax = T(1) / beta[1]; break;
+        case kH2SN: regop = beta[0] * (regop * regop); *emin = T(1) / regop; *emax = T(1) / beta[0]; break;
+        default: throw std::invalid_argument("unknown regularization norm");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Semi-Lagrangian interpolation.
+// ---------------------------------------------------------------------------
+// Periodic wrap of normalized query coordinates into (−1/N, 1)
+// (Interp3_Plan::fast_scatter, src/Interp3_Plan.cpp:328-362).
+template <class T>
+inline T wrap_coord(T q, T h) {
+    int guard = 0;
+    while (q <= -h && guard++ < 1024) q += T(1);
+    guard = 0;
+    while (q >= T(1) && guard++ < 1024) q -= T(1);
+    return q;
+}
+
+// Tricubic Lagrange interpolation of a periodic field at one normalized
+// query point.  Index space u = q·N + g with g = 3 ghost layers
+// (rescale_xyz, src/interp3.cpp:24-60); stencil base floor(u)−1 and weights
+// L_k(t) = c_k·Π_{l≠k}(t−l), c = (−1/6, 1/2, −1/2, 1/6)
+// (optimized_interp3_ghost_xyz_p, src/interp3.cpp:2508-2533); the ghost copy
+// of accfft_get_ghost_xyz (src/ghost.cpp:555-612) is replaced by modular
+// indexing (ghost index j maps to (j − g) mod N).
+template <class T>
+T interp_point(const T* f, const int* n, const T q[3]) {
+    const T lagr[4] = {static_cast<T>(-1.0 / 6.0), static_cast<T>(0.5), static_cast<T>(-0.5), static_cast<T>(1.0 / 6.0)};
+    int idx[3][4];
+    T M[3][4];
+    for (int j = 0; j < 3; ++j) {
+        const T hh = static_cast<T>(1.0 / n[j]);
+        const T qq = wrap_coord(q[j], hh);
+        T u = qq * static_cast<T>(n[j]) + static_cast<T>(3);
+        const T fu = std::floor(u);
+        int gi = std::isfinite(fu) ? static_cast<int>(fu) - 1 : 0;
+        const T x = u - static_cast<T>(gi);
+        for (int k = 0; k < 4; ++k) {
+            T m = lagr[k];
+            for (int l = 0; l < 4; ++l)
+                if (k != l) m *= (x - static_cast<T>(l));
+            M[j][k] = m;
+            long ii = static_cast<long>(gi) - 3 + k;
+            ii %= n[j];
+            if (ii < 0) ii += n[j];
+            idx[j][k] = static_cast<int>(ii);
+        }
+    }
+    T val = T(0);
+    for (int a = 0; a < 4; ++a) {
+        T sa = T(0);
+        for (int b = 0; b < 4; ++b) {
+            const T* row = f + (static_cast<int64_t>(idx[0][a]) * n[1] + idx[1][b]) * n[2];
+            T r = T(0);
+            for (int c = 0; c < 4; ++c) r += M[2][c] * row[idx[2][c]];
+            sa += M[1][b] * r;
+        }
+        val += M[0][a] * sa;
+    }
+    return val;
+}
+
+// Interpolate a scalar field at interleaved normalized query points X
+// (SemiLagrangian::Interpolate, src/SemiLagrangian.cpp:536-594).
+template <class T>
+void interp(const Grid<T>& g, const T* f, const T* X, T* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < g.nl; ++i) out[i] = interp_point(f, g.n, X + 3 * i);
+}
+
+// RK2 characteristic (SemiLagrangian::ComputeTrajectoryRK2,
+// src/SemiLagrangian.cpp:192-288).  sigma = +1 state, −1 adjoint.  Output
+// X interleaved, normalized by 2π.
+template <class T>
+void trajectory_rk2(const Grid<T>& g, const T* const v[3], T ht, int sigma, T* X) {
+    const T s = static_cast<T>(sigma);
+    const T hthalf = static_cast<T>(0.5) * ht;
+    const int n1 = g.n[1], n2 = g.n[2];
+#pragma omp parallel for schedule(static)
+    for (int64_t l = 0; l < g.nl; ++l) {
+        const int i1 = static_cast<int>(l / (static_cast<int64_t>(n1) * n2));
+        const int i2 = static_cast<int>((l / n2) % n1);
+        const int i3 = static_cast<int>(l % n2);
+        const T x[3] = {g.hx[0] * static_cast<T>(i1), g.hx[1] * static_cast<T>(i2), g.hx[2] * static_cast<T>(i3)};
+        for (int d = 0; d < 3; ++d)
+            X[3 * l + d] = static_cast<T>(static_cast<double>(x[d] - s * ht * v[d][l]) / (2.0 * kPi));
+    }
+    std::vector<T> vx(g.nl);
+    std::vector<T> out(3 * g.nl);
+    for (int d = 0; d < 3; ++d) {
+        interp(g, v[d], X, vx.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t l = 0; l < g.nl; ++l) out[3 * l + d] = vx[l];
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t l = 0; l < g.nl; ++l) {
+        const int i1 = static_cast<int>(l / (static_cast<int64_t>(n1) * n2));
+        const int i2 = static_cast<int>((l / n2) % n1);
+        const int i3 = static_cast<int>(l % n2);
+        const T x[3] = {g.hx[0] * static_cast<T>(i1), g.hx[1] * static_cast<T>(i2), g.hx[2] * static_cast<T>(i3)};
+        for (int d = 0; d < 3; ++d)
+            X[3 * l + d] = static_cast<T>(static_cast<double>(x[d] - s * hthalf * (out[3 * l + d] + v[d][l])) / (2.0 * kPi));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transport solves (GN, SL, nc = 1).
+// ---------------------------------------------------------------------------
+template <class T>
+bool velocity_is_zero(const Grid<T>& g, const T* const v[3]) {
+    // CLAIREBase::IsVelocityZero, src/CLAIREBase.cpp:578-605 (∞-norm == 0)
+    for (int d = 0; d < 3; ++d)
+        for (int64_t i = 0; i < g.nl; ++i)
+            if (v[d][i] != T(0)) return false;
+    return true;
+}
+
+// State equation m^{j+1} = m^j(X_s), m^0 = mT; v = 0 copies
+// (CLAIRE::SolveStateEquation src/CLAIRE.cpp:1896-1920,
+//  SolveStateEquationSL src/CLAIRE.cpp:2116-2170).  m has (nt+1)·nl entries.
+template <class T>
+void solve_state(const Grid<T>& g, int nt, const T* const v[3], const T* mT, T* m) {
+    std::copy(mT, mT + g.nl, m);
+    if (velocity_is_zero(g, v)) {
+        for (int j = 1; j <= nt; ++j) std::copy(mT, mT + g.nl, m + j * g.nl);
+        return;
+    }
+    const T ht = T(1) / static_cast<T>(nt);
+    std::vector<T> X(3 * g.nl);
+    trajectory_rk2(g, v, ht, +1, X.data());
+    for (int j = 0; j < nt; ++j) interp(g, m + j * g.nl, X.data(), m + (j + 1) * g.nl);
+}
+
+// SL2 distance ½·hd·Σ(mR − m^nt)² (DistanceMeasureSL2::EvaluateFunctional,
+// src/DistanceMeasureSL2.cpp:93-150).
+template <class T>
+double distance_sl2(const Grid<T>& g, int nt, const T* m, const T* mR) {
+    const T* m1 = m + static_cast<int64_t>(nt) * g.nl;
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int64_t i = 0; i < g.nl; ++i) {
+        const double dr = static_cast<double>(static_cast<T>(mR[i] - m1[i]));
+        s += dr * dr;
+    }
+    return 0.5 * static_cast<double>(g.hd) * s;
+}
+
+// Adjoint equation with body force, scaled by hd.  λ¹ = mR − m^nt
+// (SetFinalConditionAE, src/DistanceMeasureSL2.cpp:156-218);
+// SolveAdjointEquationSL src/CLAIRE.cpp:2550-2725; v = 0 branch
+// src/CLAIRE.cpp:2249-2312; ×hd src/CLAIRE.cpp:2341.
+// Outputs: lambda (value at t = 0, GN single slot) and b (3·nl).
+template <class T>
+void solve_adjoint(const Grid<T>& g, int nt, const T* const v[3], const T* m, const T* mR, T* lambda, T* const b[3]) {
+    const int64_t nl = g.nl;
+    const T* m1 = m + static_cast<int64_t>(nt) * nl;
+    for (int64_t i = 0; i < nl; ++i) lambda[i] = mR[i] - m1[i];
+    for (int d = 0; d < 3; ++d) std::fill(b[d], b[d] + nl, T(0));
+    std::vector<T> g1(nl), g2(nl), g3(nl);
+    if (velocity_is_zero(g, v)) {
+        grad(g, m, g1.data(), g2.data(), g3.data());
+        for (int64_t i = 0; i < nl; ++i) {
+            const T l = lambda[i];
+            b[0][i] += l * g1[i];
+            b[1][i] += l * g2[i];
+            b[2][i] += l * g3[i];
+        }
+    } else {
+        const T ht = T(1) / static_cast<T>(nt);
+        std::vector<T> X(3 * nl), divv(nl), divvx(nl), lx(nl);
+        trajectory_rk2(g, v, ht, -1, X.data());
+        div(g, v[0], v[1], v[2], divv.data());
+        interp(g, divv.data(), X.data(), divvx.data());
+        T scale = ht;
+        for (int j = 0; j < nt; ++j) {
+            const T* mj = m + static_cast<int64_t>(nt - j) * nl;
+            if (j == 0) scale *= static_cast<T>(0.5);
+            interp(g, lambda, X.data(), lx.data());
+            grad(g, mj, g1.data(), g2.data(), g3.data());
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < nl; ++i) {
+                const T lam = lambda[i];
+                const T lamx = lx[i];
+                const T rhs0 = lamx * divvx[i];
+                const T rhs1 = (lamx + ht * rhs0) * divv[i];
+                lambda[i] = static_cast<T>(lamx + 0.5 * ht * (rhs0 + rhs1));
+                b[0][i] += scale * g1[i] * lam;
+                b[1][i] += scale * g2[i] * lam;
+                b[2][i] += scale * g3[i] * lam;
+            }
+            if (j == 0) scale *= static_cast<T>(2.0);
+        }
+        grad(g, m, g1.data(), g2.data(), g3.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            const T lam = lambda[i];
+            b[0][i] += static_cast<T>(0.5 * scale * g1[i] * lam);
+            b[1][i] += static_cast<T>(0.5 * scale * g2[i] * lam);
+            b[2][i] += static_cast<T>(0.5 * scale * g3[i] * lam);
+        }
+    }
+    for (int d = 0; d < 3; ++d)
+        for (int64_t i = 0; i < nl; ++i) b[d][i] *= g.hd;
+}
+
+// Incremental state equation, GN, SL (SolveIncStateEquationSL,
+// src/CLAIRE.cpp:3147-3294): m̃⁰ = 0,
+//   m̃^{j+1} = m̃^j(X_s) − ht/2·[(∇m^j)(X_s)·ṽ(X_s)] − ht/2·[∇m^{j+1}·ṽ].
+// There is no v = 0 branch on the SL path (src/CLAIRE.cpp:2866-2950).
+template <class T>
+void solve_inc_state_x(const Grid<T>& g, int nt, const T* X, const T* m, const T* const vt[3], T* mtilde) {
+    const int64_t nl = g.nl;
+    const T ht = T(1) / static_cast<T>(nt);
+    const T hthalf = static_cast<T>(0.5) * ht;
+    std::vector<T> vtx0(nl), vtx1(nl), vtx2(nl), g1(nl), g2(nl), g3(nl), gx1(nl), gx2(nl), gx3(nl), tmp(nl);
+    interp(g, vt[0], X, vtx0.data());
+    interp(g, vt[1], X, vtx1.data());
+    interp(g, vt[2], X, vtx2.data());
+    std::fill(mtilde, mtilde + nl, T(0));
+    for (int j = 0; j < nt; ++j) {
+        interp(g, mtilde, X, tmp.data());
+        std::copy(tmp.begin(), tmp.end(), mtilde);
+        grad(g, m + static_cast<int64_t>(j) * nl, g1.data(), g2.data(), g3.data());
+        interp(g, g1.data(), X, gx1.data());
+        interp(g, g2.data(), X, gx2.data());
+        interp(g, g3.data(), X, gx3.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i)
+            mtilde[i] -= hthalf * (gx1[i] * vtx0[i] + gx2[i] * vtx1[i] + gx3[i] * vtx2[i]);
+        grad(g, m + static_cast<int64_t>(j + 1) * nl, g1.data(), g2.data(), g3.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i)
+            mtilde[i] -= hthalf * (g1[i] * vt[0][i] + g2[i] * vt[1][i] + g3[i] * vt[2][i]);
+    }
+}
+
+template <class T>
+void solve_inc_state(const Grid<T>& g, int nt, const T* const v[3], const T* m, const T* const vt[3], T* mtilde) {
+    std::vector<T> X(3 * g.nl);
+    trajectory_rk2(g, v, T(1) / static_cast<T>(nt), +1, X.data());
+    solve_inc_state_x(g, nt, X.data(), m, vt, mtilde);
+}
+
+// Incremental adjoint equation, GN, SL, with the incremental body force
+// scaled by hd (SolveIncAdjointEquation src/CLAIRE.cpp:3307-3491,
+// SolveIncAdjointEquationGNSL src/CLAIRE.cpp:3849-4015, final condition
+// λ̃ = −m̃ src/DistanceMeasureSL2.cpp:223-276, v = 0 branch
+// src/CLAIRE.cpp:3367-3421, ×hd src/CLAIRE.cpp:3471).
+template <class T>
+void solve_inc_adjoint_x(const Grid<T>& g, int nt, const T* const v[3], const T* Xa, const T* m, const T* mtilde,
+                         T* ltilde, T* const bt[3]) {
+    const int64_t nl = g.nl;
+    for (int64_t i = 0; i < nl; ++i) ltilde[i] = -mtilde[i];
+    for (int d = 0; d < 3; ++d) std::fill(bt[d], bt[d] + nl, T(0));
+    std::vector<T> g1(nl), g2(nl), g3(nl);
+    if (velocity_is_zero(g, v)) {
+        grad(g, m, g1.data(), g2.data(), g3.data());
+        for (int64_t i = 0; i < nl; ++i) {
+            const T l = ltilde[i];
+            bt[0][i] += l * g1[i];
+            bt[1][i] += l * g2[i];
+            bt[2][i] += l * g3[i];
+        }
+    } else {
+        const T ht = T(1) / static_cast<T>(nt);
+        const T hthalf = static_cast<T>(0.5) * ht;
+        std::vector<T> divv(nl), divvx(nl), lx(nl);
+        const T* X = Xa;
+        div(g, v[0], v[1], v[2], divv.data());
+        interp(g, divv.data(), X, divvx.data());
+        T scale = ht;
+        for (int j = 0; j < nt; ++j) {
+            const T* mj = m + static_cast<int64_t>(nt - j) * nl;
+            if (j == 0) scale *= static_cast<T>(0.5);
+            interp(g, ltilde, X, lx.data());
+            grad(g, mj, g1.data(), g2.data(), g3.data());
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < nl; ++i) {
+                const T lt = ltilde[i];
+                const T ltx = lx[i];
+                const T rhs0 = ltx * divvx[i];
+                const T rhs1 = (ltx + ht * rhs0) * divv[i];
+                ltilde[i] = ltx + hthalf * (rhs0 + rhs1);
+                bt[0][i] += scale * g1[i] * lt;
+                bt[1][i] += scale * g2[i] * lt;
+                bt[2][i] += scale * g3[i] * lt;
+            }
+            if (j == 0) scale *= static_cast<T>(2.0);
+        }
+        grad(g, m, g1.data(), g2.data(), g3.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            const T lt = ltilde[i];
+            bt[0][i] += static_cast<T>(0.5 * scale * g1[i] * lt);
+            bt[1][i] += static_cast<T>(0.5 * scale * g2[i] * lt);
+            bt[2][i] += static_cast<T>(0.5 * scale * g3[i] * lt);
+        }
+    }
+    for (int d = 0; d < 3; ++d)
+        for (int64_t i = 0; i < nl; ++i) bt[d][i] *= g.hd;
+}
+
+template <class T>
+void solve_inc_adjoint(const Grid<T>& g, int nt, const T* const v[3], const T* m, const T* mtilde, T* ltilde,
+                       T* const bt[3]) {
+    std::vector<T> X;
+    if (!velocity_is_zero(g, v)) {
+        X.resize(3 * g.nl);
+        trajectory_rk2(g, v, T(1) / static_cast<T>(nt), -1, X.data());
+    }
+    solve_inc_adjoint_x(g, nt, v, X.data(), m, mtilde, ltilde, bt);
+}
+
+// GN Hessian matvec Hṽ = R''ṽ + b̃ (CLAIRE::HessMatVec src/CLAIRE.cpp:1311-1379),
+// optionally unscaled by 1/hd (CLAIRE::HessianMatVec src/CLAIRE.cpp:1283-1286).
+// The trajectories X_s, X_a are computed once per velocity (by the state and
+// adjoint solves in the reference, src/CLAIRE.cpp:2149,2607) and reused.
+template <class T>
+void hessian_matvec_x(const Grid<T>& g, int nt, int norm, const T* beta, const T* const v[3], const T* Xs,
+                      const T* Xa, const T* m, const T* const vt[3], T* const hv[3], bool scale) {
+    const int64_t nl = g.nl;
+    std::vector<T> mt(nl), lt(nl), b0(nl), b1(nl), b2(nl);
+    T* bt[3] = {b0.data(), b1.data(), b2.data()};
+    solve_inc_state_x(g, nt, Xs, m, vt, mt.data());
+    solve_inc_adjoint_x(g, nt, v, Xa, m, mt.data(), lt.data(), bt);
+    reg_apply(g, norm, beta, vt, hv);
+    for (int d = 0; d < 3; ++d)
+        for (int64_t i = 0; i < nl; ++i) hv[d][i] += bt[d][i];
+    if (!scale) {
+        for (int d = 0; d < 3; ++d)
+            for (int64_t i = 0; i < nl; ++i) hv[d][i] *= T(1) / g.hd;
+    }
+}
+
+template <class T>
+void hessian_matvec(const Grid<T>& g, int nt, int norm, const T* beta, const T* const v[3], const T* m,
+                    const T* const vt[3], T* const hv[3], bool scale) {
+    const T ht = T(1) / static_cast<T>(nt);
+    std::vector<T> Xs(3 * g.nl), Xa(3 * g.nl);
+    trajectory_rk2(g, v, ht, +1, Xs.data());
+    trajectory_rk2(g, v, ht, -1, Xa.data());
+    hessian_matvec_x(g, nt, norm, beta, v, Xs.data(), Xa.data(), m, vt, hv, scale);
+}
+
+// Reduced L2 gradient g = R'(v) + b (CLAIRE::EvaluateGradient
+// src/CLAIRE.cpp:871-1008, EvaluateL2Gradient src/CLAIRE.cpp:1015-1042);
+// with v = 0 the gradient is b (src/CLAIRE.cpp:960-965).  Runs the state and
+// adjoint solves.
+template <class T>
+void gradient(const Grid<T>& g, int nt, int norm, const T* beta, const T* const v[3], const T* mT, const T* mR,
+              T* const gout[3]) {
+    const int64_t nl = g.nl;
+    std::vector<T> m((nt + 1) * nl), lam(nl), b0(nl), b1(nl), b2(nl);
+    T* b[3] = {b0.data(), b1.data(), b2.data()};
+    solve_state(g, nt, v, mT, m.data());
+    solve_adjoint(g, nt, v, m.data(), mR, lam.data(), b);
+    if (velocity_is_zero(g, v)) {
+        for (int d = 0; d < 3; ++d) std::copy(b[d], b[d] + nl, gout[d]);
+        return;
+    }
+    reg_apply(g, norm, beta, v, gout);
+    for (int d = 0; d < 3; ++d)
+        for (int64_t i = 0; i < nl; ++i) gout[d][i] += b[d][i];
+}
+
+// Objective J = D + R (CLAIRE::EvaluateObjective src/CLAIRE.cpp:793-865;
+// R skipped when v = 0).
+template <class T>
+void objective(const Grid<T>& g, int nt, int norm, const T* beta, const T* const v[3], const T* mT, const T* mR,
+               double* J, double* D, double* R) {
+    std::vector<T> m((nt + 1) * g.nl);
+    solve_state(g, nt, v, mT, m.data());
+    *D = distance_sl2(g, nt, m.data(), mR);
+    *R = velocity_is_zero(g, v) ? 0.0 : reg_functional(g, norm, beta, v);
+    *J = *D + *R;
+}
+
+}  // namespace oracle

@@ -1,0 +1,238 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+benchmark's CPU-baseline leg, always as the checker.  Product code never
+imports this module.
+
+Every function takes numpy arrays in the reference's layouts:
+  scalar field       (n1, n2, n3), x3 fastest
+  vector field       (3, n1, n2, n3)  (VecField SoA, include/VecField.hpp:93-95)
+  time series        (nt+1, n1, n2, n3)
+  half spectrum      (n1, n2, n3//2+1) complex
+  query points       (n1, n2, n3, 3) normalized by 2π, interleaved
+and a precision flag: 'f64' (accuracy oracle) or 'f32' (reference CPU path).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+REG = {"h1": 0, "h2": 1, "h1sn": 2, "h2sn": 3}
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.oracle_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def set_num_threads(n: int) -> None:
+    lib().oracle_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(lib().oracle_num_threads())
+
+
+def _dt(prec):
+    return np.float64 if prec == "f64" else np.float32
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _nx(shape):
+    return (ctypes.c_int * 3)(*[int(s) for s in shape])
+
+
+def _beta(beta):
+    b = list(beta) + [0.0] * (3 - len(beta))
+    return (ctypes.c_double * 3)(*b[:3])
+
+
+def _call(name, prec, *args):
+    fn = getattr(lib(), f"{name}_{prec}")
+    rc = fn(*args)
+    if rc != 0:
+        raise RuntimeError(f"{name}_{prec}: {lib().oracle_last_error().decode()}")
+
+
+def _c(a, prec):
+    return np.ascontiguousarray(a, dtype=_dt(prec))
+
+
+def fft1d(x, sign=-1, prec="f64"):
+    dt = _dt(prec)
+    x = np.ascontiguousarray(x, dtype=np.complex128 if prec == "f64" else np.complex64)
+    out = np.empty_like(x)
+    _call("oracle_fft1d", prec, ctypes.c_int(x.size), _p(x.view(dt)), _p(out.view(dt)), ctypes.c_int(sign))
+    return out
+
+
+def r2c(f, prec="f64"):
+    f = _c(f, prec)
+    n = f.shape
+    ct = np.complex128 if prec == "f64" else np.complex64
+    out = np.empty((n[0], n[1], n[2] // 2 + 1), dtype=ct)
+    _call("oracle_r2c", prec, _nx(n), _p(f), _p(out))
+    return out
+
+
+def c2r(fh, n, prec="f64"):
+    ct = np.complex128 if prec == "f64" else np.complex64
+    fh = np.ascontiguousarray(fh, dtype=ct)
+    out = np.empty(tuple(n), dtype=_dt(prec))
+    _call("oracle_c2r", prec, _nx(n), _p(fh), _p(out))
+    return out
+
+
+def grad(f, prec="f64"):
+    f = _c(f, prec)
+    out = np.empty((3,) + f.shape, dtype=_dt(prec))
+    _call("oracle_grad", prec, _nx(f.shape), _p(f), _p(out[0]), _p(out[1]), _p(out[2]))
+    return out
+
+
+def div(v, prec="f64"):
+    v = _c(v, prec)
+    out = np.empty(v.shape[1:], dtype=_dt(prec))
+    _call("oracle_div", prec, _nx(v.shape[1:]), _p(v[0]), _p(v[1]), _p(v[2]), _p(out))
+    return out
+
+
+def reg_apply(v, regnorm, beta, prec="f64"):
+    v = _c(v, prec)
+    out = np.empty_like(v)
+    _call("oracle_reg_apply", prec, _nx(v.shape[1:]), ctypes.c_int(REG[regnorm]), _beta(beta),
+          _p(v[0]), _p(v[1]), _p(v[2]), _p(out[0]), _p(out[1]), _p(out[2]))
+    return out
+
+
+def reg_inverse(v, regnorm, beta, applysqrt=False, prec="f64"):
+    v = _c(v, prec)
+    out = np.empty_like(v)
+    _call("oracle_reg_inverse", prec, _nx(v.shape[1:]), ctypes.c_int(REG[regnorm]), _beta(beta),
+          _p(v[0]), _p(v[1]), _p(v[2]), _p(out[0]), _p(out[1]), _p(out[2]), ctypes.c_int(int(applysqrt)))
+    return out
+
+
+def reg_functional(v, regnorm, beta, prec="f64"):
+    v = _c(v, prec)
+    r = ctypes.c_double()
+    _call("oracle_reg_functional", prec, _nx(v.shape[1:]), ctypes.c_int(REG[regnorm]), _beta(beta),
+          _p(v[0]), _p(v[1]), _p(v[2]), ctypes.byref(r))
+    return r.value
+
+
+def reg_eigvals_inv(n, regnorm, beta, prec="f64"):
+    lo, hi = ctypes.c_double(), ctypes.c_double()
+    _call("oracle_reg_eigvals_inv", prec, _nx(n), ctypes.c_int(REG[regnorm]), _beta(beta),
+          ctypes.byref(lo), ctypes.byref(hi))
+    return lo.value, hi.value
+
+
+def interp(f, X, prec="f64"):
+    """Interpolate f at query points X of shape (..., 3) (normalized by 2π)."""
+    f = _c(f, prec)
+    X = _c(X, prec)
+    npts = X.size // 3
+    out = np.empty(X.shape[:-1], dtype=_dt(prec))
+    _call("oracle_interp_points", prec, _nx(f.shape), _p(f), ctypes.c_int64(npts), _p(X), _p(out))
+    return out
+
+
+def trajectory(v, nt, sigma, prec="f64"):
+    v = _c(v, prec)
+    n = v.shape[1:]
+    X = np.empty(tuple(n) + (3,), dtype=_dt(prec))
+    _call("oracle_trajectory", prec, _nx(n), ctypes.c_int(nt), _p(v[0]), _p(v[1]), _p(v[2]),
+          ctypes.c_int(sigma), _p(X))
+    return X
+
+
+def solve_state(v, mT, nt, prec="f64"):
+    v = _c(v, prec)
+    mT = _c(mT, prec)
+    m = np.empty((nt + 1,) + mT.shape, dtype=_dt(prec))
+    _call("oracle_solve_state", prec, _nx(mT.shape), ctypes.c_int(nt), _p(v[0]), _p(v[1]), _p(v[2]),
+          _p(mT), _p(m))
+    return m
+
+
+def solve_adjoint(v, m, mR, nt, prec="f64"):
+    v, m, mR = _c(v, prec), _c(m, prec), _c(mR, prec)
+    lam = np.empty(mR.shape, dtype=_dt(prec))
+    b = np.empty((3,) + mR.shape, dtype=_dt(prec))
+    _call("oracle_solve_adjoint", prec, _nx(mR.shape), ctypes.c_int(nt), _p(v[0]), _p(v[1]), _p(v[2]),
+          _p(m), _p(mR), _p(lam), _p(b[0]), _p(b[1]), _p(b[2]))
+    return lam, b
+
+
+def solve_inc_state(v, m, vt, nt, prec="f64"):
+    v, m, vt = _c(v, prec), _c(m, prec), _c(vt, prec)
+    mt = np.empty(v.shape[1:], dtype=_dt(prec))
+    _call("oracle_solve_inc_state", prec, _nx(v.shape[1:]), ctypes.c_int(nt), _p(v[0]), _p(v[1]), _p(v[2]),
+          _p(m), _p(vt[0]), _p(vt[1]), _p(vt[2]), _p(mt))
+    return mt
+
+
+def solve_inc_adjoint(v, m, mt, nt, prec="f64"):
+    v, m, mt = _c(v, prec), _c(m, prec), _c(mt, prec)
+    lt = np.empty(mt.shape, dtype=_dt(prec))
+    b = np.empty((3,) + mt.shape, dtype=_dt(prec))
+    _call("oracle_solve_inc_adjoint", prec, _nx(mt.shape), ctypes.c_int(nt), _p(v[0]), _p(v[1]), _p(v[2]),
+          _p(m), _p(mt), _p(lt), _p(b[0]), _p(b[1]), _p(b[2]))
+    return lt, b
+
+
+def hessian_matvec(v, m, vt, nt, regnorm, beta, scale=True, prec="f64"):
+    v, m, vt = _c(v, prec), _c(m, prec), _c(vt, prec)
+    h = np.empty_like(vt)
+    _call("oracle_hessian_matvec", prec, _nx(v.shape[1:]), ctypes.c_int(nt), ctypes.c_int(REG[regnorm]),
+          _beta(beta), _p(v[0]), _p(v[1]), _p(v[2]), _p(m), _p(vt[0]), _p(vt[1]), _p(vt[2]),
+          _p(h[0]), _p(h[1]), _p(h[2]), ctypes.c_int(int(scale)))
+    return h
+
+
+def gradient(v, mT, mR, nt, regnorm, beta, prec="f64"):
+    v, mT, mR = _c(v, prec), _c(mT, prec), _c(mR, prec)
+    g = np.empty_like(v)
+    _call("oracle_gradient", prec, _nx(mT.shape), ctypes.c_int(nt), ctypes.c_int(REG[regnorm]), _beta(beta),
+          _p(v[0]), _p(v[1]), _p(v[2]), _p(mT), _p(mR), _p(g[0]), _p(g[1]), _p(g[2]))
+    return g
+
+
+def objective(v, mT, mR, nt, regnorm, beta, prec="f64"):
+    v, mT, mR = _c(v, prec), _c(mT, prec), _c(mR, prec)
+    J, D, R = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _call("oracle_objective", prec, _nx(mT.shape), ctypes.c_int(nt), ctypes.c_int(REG[regnorm]), _beta(beta),
+          _p(v[0]), _p(v[1]), _p(v[2]), _p(mT), _p(mR), ctypes.byref(J), ctypes.byref(D), ctypes.byref(R))
+    return J.value, D.value, R.value
+
+
+def hessian_bench(v, m, vt, nt, regnorm, beta, reps=1, prec="f32"):
+    """Time `reps` Hessian matvecs of the CPU reference path; returns (H, seconds)."""
+    v, m, vt = _c(v, prec), _c(m, prec), _c(vt, prec)
+    h = np.empty_like(vt)
+    sec = ctypes.c_double()
+    _call("oracle_hessian_bench", prec, _nx(v.shape[1:]), ctypes.c_int(nt), ctypes.c_int(REG[regnorm]), _beta(beta),
+          _p(v[0]), _p(v[1]), _p(v[2]), _p(m), _p(vt[0]), _p(vt[1]), _p(vt[2]), _p(h[0]), _p(h[1]), _p(h[2]),
+          ctypes.c_int(reps), ctypes.byref(sec))
+    return h, sec.value

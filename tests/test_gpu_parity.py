@@ -1,0 +1,235 @@
+"""GPU parity: every C-ABI entry point of libclaire_amd.so against the oracle.
+
+Bars (stated per test): plain FP32 outputs within 1e-5 relative L2 of the
+oracle's FP32 reference path (the reference's single-precision build
+restated) and of the FP64 oracle unless the quantity is ill-conditioned;
+trajectories within a few FP32 ulps.
+"""
+import numpy as np
+import pytest
+
+from claire_amd import ADJOINT, STATE, Claire, synth
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 16, 16), (32, 16, 64)]
+NORMS = ["h1", "h2", "h1sn", "h2sn"]
+BETA = (1e-2, 1e-4, 0.0)
+NT = 4
+
+
+def _data(shape, seed=0, cfl=2.0):
+    v = synth.bandlimited_velocity(shape, kmax=min(8, min(shape) // 4), cfl=cfl, nt=NT, seed=seed)
+    mT = synth.gaussian_blobs(shape, count=8, seed=seed + 1)
+    mR = synth.gaussian_blobs(shape, count=8, seed=seed + 2)
+    vt = synth.bandlimited_velocity(shape, kmax=4, cfl=1.0, nt=NT, seed=seed + 3)
+    return v, mT, mR, vt
+
+
+@pytest.fixture(scope="module", params=SHAPES, ids=lambda s: "x".join(map(str, s)))
+def case(request):
+    shape = request.param
+    print(f"\n[gpu case] shape={shape}", flush=True)
+    v, mT, mR, vt = _data(shape)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA)
+    c.set_velocity(v)
+    c.set_template_image(mT)
+    c.set_reference_image(mR)
+    yield shape, c, v, mT, mR, vt
+    c.close()
+
+
+def test_fft_r2c_c2r(case):
+    shape, c, v, mT, mR, vt = case
+    fh = c.fft_r2c(mT)
+    assert relerr(fh, np.fft.rfftn(mT.astype(np.float64))) < 2e-6
+    back = c.fft_c2r(np.fft.rfftn(mT.astype(np.float64)))
+    assert relerr(back, mT.astype(np.float64) * mT.size) < 2e-6
+
+
+def test_grad_div(case, oracle):
+    shape, c, v, mT, mR, vt = case
+    g = c.grad(mT)
+    assert relerr(g, oracle.grad(mT, prec="f32")) < 1e-5
+    assert relerr(g, oracle.grad(mT)) < 1e-5
+    d = c.div(v)
+    assert relerr(d, oracle.div(v)) < 1e-5
+
+
+@pytest.mark.parametrize("norm", NORMS)
+def test_regularization_ops(norm, oracle):
+    shape = (32, 16, 64)
+    print(f"\n[gpu case] reg {norm} shape={shape}", flush=True)
+    _, _, _, vt = _data(shape, seed=5)
+    c = Claire(shape, nt=NT, regnorm=norm, beta=BETA)
+    try:
+        assert relerr(c.reg_apply(vt), oracle.reg_apply(vt, norm, BETA)) < 1e-5
+        assert relerr(c.apply_inv_reg(vt), oracle.reg_inverse(vt, norm, BETA)) < 1e-5
+        assert relerr(c.apply_inv_reg(vt, applysqrt=True), oracle.reg_inverse(vt, norm, BETA, applysqrt=True)) < 1e-5
+        r = c.reg_functional(vt)
+        assert abs(r - oracle.reg_functional(vt, norm, BETA)) < 1e-5 * abs(r)
+        lo, hi = c.reg_eigvals_inv()
+        elo, ehi = oracle.reg_eigvals_inv(shape, norm, BETA, prec="f32")
+        assert abs(lo - elo) <= 1e-6 * abs(elo) and abs(hi - ehi) <= 1e-6 * abs(ehi)
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("flag", [STATE, ADJOINT])
+def test_trajectory(case, flag, oracle):
+    shape, c, v, mT, mR, vt = case
+    X = c.trajectory(flag)
+    Xo = oracle.trajectory(v, NT, +1 if flag == STATE else -1, prec="f32")
+    # normalized coordinates in [−0.1, 1.1]: a few FP32 ulps
+    assert np.abs(X - Xo).max() < 1e-6
+    X64 = oracle.trajectory(v, NT, +1 if flag == STATE else -1)
+    assert np.abs(X - X64).max() < 2e-6
+
+
+def test_interp_points_and_along_trajectory(case, oracle):
+    shape, c, v, mT, mR, vt = case
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-0.05, 1.05, shape + (3,)).astype(np.float32)
+    got = c.interp_points(mT, X)
+    assert relerr(got, oracle.interp(mT, X, prec="f32")) < 1e-6
+    assert relerr(got, oracle.interp(mT.astype(np.float64), X.astype(np.float64))) < 1e-6
+    Xs = oracle.trajectory(v, NT, +1, prec="f32")
+    assert relerr(c.interp_scalar(mT, STATE), oracle.interp(mT, Xs, prec="f32")) < 1e-5
+    Xa = oracle.trajectory(v, NT, -1, prec="f32")
+    vv = c.interp_vector(vt, ADJOINT)
+    for d in range(3):
+        assert relerr(vv[d], oracle.interp(vt[d], Xa, prec="f32")) < 1e-5
+
+
+def test_state_equation(case, oracle):
+    shape, c, v, mT, mR, vt = case
+    m = c.solve_state()
+    assert relerr(m, oracle.solve_state(v, mT, NT, prec="f32")) < 1e-5
+    assert relerr(m, oracle.solve_state(v, mT, NT)) < 1e-5
+
+
+def test_adjoint_equation(case, oracle):
+    shape, c, v, mT, mR, vt = case
+    lam, b = c.solve_adjoint()
+    m32 = oracle.solve_state(v, mT, NT, prec="f32")
+    lo, bo = oracle.solve_adjoint(v, m32, mR, NT, prec="f32")
+    assert relerr(lam, lo) < 1e-5
+    assert relerr(b, bo) < 1e-5
+    m64 = oracle.solve_state(v, mT, NT)
+    l64, b64 = oracle.solve_adjoint(v, m64, mR, NT)
+    assert relerr(b, b64) < 1e-4
+
+
+def test_incremental_state_and_adjoint(case, oracle):
+    shape, c, v, mT, mR, vt = case
+    c.solve_state()
+    mt = c.solve_inc_state(vt)
+    m32 = oracle.solve_state(v, mT, NT, prec="f32")
+    mto = oracle.solve_inc_state(v, m32, vt, NT, prec="f32")
+    assert relerr(mt, mto) < 1e-5
+    lt, bt = c.solve_inc_adjoint(mto)
+    lto, bto = oracle.solve_inc_adjoint(v, m32, mto, NT, prec="f32")
+    assert relerr(lt, lto) < 1e-5
+    assert relerr(bt, bto) < 1e-5
+
+
+@pytest.mark.parametrize("scale", [True, False])
+def test_hessian_matvec(case, scale, oracle):
+    shape, c, v, mT, mR, vt = case
+    c.solve_state()
+    H = c.hessian_matvec(vt, scale=scale)
+    m32 = oracle.solve_state(v, mT, NT, prec="f32")
+    Ho = oracle.hessian_matvec(v, m32, vt, NT, "h1sn", BETA, scale=scale, prec="f32")
+    assert relerr(H, Ho) < 1e-5
+    m64 = oracle.solve_state(v, mT, NT)
+    H64 = oracle.hessian_matvec(v, m64, vt, NT, "h1sn", BETA, scale=scale)
+    assert relerr(H, H64) < 1e-4
+    # repeated application gives the same result (caches are reused)
+    assert np.array_equal(c.hessian_matvec(vt, scale=scale), H)
+
+
+@pytest.mark.parametrize("norm", ["h1", "h2", "h2sn"])
+def test_hessian_other_norms(norm, oracle):
+    shape = (32, 32, 32)
+    print(f"\n[gpu case] hessian {norm} shape={shape}", flush=True)
+    v, mT, mR, vt = _data(shape, seed=20)
+    c = Claire(shape, nt=NT, regnorm=norm, beta=(1e-3, 1e-4, 0.0))
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        H = c.hessian_matvec(vt)
+        m32 = oracle.solve_state(v, mT, NT, prec="f32")
+        Ho = oracle.hessian_matvec(v, m32, vt, NT, norm, (1e-3, 1e-4, 0.0), prec="f32")
+        assert relerr(H, Ho) < 1e-5
+    finally:
+        c.close()
+
+
+def test_objective_and_gradient(case, oracle):
+    shape, c, v, mT, mR, vt = case
+    J, D, R = c.evaluate_objective(v)
+    Jo, Do, Ro = oracle.objective(v, mT, mR, NT, "h1sn", BETA, prec="f32")
+    assert abs(D - Do) < 1e-5 * abs(Do) and abs(R - Ro) < 1e-5 * abs(Ro)
+    g = c.evaluate_gradient(v)
+    go = oracle.gradient(v, mT, mR, NT, "h1sn", BETA, prec="f32")
+    assert relerr(g, go) < 1e-5
+
+
+def test_zero_velocity_branches(oracle):
+    shape = (16, 32, 16)
+    print(f"\n[gpu case] v=0 shape={shape}", flush=True)
+    _, mT, mR, vt = _data(shape, seed=30)
+    v = np.zeros((3,) + shape, np.float32)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA)
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        c.set_reference_image(mR)
+        m = c.solve_state()
+        assert all(np.array_equal(m[j], mT) for j in range(NT + 1))
+        lam, b = c.solve_adjoint()
+        lo, bo = oracle.solve_adjoint(v, oracle.solve_state(v, mT, NT, prec="f32"), mR, NT, prec="f32")
+        assert relerr(lam, lo) < 1e-6 and relerr(b, bo) < 1e-5
+        H = c.hessian_matvec(vt)
+        Ho = oracle.hessian_matvec(v, oracle.solve_state(v, mT, NT, prec="f32"), vt, NT, "h1sn", BETA, prec="f32")
+        assert relerr(H, Ho) < 1e-5
+        g = c.evaluate_gradient(v)
+        assert relerr(g, bo) < 1e-5  # v = 0: gradient = body force (src/CLAIRE.cpp:960-965)
+        J, D, R = c.evaluate_objective(v)
+        assert R == 0.0
+    finally:
+        c.close()
+
+
+def test_errors_are_reported():
+    from claire_amd import ClaireError
+
+    with pytest.raises(ClaireError):
+        Claire((16, 16, 12))  # not a supported FFT size
+    c = Claire((16, 16, 16))
+    try:
+        with pytest.raises(ClaireError):
+            c.solve_state()  # no velocity / template yet
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("cfl", [2.0, 5.0])
+def test_lds_gather_matches_direct_gather(cfl):
+    """The LDS-staged gather (along a trajectory) and the direct global gather
+    (arbitrary points) evaluate the same stencil sum."""
+    shape = (64, 64, 64)
+    print(f"\n[gpu case] lds vs direct cfl={cfl}", flush=True)
+    v = synth.bandlimited_velocity(shape, kmax=6, cfl=cfl, nt=NT, seed=40)
+    f = synth.gaussian_blobs(shape, count=12, seed=41)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA)
+    try:
+        c.set_velocity(v)
+        for flag in (STATE, ADJOINT):
+            X = c.trajectory(flag)
+            a = c.interp_scalar(f, flag)
+            b = c.interp_points(f, X)
+            assert relerr(a, b) < 1e-6
+    finally:
+        c.close()

@@ -1,0 +1,186 @@
+"""Pins for the oracle's FFT and spectral operators (CPU only).
+
+Independent checks: numpy.fft (pocketfft), a brute-force DFT, and closed
+forms for derivatives of trigonometric polynomials with the Nyquist mode
+mapped to zero (include/CLAIREUtils.hpp:248-255).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 8, 12, 16, 28, 96, 128])
+def test_fft1d_matches_numpy(oracle, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert relerr(oracle.fft1d(x, -1), np.fft.fft(x)) < 1e-13
+    assert relerr(oracle.fft1d(x, +1), np.fft.ifft(x) * n) < 1e-13
+
+
+@pytest.mark.parametrize("shape", [(8, 12, 16), (16, 16, 16), (6, 10, 8), (8, 8, 9), (32, 8, 16)])
+def test_r2c_c2r_match_numpy(oracle, shape):
+    rng = np.random.default_rng(sum(shape))
+    f = rng.standard_normal(shape)
+    fh = oracle.r2c(f)
+    assert relerr(fh, np.fft.rfftn(f)) < 1e-13
+    back = oracle.c2r(np.fft.rfftn(f), shape)
+    assert relerr(back, f * f.size) < 1e-13
+    # FP32 twin
+    assert relerr(oracle.r2c(f, prec="f32"), np.fft.rfftn(f)) < 2e-6
+
+
+def test_r2c_brute_force_dft(oracle):
+    shape = (4, 6, 4)
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal(shape)
+    fh = oracle.r2c(f)
+    n1, n2, n3 = shape
+    for k1, k2, k3 in itertools.product(range(n1), range(n2), range(n3 // 2 + 1)):
+        acc = 0j
+        for i1, i2, i3 in itertools.product(range(n1), range(n2), range(n3)):
+            acc += f[i1, i2, i3] * np.exp(-2j * np.pi * (k1 * i1 / n1 + k2 * i2 / n2 + k3 * i3 / n3))
+        assert abs(fh[k1, k2, k3] - acc) < 1e-11
+
+
+def test_c2r_discards_imaginary_dc_and_nyquist(oracle):
+    shape = (8, 8, 8)
+    rng = np.random.default_rng(4)
+    fh = np.fft.rfftn(rng.standard_normal(shape))
+    g = fh.copy()
+    g[:, :, 0] += 1j * 0.0  # keep Hermitian planes consistent
+    g[0, 0, 0] += 5j
+    g[0, 0, 4] += 3j
+    assert relerr(oracle.c2r(g, shape), oracle.c2r(fh, shape)) < 1e-13
+
+
+def _grid(shape):
+    h = [2 * np.pi / n for n in shape]
+    return np.meshgrid(*[np.arange(n) * h[a] for a, n in enumerate(shape)], indexing="ij")
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 16, 32)])
+def test_grad_closed_form(oracle, shape):
+    x1, x2, x3 = _grid(shape)
+    f = np.sin(2 * x1) * np.cos(3 * x2) + np.cos(x3) * np.sin(x1)
+    g = oracle.grad(f)
+    exact = [2 * np.cos(2 * x1) * np.cos(3 * x2) + np.cos(x3) * np.cos(x1),
+             -3 * np.sin(2 * x1) * np.sin(3 * x2),
+             -np.sin(x3) * np.sin(x1)]
+    for d in range(3):
+        assert np.abs(g[d] - exact[d]).max() < 1e-12
+
+
+def test_grad_nyquist_is_zero(oracle):
+    shape = (8, 8, 8)
+    x1, x2, x3 = _grid(shape)
+    f = np.cos(4 * x1) + np.cos(4 * x2) * np.cos(x3) + np.cos(4 * x3)
+    g = oracle.grad(f)
+    # Nyquist wave numbers map to 0 (CheckWaveNumbers), so only ∂_3 of cos(4 x2)cos(x3) survives
+    assert np.abs(g[0]).max() < 1e-12
+    assert np.abs(g[1]).max() < 1e-12
+    assert np.abs(g[2] + np.cos(4 * x2) * np.sin(x3)).max() < 1e-12
+
+
+def test_div_closed_form(oracle):
+    shape = (16, 8, 16)
+    x1, x2, x3 = _grid(shape)
+    v = np.stack([np.sin(x1) * np.cos(x2), np.cos(2 * x2) * np.sin(x3), np.sin(3 * x3) + np.cos(x1)])
+    exact = np.cos(x1) * np.cos(x2) - 2 * np.sin(2 * x2) * np.sin(x3) + 3 * np.cos(3 * x3)
+    assert np.abs(oracle.div(v) - exact).max() < 1e-12
+
+
+def test_grad_div_fp32_twin(oracle):
+    from claire_amd import synth
+
+    f = synth.gaussian_blobs((32, 32, 32), count=6, seed=2).astype(np.float64)
+    g64 = oracle.grad(f)
+    g32 = oracle.grad(f, prec="f32")
+    assert relerr(g32, g64) < 1e-5
+    v = synth.bandlimited_velocity((32, 32, 32), seed=3).astype(np.float64)
+    assert relerr(oracle.div(v, prec="f32"), oracle.div(v)) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# Regularization operators: Fourier-mode eigenpairs.
+# ---------------------------------------------------------------------------
+def _mode(shape, w):
+    x1, x2, x3 = _grid(shape)
+    return np.cos(w[0] * x1 + w[1] * x2 + w[2] * x3)
+
+
+def _apply_eig(norm, wsq, beta, hd):
+    b0, b1 = beta[0], beta[1]
+    return {"h1": hd * b0 * (wsq + b1), "h2": b0 * (wsq * wsq + b1),
+            "h1sn": hd * b0 * wsq, "h2sn": hd * b0 * wsq * wsq}[norm]
+
+
+@pytest.mark.parametrize("norm", ["h1", "h2", "h1sn", "h2sn"])
+@pytest.mark.parametrize("w", [(1, 0, 0), (2, -3, 1), (0, 0, 5)])
+def test_reg_apply_eigenpairs(oracle, norm, w):
+    shape = (16, 16, 16)
+    beta = (1e-2, 1e-4, 0.0)
+    hd = np.prod([2 * np.pi / n for n in shape])
+    e = _mode(shape, w)
+    v = np.stack([e, 2 * e, -e])
+    out = oracle.reg_apply(v, norm, beta)
+    lam = _apply_eig(norm, float(np.dot(w, w)), beta, hd)
+    assert relerr(out, lam * v) < 1e-12
+    inv = oracle.reg_inverse(out, norm, beta)
+    expect = hd * v if norm != "h2" else v
+    assert relerr(inv, expect) < 1e-8  # the inverse amplifies FFT round-off at w = 0 by 1/(β0·β1)
+    s = oracle.reg_inverse(oracle.reg_inverse(v, norm, beta, applysqrt=True), norm, beta, applysqrt=True)
+    assert relerr(s, oracle.reg_inverse(v, norm, beta)) < 1e-10
+
+
+@pytest.mark.parametrize("norm", ["h1sn", "h2sn"])
+def test_reg_inverse_constant_mode(oracle, norm):
+    shape = (8, 8, 8)
+    v = np.ones((3,) + shape)
+    out = oracle.reg_inverse(v, norm, (0.5, 1e-4, 0.0))
+    assert np.abs(out - 2.0).max() < 1e-13  # β at w = 0 (src/RegularizationH1SN.cpp:356)
+
+
+def test_reg_nyquist_mode_is_seen_as_zero(oracle):
+    shape = (8, 8, 8)
+    v = np.stack([_mode(shape, (4, 0, 0))] * 3)
+    # w = N/2 maps to 0, so the seminorm operators annihilate it
+    assert np.abs(oracle.reg_apply(v, "h1sn", (1.0, 0.0, 0.0))).max() < 1e-13
+
+
+@pytest.mark.parametrize("norm", ["h1", "h2", "h1sn", "h2sn"])
+def test_reg_functional_parseval(oracle, norm):
+    shape = (16, 16, 16)
+    beta = (1e-2, 1e-4, 0.0)
+    hd = np.prod([2 * np.pi / n for n in shape])
+    w = (2, 1, -3)
+    e = _mode(shape, w)
+    v = np.stack([e, 0.5 * e, e])
+    wsq = float(np.dot(w, w))
+    sumsq = float(np.sum(v * v))  # = Σ_i ‖v_i‖²
+    b0, b1 = beta[0], beta[1]
+    expect = {
+        "h1sn": 0.5 * hd * b0 * wsq * sumsq,
+        "h1": 0.5 * hd * b0 * (wsq * sumsq + b1 * sumsq),
+        # ½·hd·‖√β0(−|w|² + √β1) v‖²  (the reference's H2 functional, src/RegularizationH2.cpp:128)
+        "h2": 0.5 * hd * b0 * (-wsq + np.sqrt(b1)) ** 2 * sumsq,
+        "h2sn": 0.5 * b0 * hd * wsq * wsq * sumsq,
+    }[norm]
+    assert abs(oracle.reg_functional(v, norm, beta) - expect) < 1e-11 * abs(expect)
+
+
+def test_reg_eigvals_inv_reference_formulas(oracle):
+    n = (16, 32, 8)
+    b0, b1 = 1e-2, 1e-4
+    lap = -(8.0 ** 2 + 16.0 ** 2 + 4.0 ** 2)
+    expect = {
+        "h1": (1 / (b0 * (-lap + b1)), 1 / b1),   # src/RegularizationH1.cpp:444-448
+        "h2": (1 / (b0 * lap * lap + b1), 1 / b1),  # src/RegularizationH2.cpp:443-446
+        "h1sn": (1 / (-b0 * lap + b1), 1 / b1),   # src/RegularizationH1SN.cpp:420-424
+        "h2sn": (1 / (b0 * lap * lap), 1 / b0),   # src/RegularizationH2SN.cpp:451-455
+    }
+    for norm, (lo, hi) in expect.items():
+        got = oracle.reg_eigvals_inv(n, norm, (b0, b1, 0.0))
+        assert abs(got[0] - lo) < 1e-12 * lo and abs(got[1] - hi) < 1e-12 * hi

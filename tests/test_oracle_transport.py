@@ -1,0 +1,228 @@
+"""Pins for the oracle's interpolation, trajectory and transport solves (CPU).
+
+Independent checks: the textbook Lagrange basis Π_{l≠k}(t−l)/(k−l)
+evaluated in pure Python, polynomial exactness, integer shifts, O(h⁴)
+convergence, constant-velocity closed forms, the v = 0 branches of the
+reference (src/CLAIRE.cpp:1909-1920, 2249-2312, 3367-3421), linearity and
+the gradient Taylor test (src/OptimizationProblem.cpp:157-239).
+"""
+import numpy as np
+import pytest
+
+from claire_amd import synth
+from conftest import relerr
+
+
+def _grid(shape):
+    h = [2 * np.pi / n for n in shape]
+    return np.meshgrid(*[np.arange(n) * h[a] for a, n in enumerate(shape)], indexing="ij")
+
+
+def _lagrange_textbook(t):
+    nodes = [0.0, 1.0, 2.0, 3.0]
+    out = []
+    for k in range(4):
+        v = 1.0
+        for l in range(4):
+            if l != k:
+                v *= (t - nodes[l]) / (nodes[k] - nodes[l])
+        out.append(v)
+    return out
+
+
+def _interp_textbook(f, q):
+    """Tricubic interpolation at normalized point q (periodic), from the
+    textbook Lagrange basis on the nodes floor(qN)−1 … floor(qN)+2."""
+    n = f.shape
+    idx, w = [], []
+    for a in range(3):
+        u = (q[a] % 1.0) * n[a]
+        i0 = int(np.floor(u)) - 1
+        t = u - i0
+        idx.append([(i0 + k) % n[a] for k in range(4)])
+        w.append(_lagrange_textbook(t))
+    val = 0.0
+    for a in range(4):
+        for b in range(4):
+            for c in range(4):
+                val += w[0][a] * w[1][b] * w[2][c] * f[idx[0][a], idx[1][b], idx[2][c]]
+    return val
+
+
+def test_interp_matches_textbook_lagrange(oracle):
+    rng = np.random.default_rng(0)
+    shape = (8, 12, 16)
+    f = rng.standard_normal(shape)
+    q = rng.uniform(-0.3, 1.3, (40, 3))
+    got = oracle.interp(f, q)
+    for i in range(q.shape[0]):
+        assert abs(got[i] - _interp_textbook(f, q[i])) < 1e-12
+
+
+def test_interp_partition_of_unity_and_nodes(oracle):
+    rng = np.random.default_rng(1)
+    shape = (16, 16, 16)
+    q = rng.uniform(-0.05, 1.05, (200, 3))
+    assert np.abs(oracle.interp(np.ones(shape), q) - 1.0).max() < 1e-13
+    f = rng.standard_normal(shape)
+    nodes = np.stack(np.meshgrid(*[np.arange(n) / n for n in shape], indexing="ij"), -1)
+    assert np.abs(oracle.interp(f, nodes) - f).max() < 1e-12
+
+
+def test_interp_integer_shift(oracle):
+    rng = np.random.default_rng(2)
+    shape = (16, 8, 32)
+    f = rng.standard_normal(shape)
+    g = np.stack(np.meshgrid(*[np.arange(n) for n in shape], indexing="ij"), -1).astype(np.float64)
+    shift = np.array([3, -2, 5])
+    q = (g + shift) / np.array(shape)
+    expect = np.roll(f, shift=tuple(-shift), axis=(0, 1, 2))
+    assert np.abs(oracle.interp(f, q) - expect).max() < 1e-12
+
+
+def test_interp_exact_for_cubics(oracle):
+    shape = (32, 32, 32)
+    rng = np.random.default_rng(3)
+    i = np.stack(np.meshgrid(*[np.arange(n) for n in shape], indexing="ij"), 0).astype(np.float64)
+    c = rng.standard_normal((3, 4))
+    f = sum(np.polyval(c[a], i[a] / 10.0) for a in range(3)) + (i[0] / 10) ** 3 * (i[1] / 10) * (i[2] / 10) ** 2
+    # query points whose stencils stay inside the grid (no periodic wrap)
+    u = rng.uniform(3.0, 27.0, (300, 3))
+    exact = sum(np.polyval(c[a], u[:, a] / 10.0) for a in range(3)) + (u[:, 0] / 10) ** 3 * (u[:, 1] / 10) * (u[:, 2] / 10) ** 2
+    got = oracle.interp(f, u / 32.0)
+    assert np.abs(got - exact).max() < 1e-10
+
+
+def test_interp_fourth_order_convergence(oracle):
+    rng = np.random.default_rng(4)
+    q = rng.uniform(0, 1, (500, 3))
+    errs = []
+    for n in (16, 32, 64):
+        x1, x2, x3 = _grid((n, n, n))
+        f = np.sin(x1) * np.cos(2 * x2) * np.sin(x3 + 0.3)
+        X = 2 * np.pi * q
+        exact = np.sin(X[:, 0]) * np.cos(2 * X[:, 1]) * np.sin(X[:, 2] + 0.3)
+        errs.append(np.abs(oracle.interp(f, q) - exact).max())
+    assert errs[0] / errs[1] > 12 and errs[1] / errs[2] > 12
+
+
+@pytest.mark.parametrize("sigma", [1, -1])
+def test_trajectory_constant_velocity(oracle, sigma):
+    shape = (8, 16, 8)
+    nt = 4
+    c = np.array([0.3, -0.7, 1.1])
+    v = np.stack([np.full(shape, c[a]) for a in range(3)])
+    X = oracle.trajectory(v, nt, sigma)
+    x = np.stack(_grid(shape), -1)
+    expect = (x - sigma * c / nt) / (2 * np.pi)
+    assert np.abs(X - expect).max() < 1e-14
+
+
+def test_state_translation_by_whole_cells(oracle):
+    shape = (16, 16, 16)
+    nt = 4
+    h = 2 * np.pi / 16
+    shift = np.array([1, 2, -1])            # cells per time step
+    v = np.stack([np.full(shape, shift[a] * h * nt) for a in range(3)])
+    mT = synth.gaussian_blobs(shape, count=5, seed=3).astype(np.float64)
+    m = oracle.solve_state(v, mT, nt)
+    for j in range(nt + 1):
+        expect = np.roll(mT, shift=tuple(j * shift), axis=(0, 1, 2))
+        assert np.abs(m[j] - expect).max() < 1e-12
+
+
+def test_zero_velocity_closed_forms(oracle):
+    shape = (16, 16, 16)
+    nt = 4
+    hd = (2 * np.pi / 16) ** 3
+    v = np.zeros((3,) + shape)
+    mT = synth.gaussian_blobs(shape, count=5, seed=4).astype(np.float64)
+    mR = synth.gaussian_blobs(shape, count=5, seed=5).astype(np.float64)
+    vt = synth.bandlimited_velocity(shape, kmax=3, cfl=1.0, seed=6).astype(np.float64)
+    m = oracle.solve_state(v, mT, nt)
+    assert all(np.array_equal(m[j], mT) for j in range(nt + 1))
+    gm = oracle.grad(mT)
+    lam, b = oracle.solve_adjoint(v, m, mR, nt)
+    assert np.abs(lam - (mR - mT)).max() < 1e-14
+    assert relerr(b, hd * (mR - mT) * gm) < 1e-13
+    mt = oracle.solve_inc_state(v, m, vt, nt)
+    gdotv = np.sum(gm * vt, axis=0)
+    assert relerr(mt, -gdotv) < 1e-12
+    lt, bt = oracle.solve_inc_adjoint(v, m, mt, nt)
+    assert relerr(bt, hd * gdotv * gm) < 1e-12
+    for norm in ("h1sn", "h2"):
+        H = oracle.hessian_matvec(v, m, vt, nt, norm, (1e-2, 1e-4, 0.0))
+        expect = oracle.reg_apply(vt, norm, (1e-2, 1e-4, 0.0)) + hd * gdotv * gm
+        assert relerr(H, expect) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def problem():
+    shape = (16, 16, 16)
+    nt = 4
+    v = synth.bandlimited_velocity(shape, kmax=3, cfl=1.5, nt=nt, seed=7).astype(np.float64)
+    mT = synth.gaussian_blobs(shape, count=6, seed=8).astype(np.float64)
+    mR = synth.gaussian_blobs(shape, count=6, seed=9).astype(np.float64)
+    return shape, nt, v, mT, mR
+
+
+def test_hessian_linearity(oracle, problem):
+    shape, nt, v, mT, mR = problem
+    m = oracle.solve_state(v, mT, nt)
+    a = synth.bandlimited_velocity(shape, kmax=4, cfl=1.0, seed=10).astype(np.float64)
+    b = synth.bandlimited_velocity(shape, kmax=4, cfl=1.0, seed=11).astype(np.float64)
+    H = lambda x: oracle.hessian_matvec(v, m, x, nt, "h1sn", (1e-2, 1e-4, 0.0))
+    assert relerr(H(2.0 * a - 3.0 * b), 2.0 * H(a) - 3.0 * H(b)) < 1e-12
+
+
+def test_hessian_symmetry_converges(oracle):
+    """The GN Hessian of the SL scheme (optimize-then-discretize) is symmetric
+    up to discretization error (HessianSymmetryCheck, src/OptimizationProblem.cpp:
+    439-491): the normalized asymmetry must fall at least 8× per grid doubling
+    and the Hessian must be positive on a test direction.  A wrong sign, time
+    index or adjoint term would leave an O(1) asymmetry."""
+    nt = 4
+    beta = (1e-2, 1e-4, 0.0)
+    asym = []
+    for n in (16, 32):
+        shape = (n, n, n)
+        v = synth.bandlimited_velocity(shape, kmax=3, cfl=1.5, nt=nt, seed=7).astype(np.float64)
+        mT = synth.benchmark_template(shape, dtype=np.float64)
+        m = oracle.solve_state(v, mT, nt)
+        a = synth.bandlimited_velocity(shape, kmax=3, cfl=1.0, seed=12).astype(np.float64)
+        b = synth.bandlimited_velocity(shape, kmax=3, cfl=1.0, seed=13).astype(np.float64)
+        Ha = oracle.hessian_matvec(v, m, a, nt, "h1sn", beta)
+        Hb = oracle.hessian_matvec(v, m, b, nt, "h1sn", beta)
+        scale = 0.5 * (np.linalg.norm(Ha) * np.linalg.norm(b) + np.linalg.norm(a) * np.linalg.norm(Hb))
+        asym.append(abs(float(np.sum(Ha * b)) - float(np.sum(a * Hb))) / scale)
+        assert float(np.sum(Ha * a)) > 0
+    assert asym[0] < 1e-3 and asym[1] < asym[0] / 8
+
+
+def test_gradient_taylor(oracle, problem):
+    """J(v + εw) − J(v) − ε⟨g, w⟩ must shrink like ε² until the SL
+    discretization error dominates."""
+    shape, nt, v, mT, mR = problem
+    beta = (1e-2, 1e-4, 0.0)
+    w = synth.bandlimited_velocity(shape, kmax=3, cfl=1.0, seed=14).astype(np.float64)
+    J0 = oracle.objective(v, mT, mR, nt, "h1sn", beta)[0]
+    g = oracle.gradient(v, mT, mR, nt, "h1sn", beta)
+    gw = float(np.sum(g * w))
+    for eps in (1e-1, 3e-2):
+        dJ = oracle.objective(v + eps * w, mT, mR, nt, "h1sn", beta)[0] - J0
+        assert abs(dJ - eps * gw) < 0.05 * abs(dJ)
+
+
+def test_fp32_twin_tracks_fp64(oracle, problem):
+    shape, nt, v, mT, mR = problem
+    beta = (1e-2, 1e-4, 0.0)
+    m64 = oracle.solve_state(v, mT, nt)
+    m32 = oracle.solve_state(v, mT, nt, prec="f32")
+    assert relerr(m32, m64) < 1e-5
+    vt = synth.bandlimited_velocity(shape, kmax=4, cfl=1.0, seed=15).astype(np.float64)
+    h64 = oracle.hessian_matvec(v, m64, vt, nt, "h1sn", beta)
+    h32 = oracle.hessian_matvec(v, m32, vt, nt, "h1sn", beta, prec="f32")
+    assert relerr(h32, h64) < 1e-5
+    g64 = oracle.gradient(v, mT, mR, nt, "h1sn", beta)
+    g32 = oracle.gradient(v, mT, mR, nt, "h1sn", beta, prec="f32")
+    assert relerr(g32, g64) < 1e-4
