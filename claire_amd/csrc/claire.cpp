@@ -383,9 +383,16 @@ int SemiLagrangian::ComputeTrajectory(const VecField* v, const std::string& flag
         launch_trajectory_rk2(m_Opt->domain, vc, w == 0 ? 1.0f : -1.0f, m_Opt->GetTimeStepSize(), m_X[w].get(),
                               m_Opt->stream);
     }
-    m_D[w] = max_displacement(gather_dims(m_Opt->domain), m_X[w].get(), 3, m_Scratch.get(), m_Opt->stream);
+    m_U[w].resize(4 * static_cast<size_t>(m_Opt->domain.nl));
+    m_D[w] = launch_make_plan(gather_dims(m_Opt->domain), m_X[w].get(), m_U[w].get(), m_Scratch.get(), m_Opt->stream);
     m_Valid[w] = true;
     return 0;
+}
+
+float* SemiLagrangian::Plan(const std::string& flag) {
+    const int w = Which(flag);
+    CLR_REQUIRE(m_Valid[w], kErrState, "trajectory '" + flag + "' has not been computed");
+    return m_U[w].get();
 }
 
 bool SemiLagrangian::HasTrajectory(const std::string& flag) const { return m_Valid[Which(flag)]; }
@@ -397,27 +404,32 @@ const float* SemiLagrangian::Trajectory(const std::string& flag) const {
 }
 
 void SemiLagrangian::Gather(float* xo, const float* xi, const std::string& flag) {
-    const float* X = Trajectory(flag);
+    const float* U = Plan(flag);
     const GatherDims g = gather_dims(m_Opt->domain);
-    KernelScope k(m_Opt->prof, "gather_scalar", 20.0 * g.nl);
-    if (UseLds(flag)) {
-        const float* in[1] = {xi};
-        float* out[1] = {xo};
-        launch_lds_gather(DisplacementBound(flag), g, in, 1, X, out, m_Opt->stream);
-    } else {
-        launch_gather1(g, xi, X, xo, m_Opt->stream);
-    }
+    const float* in[1] = {xi};
+    float* out[1] = {xo};
+    // bytes: plan 16 + gathered field 4 + output 4 per point
+    KernelScope k(m_Opt->prof, "gather_scalar", 24.0 * g.nl);
+    if (UseLds(flag))
+        launch_lds_gather(DisplacementBound(flag), g, in, 1, U, out, m_Opt->stream);
+    else
+        launch_direct_gather(g, in, 1, U, out, m_Opt->stream);
 }
 
 void SemiLagrangian::Gather3(const float* const in[3], float* const out[3], const std::string& flag) {
-    const float* X = Trajectory(flag);
+    const float* U = Plan(flag);
     const GatherDims g = gather_dims(m_Opt->domain);
-    KernelScope k(m_Opt->prof, "gather_vector", 36.0 * g.nl);
     const bool spaced = (in[1] - in[0] == in[2] - in[1]) && (out[1] - out[0] == out[2] - out[1]);
-    if (UseLds(flag) && spaced)
-        launch_lds_gather(DisplacementBound(flag), g, in, 3, X, out, m_Opt->stream);
+    if (!spaced) {
+        for (int d = 0; d < 3; ++d) Gather(out[d], in[d], flag);
+        return;
+    }
+    // bytes: 3 × (plan 16 + gathered field 4 + output 4) per point
+    KernelScope k(m_Opt->prof, "gather_vector", 72.0 * g.nl);
+    if (UseLds(flag))
+        launch_lds_gather(DisplacementBound(flag), g, in, 3, U, out, m_Opt->stream);
     else
-        launch_gather3(g, in, X, out, m_Opt->stream);
+        launch_direct_gather(g, in, 3, U, out, m_Opt->stream);
 }
 
 int SemiLagrangian::Interpolate(float* xo, const float* xi, const std::string& flag) {
@@ -478,11 +490,10 @@ CLAIRE::CLAIRE(const RegOpt& opt) : m_Opt(opt) {
     m_IncState[0].resize(nl);
     m_IncState[1].resize(nl);
     m_IncAdjointT0.resize(nl);
-    m_X4.resize(4 * nl);
     m_GradM.resize((nt + 1) * 3 * nl);
     m_GradMX.resize(nt * 3 * nl);
     m_S.resize(nt * nl);
-    m_Lam.resize(nt * nl);
+    m_Lam.resize((nt + 1) * nl);
     m_Scalar.resize(nl);
     m_VtX.resize(3 * nl);
 }
@@ -540,9 +551,9 @@ void CLAIRE::EnsureTrajectories() {
     const float* vc[3] = {m_VelocityField->m_X1, m_VelocityField->m_X2, m_VelocityField->m_X3};
     m_Spectral->Div(m_Scalar.get(), vc);
     {
-        KernelScope k(m_Opt.prof, "gather_adjoint_coeff", 36.0 * m_Opt.domain.nl);
-        launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Trajectory("adjoint"), m_Scalar.get(),
-                             m_Opt.GetTimeStepSize(), m_X4.get(), m_Opt.stream);
+        KernelScope k(m_Opt.prof, "gather_adjoint_coeff", 40.0 * m_Opt.domain.nl);
+        launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Plan("adjoint"), m_Scalar.get(), m_Opt.GetTimeStepSize(),
+                             m_Opt.stream);
     }
     m_TrajValid = true;
 }
@@ -613,24 +624,22 @@ void CLAIRE::IncAdjointCore(const float* lt0, VecField* bt) {
         launch_body_force_static(nl, m_Opt.GetLebesgueMeasure(), m_Lam.get(), m_GradM.get(), b, m_Opt.stream);
         return;
     }
-    const float* X4 = m_X4.get();
+    const float* U = m_SL->Plan("adjoint");
     const bool lds = m_SL->UseLds("adjoint");
     const int db = m_SL->DisplacementBound("adjoint");
-    for (int j = 0; j + 1 < nt; ++j) {
+    // λ^{j+1} = c·λ^j(X_a) (src/CLAIRE.cpp:2668-2673, 3955-3962), j = 0..nt−1
+    for (int j = 0; j < nt; ++j) {
+        // bytes: plan 16 + gathered λ 4 + output 4 per point
         KernelScope k(m_Opt.prof, "adjoint_step", 24.0 * nl);
         if (lds)
-            launch_lds_adjoint_step(db, g, X4, m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl, m_Opt.stream);
+            launch_lds_adjoint_step(db, g, U, m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl, m_Opt.stream);
         else
-            launch_adjoint_step(g, X4, m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl, m_Opt.stream);
+            launch_direct_adjoint_step(g, U, m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl, m_Opt.stream);
     }
-    // bytes: X4 16 + gathered λ 4 + λ^0..λ^{nt−1} 4nt + ∇m 12(nt+1) + b 12 + λ(t=0) 4
-    KernelScope k(m_Opt.prof, "adjoint_final", (16.0 + 4.0 + 4.0 * nt + 12.0 * (nt + 1) + 12.0 + 4.0) * nl);
-    if (lds)
-        launch_lds_adjoint_final(db, g, nt, m_Opt.GetTimeStepSize(), m_Opt.GetLebesgueMeasure(), X4, m_Lam.get(),
-                                 m_GradM.get(), b, m_Scalar.get(), m_Opt.stream);
-    else
-        launch_adjoint_final(g, nt, m_Opt.GetTimeStepSize(), m_Opt.GetLebesgueMeasure(), X4, m_Lam.get(),
-                             m_GradM.get(), b, m_Scalar.get(), m_Opt.stream);
+    // bytes: λ^0..λ^nt 4(nt+1) + ∇m 12(nt+1) + b 12
+    KernelScope k(m_Opt.prof, "body_force", (16.0 * (nt + 1) + 12.0) * nl);
+    launch_body_force(nl, nt, m_Opt.GetTimeStepSize(), m_Opt.GetLebesgueMeasure(), m_Lam.get(), m_GradM.get(), b,
+                      m_Opt.stream);
 }
 
 // Incremental state: writes λ̃^0 = −m̃^nt into out (or m̃^nt when !negate_last).
@@ -639,7 +648,7 @@ void CLAIRE::IncStateCore(const VecField* vt, float* out) {
     const int nt = m_Opt.nt;
     EnsureGradientsAtStateTrajectory();
     const GatherDims g = gather_dims(m_Opt.domain);
-    const float* Xs = m_SL->Trajectory("state");
+    const float* Us = m_SL->Plan("state");
     const float* v[3] = {vt->m_X1, vt->m_X2, vt->m_X3};
     float* first = (nt == 1) ? out : m_IncState[0].get();
     float* vx[3] = {m_VtX.get(), m_VtX.get() + nl, m_VtX.get() + 2 * nl};
@@ -657,12 +666,12 @@ void CLAIRE::IncStateCore(const VecField* vt, float* out) {
         const float* src = m_IncState[(j - 1) % 2].get();
         const bool last = (j == nt - 1);
         float* dst = last ? out : m_IncState[j % 2].get();
-        // bytes: X 12 + gathered m̃ 4 + s_j 4 + out 4
-        KernelScope k(m_Opt.prof, "inc_state_step", 24.0 * nl);
+        // bytes: plan 16 + gathered m̃ 4 + s_j 4 + out 4
+        KernelScope k(m_Opt.prof, "inc_state_step", 28.0 * nl);
         if (lds)
-            launch_lds_inc_state_step(db, g, Xs, src, m_S.get() + j * nl, dst, last ? 1 : 0, m_Opt.stream);
+            launch_lds_inc_state_step(db, g, Us, src, m_S.get() + j * nl, dst, last ? 1 : 0, m_Opt.stream);
         else
-            launch_inc_state_step(g, Xs, src, m_S.get() + j * nl, dst, last ? 1 : 0, m_Opt.stream);
+            launch_direct_inc_state_step(g, Us, src, m_S.get() + j * nl, dst, last ? 1 : 0, m_Opt.stream);
     }
 }
 
@@ -688,10 +697,8 @@ int CLAIRE::SolveIncAdjointFrom(const float* mtilde) {
     launch_copy(mtilde, m_Lam.get(), nl, m_Opt.stream);
     launch_scale(m_Lam.get(), -1.0f, nl, m_Opt.stream);
     IncAdjointCore(m_Lam.get(), m_IncBodyForce.get());
-    if (m_VelocityIsZero)
-        launch_copy(m_Lam.get(), m_IncAdjointT0.get(), nl, m_Opt.stream);
-    else
-        launch_copy(m_Scalar.get(), m_IncAdjointT0.get(), nl, m_Opt.stream);
+    // λ̃ at t = 0: the last slot (constant in time when v = 0)
+    launch_copy(m_VelocityIsZero ? m_Lam.get() : m_Lam.get() + m_Opt.nt * nl, m_IncAdjointT0.get(), nl, m_Opt.stream);
     return 0;
 }
 
@@ -730,10 +737,7 @@ int CLAIRE::SolveAdjointEquation() {
     // λ¹ = mR − m^nt (src/DistanceMeasureSL2.cpp:198-205)
     launch_sub(m_ReferenceImage.get(), m_State.get() + m_Opt.nt * nl, m_Lam.get(), nl, m_Opt.stream);
     IncAdjointCore(m_Lam.get(), m_BodyForce.get());
-    if (m_VelocityIsZero)
-        launch_copy(m_Lam.get(), m_Adjoint.get(), nl, m_Opt.stream);
-    else
-        launch_copy(m_Scalar.get(), m_Adjoint.get(), nl, m_Opt.stream);
+    launch_copy(m_VelocityIsZero ? m_Lam.get() : m_Lam.get() + m_Opt.nt * nl, m_Adjoint.get(), nl, m_Opt.stream);
     return 0;
 }
 

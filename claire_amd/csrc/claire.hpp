@@ -229,14 +229,19 @@ class SemiLagrangian {
     // flag "state" (σ = +1) or "adjoint" (σ = −1)
     int ComputeTrajectory(const VecField* v, const std::string& flag);
     int Interpolate(float* xo, const float* xi, const std::string& flag);
+    int Interpolate(VecField* vo, const VecField* vi, const std::string& flag);
     // Out-of-place gathers along a trajectory (no aliasing allowed).
     void Gather(float* xo, const float* xi, const std::string& flag);
     void Gather3(const float* const in[3], float* const out[3], const std::string& flag);
-    int Interpolate(VecField* vo, const VecField* vi, const std::string& flag);
+    // Trajectory X (3 floats per point, normalized by 2π; the reference's m_X)
     const float* Trajectory(const std::string& flag) const;
+    // Interpolation plan of the trajectory (Interp3_Plan after scatter):
+    // 4 floats per point, rescaled coordinates u = q·N + 3 and one per-point
+    // coefficient slot.
+    float* Plan(const std::string& flag);
     bool HasTrajectory(const std::string& flag) const;
-    // Per-axis displacement bound (cells) of the trajectory; gathers along it
-    // use the LDS-staged kernels when it is at most lds_gather_max_displacement().
+    // Per-axis stencil displacement bound (cells) of the plan; gathers along
+    // it use the LDS-staged kernels when it is at most lds_gather_max_displacement().
     int DisplacementBound(const std::string& flag) const { return m_D[Which(flag)]; }
     bool UseLds(const std::string& flag) const {
         return m_D[Which(flag)] <= claire_amd::lds_gather_max_displacement();
@@ -246,6 +251,7 @@ class SemiLagrangian {
     int Which(const std::string& flag) const;
     const RegOpt* m_Opt;
     DeviceArray<float> m_X[2];
+    DeviceArray<float> m_U[2];
     DeviceArray<int> m_Scratch;
     bool m_Valid[2] = {false, false};
     int m_D[2] = {0, 0};
@@ -310,11 +316,10 @@ class CLAIRE {
     DeviceArray<float> m_IncState[2];    // ping-pong m̃
     int m_IncStateSlot = 0;
     DeviceArray<float> m_IncAdjointT0;   // λ̃ at t = 0
-    DeviceArray<float> m_X4;             // adjoint trajectory + update coefficient, 4·nl
     DeviceArray<float> m_GradM;          // ∇m^j, (nt+1)·3·nl
     DeviceArray<float> m_GradMX;         // (∇m^j)(X_s), nt·3·nl
     DeviceArray<float> m_S;              // inc-state decrements, nt·nl
-    DeviceArray<float> m_Lam;            // adjoint time series λ^0..λ^{nt−1}, nt·nl
+    DeviceArray<float> m_Lam;            // adjoint time series λ^0..λ^nt, (nt+1)·nl
     DeviceArray<float> m_Scalar;         // nl work
     DeviceArray<float> m_VtX;            // ṽ(X_s), 3·nl
 

@@ -1,22 +1,16 @@
-// interp.hip -- semi-Lagrangian gathers for gfx950.
+// interp.hip -- semi-Lagrangian trajectory, plans and direct gathers (gfx950).
 //
 // Replaces Interp3_Plan::scatter/interpolate + optimized_interp3_ghost_xyz_p
 // (src/Interp3_Plan.cpp:294-845, src/interp3.cpp:2492-2719) and the ghost
-// exchange accfft_get_ghost_xyz (src/ghost.cpp:555-612) on one GPU:
-//   * the periodic wrap of the query point is the reference's
-//     (src/Interp3_Plan.cpp:340-362): q ∈ (−1/N, 1);
-//   * index space u = q·N + 3 with the ghost width 3 of the reference
-//     (rescale_xyz, src/interp3.cpp:44-47), stencil base floor(u) − 1 and
-//     weights c_k·Π_{l≠k}(t − l) in the reference's FP32 product order
-//     (src/interp3.cpp:2508-2533);
-//   * the ghost copy is replaced by modular indexing: ghost index j is grid
-//     index (j − 3) mod N, so no padded copy of the field is ever written.
-// One thread computes one output point; a block is 256 consecutive points
-// (whole or partial x3 rows).  The four x3-consecutive stencil values of a
-// row are one unaligned 16-byte load unless the stencil wraps around the row
-// end.  Blocks are remapped so that each of the 8 XCDs works on one
-// contiguous slab of the grid and its L2 holds that slab's stencil rows.
-// The transport updates of the solves are fused into the gathers.
+// exchange accfft_get_ghost_xyz (src/ghost.cpp:555-612) on one GPU.  The
+// arithmetic per point is in gather_device.hpp.  Kernels here:
+//   * the fused RK2 characteristic (Euler predictor kept in registers);
+//   * the plan: wrapped, rescaled query points u = q·N + 3 stored once per
+//     trajectory as float4, together with the exact per-axis displacement
+//     bound that decides whether the LDS-staged gathers can be used;
+//   * direct gathers (one thread per point, stencil rows read from global
+//     memory as unaligned 16-byte loads) for arbitrary query points and for
+//     trajectories whose displacement exceeds the LDS window.
 #include "gather_device.hpp"
 
 namespace claire_amd {
@@ -27,12 +21,8 @@ using namespace gather;
 
 constexpr int kThreads = 256;
 
-// Block → contiguous range of points, spread so that each XCD (blocks are
-// dispatched round-robin over 8 XCDs) owns one contiguous slab.
 __device__ __forceinline__ int64_t point_index(int nblocks) {
-    const int b = blockIdx.x;
-    const int L = (nblocks % 8 == 0) ? ((b & 7) * (nblocks >> 3) + (b >> 3)) : b;
-    return static_cast<int64_t>(L) * kThreads + threadIdx.x;
+    return static_cast<int64_t>(xcd_remap(blockIdx.x, nblocks)) * kThreads + threadIdx.x;
 }
 
 __device__ __forceinline__ void load_q3(const float* X, int64_t i, float q[3]) {
@@ -79,8 +69,8 @@ __global__ __launch_bounds__(kThreads) void k_gather3(GatherDims g, HH hh, P3 p,
 // with the Euler predictor kept in registers.
 struct TrajParams {
     float hx[3];
-    float sht;    // σ·ht
-    float shth;   // σ·ht/2
+    float sht;   // σ·ht
+    float shth;  // σ·ht/2
 };
 
 __global__ __launch_bounds__(kThreads) void k_traj_rk2(GatherDims g, HH hh, TrajParams tp, P3 v,
@@ -112,92 +102,81 @@ __global__ __launch_bounds__(kThreads) void k_traj_rk2(GatherDims g, HH hh, Traj
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_adjoint_coeff(GatherDims g, HH hh, const float* __restrict__ Xa,
-                                                            const float* __restrict__ divv, float ht,
-                                                            float4* __restrict__ X4, int nblocks) {
+// Plan of a trajectory: U[i] = (u1, u2, u3, 0) with u = wrap(q)·N + 3, and the
+// smallest D such that every stencil lies within [i − D − 1, i + D + 2] per
+// axis (the window of the LDS-staged gathers), computed from the same floor
+// that the gathers use.
+__global__ __launch_bounds__(kThreads) void k_make_plan(GatherDims g, HH hh, const float* __restrict__ X,
+                                                        float4* __restrict__ U, int* __restrict__ dmax, int nblocks) {
+    __shared__ int sh[kThreads];
+    const int64_t i = point_index(nblocks);
+    int need = 0;
+    if (i < g.nl) {
+        const int64_t plane = static_cast<int64_t>(g.n2) * g.n3;
+        const int idx[3] = {static_cast<int>(i / plane), static_cast<int>((i / g.n3) % g.n2),
+                            static_cast<int>(i % g.n3)};
+        const int n[3] = {g.n1, g.n2, g.n3};
+        float u[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            u[a] = index_coord(X[3 * i + a], n[a], hh.h[a]);
+            if (!(u[a] >= 0.0f && u[a] < static_cast<float>(n[a] + 8))) {
+                need = 1 << 20;  // non-finite or not wrapped: direct gathers only
+                continue;
+            }
+            int lo = stencil_start(u[a]) - idx[a];
+            // nearest periodic representative in [−n/2, n/2)
+            lo = ((lo + n[a] / 2) % n[a] + n[a]) % n[a] - n[a] / 2;
+            need = max(need, max(-lo - 1, lo + 1));
+        }
+        U[i] = make_float4(u[0], u[1], u[2], 0.0f);
+    }
+    sh[threadIdx.x] = need;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = max(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(dmax, sh[0]);
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
+// Adjoint update coefficient c = 1 + ht/2·(dX + (1 + ht·dX)·d), d = ∇·v,
+// dX = d(X_a), stored in the w component of the adjoint plan.  The SL adjoint
+// step λ⁺ = λX + ht/2·(λX·dX + (λX + ht·λX·dX)·d) (src/CLAIRE.cpp:2668-2673,
+// 3955-3962) is then λ⁺ = c·λX.
+__global__ __launch_bounds__(kThreads) void k_adjoint_coeff(GatherDims g, float4* __restrict__ U,
+                                                            const float* __restrict__ divv, float ht, int nblocks) {
     const int64_t i = point_index(nblocks);
     if (i >= g.nl) return;
-    float q[3];
-    load_q3(Xa, i, q);
+    float4 a = U[i];
+    const float u[3] = {a.x, a.y, a.z};
     Stencil S;
-    setup_stencil(q, g, hh.h, S);
+    setup_stencil_u(u, g, S);
     const float* fs[1] = {divv};
     float dx[1];
     eval_stencil<1>(fs, S, g, dx);
     const float d = divv[i];
     const float hthalf = 0.5f * ht;
-    const float c = 1.0f + hthalf * (dx[0] + (1.0f + ht * dx[0]) * d);
-    X4[i] = make_float4(q[0], q[1], q[2], c);
+    a.w = 1.0f + hthalf * (dx[0] + (1.0f + ht * dx[0]) * d);
+    U[i] = a;
 }
 
-__global__ __launch_bounds__(kThreads) void k_inc_state_step(GatherDims g, HH hh, const float* __restrict__ Xs,
-                                                             const float* __restrict__ mt, const float* __restrict__ sj,
-                                                             float* __restrict__ out, float sign, int nblocks) {
+template <class Epi>
+__global__ __launch_bounds__(kThreads) void k_gather_direct(GatherDims g, const float* __restrict__ fbase,
+                                                            int64_t fzstride, Epi epi, int nblocks) {
     const int64_t i = point_index(nblocks);
     if (i >= g.nl) return;
-    float q[3];
-    load_q3(Xs, i, q);
+    const float* f = fbase + static_cast<int64_t>(blockIdx.y) * fzstride;
+    epi.select(blockIdx.y);
+    const QPre p = epi.load(i);
     Stencil S;
-    setup_stencil(q, g, hh.h, S);
-    const float* fs[1] = {mt};
+    setup_stencil_u(p.u, g, S);
+    const float* fs[1] = {f};
     float r[1];
     eval_stencil<1>(fs, S, g, r);
-    out[i] = sign * (r[0] - sj[i]);
-}
-
-__global__ __launch_bounds__(kThreads) void k_adjoint_step(GatherDims g, HH hh, const float4* __restrict__ X4,
-                                                           const float* __restrict__ lam, float* __restrict__ out,
-                                                           int nblocks) {
-    const int64_t i = point_index(nblocks);
-    if (i >= g.nl) return;
-    const float4 a = X4[i];
-    const float q[3] = {a.x, a.y, a.z};
-    Stencil S;
-    setup_stencil(q, g, hh.h, S);
-    const float* fs[1] = {lam};
-    float r[1];
-    eval_stencil<1>(fs, S, g, r);
-    out[i] = a.w * r[0];
-}
-
-struct FinalP {
-    const float* lam;  // nt fields λ^0..λ^{nt−1}
-    const float* gm;   // (nt+1)·3 fields
-    float* b[3];
-    float* lam_out;    // optional: λ^nt
-};
-
-__global__ __launch_bounds__(kThreads) void k_adjoint_final(GatherDims g, HH hh, int nt, float ht, float hd,
-                                                            const float4* __restrict__ X4, FinalP p, int nblocks) {
-    const int64_t i = point_index(nblocks);
-    if (i >= g.nl) return;
-    const int64_t nl = g.nl;
-    const float4 a = X4[i];
-    const float q[3] = {a.x, a.y, a.z};
-    Stencil S;
-    setup_stencil(q, g, hh.h, S);
-    const float* fs[1] = {p.lam + static_cast<int64_t>(nt - 1) * nl};
-    float r[1];
-    eval_stencil<1>(fs, S, g, r);
-    const float lam_nt = a.w * r[0];
-    if (p.lam_out) p.lam_out[i] = lam_nt;
-    // body force, accumulated in the reference's order (src/CLAIRE.cpp:3940-3999)
-    float b0 = 0.f, b1 = 0.f, b2 = 0.f;
-    for (int j = 0; j < nt; ++j) {
-        const float w = (j == 0) ? 0.5f * ht : ht;
-        const float l = p.lam[static_cast<int64_t>(j) * nl + i];
-        const float* gmj = p.gm + static_cast<int64_t>(3 * (nt - j)) * nl;
-        b0 += w * gmj[i] * l;
-        b1 += w * gmj[nl + i] * l;
-        b2 += w * gmj[2 * nl + i] * l;
-    }
-    const float wl = 0.5f * ht;
-    b0 += wl * p.gm[i] * lam_nt;
-    b1 += wl * p.gm[nl + i] * lam_nt;
-    b2 += wl * p.gm[2 * nl + i] * lam_nt;
-    p.b[0][i] = b0 * hd;
-    p.b[1][i] = b1 * hd;
-    p.b[2][i] = b2 * hd;
+    epi.store(i, r[0], p);
 }
 
 __global__ __launch_bounds__(kThreads) void k_body_force_static(int64_t nl, float hd, const float* __restrict__ lam,
@@ -210,13 +189,19 @@ __global__ __launch_bounds__(kThreads) void k_body_force_static(int64_t nl, floa
     b.out[2][i] = (l * gm[2 * nl + i]) * hd;
 }
 
-
 int nblocks_for(int64_t nl) { return static_cast<int>((nl + kThreads - 1) / kThreads); }
 
 void check_gather_dims(const GatherDims& g) {
     CLR_REQUIRE(g.n1 >= 4 && g.n2 >= 4 && g.n3 >= 4, kErrInvalidArgument, "gather needs n >= 4 per axis");
     CLR_REQUIRE(static_cast<int64_t>(g.n1) * g.n2 * g.n3 == g.nl, kErrInvalidArgument, "inconsistent gather dims");
     CLR_REQUIRE(g.nl < (int64_t(1) << 31), kErrInvalidArgument, "grid too large for 32-bit stencil offsets");
+}
+
+template <class Epi>
+void launch_direct(const GatherDims& g, const float* f, int64_t fz, int nz, const Epi& e, hipStream_t st) {
+    check_gather_dims(g);
+    const int nb = nblocks_for(g.nl);
+    hipLaunchKernelGGL((k_gather_direct<Epi>), dim3(nb, nz), dim3(kThreads), 0, st, g, f, fz, e, nb);
 }
 
 }  // namespace
@@ -249,39 +234,43 @@ void launch_trajectory_rk2(const Dims& d, const float* const v[3], float sigma, 
     hipLaunchKernelGGL(k_traj_rk2, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), tp, p, X, nb);
 }
 
-void launch_adjoint_coeff(const GatherDims& g, const float* Xa, const float* divv, float ht, float* X4,
+int launch_make_plan(const GatherDims& g, const float* X, float* U, int* scratch, hipStream_t st) {
+    check_gather_dims(g);
+    const int nb = nblocks_for(g.nl);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, scratch);
+    hipLaunchKernelGGL(k_make_plan, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), X, reinterpret_cast<float4*>(U),
+                       scratch, nb);
+    int h = 0;
+    CLR_HIP_CHECK(hipMemcpyAsync(&h, scratch, sizeof(int), hipMemcpyDeviceToHost, st));
+    CLR_HIP_CHECK(hipStreamSynchronize(st));
+    return h;
+}
+
+void launch_adjoint_coeff(const GatherDims& g, float* U, const float* divv, float ht, hipStream_t st) {
+    check_gather_dims(g);
+    const int nb = nblocks_for(g.nl);
+    hipLaunchKernelGGL(k_adjoint_coeff, dim3(nb), dim3(kThreads), 0, st, g, reinterpret_cast<float4*>(U), divv, ht, nb);
+}
+
+void launch_direct_gather(const GatherDims& g, const float* const f[], int nf, const float* U, float* const out[],
                           hipStream_t st) {
-    check_gather_dims(g);
-    const int nb = nblocks_for(g.nl);
-    hipLaunchKernelGGL(k_adjoint_coeff, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), Xa, divv, ht,
-                       reinterpret_cast<float4*>(X4), nb);
+    const int64_t fz = nf > 1 ? (f[1] - f[0]) : 0;
+    const int64_t oz = nf > 1 ? (out[1] - out[0]) : 0;
+    for (int d = 2; d < nf; ++d)
+        CLR_REQUIRE(f[d] - f[d - 1] == fz && out[d] - out[d - 1] == oz, kErrInvalidArgument,
+                    "vector gather needs equally spaced components");
+    launch_direct(g, f[0], fz, nf, EpiPlain{reinterpret_cast<const float4*>(U), out[0], oz}, st);
 }
 
-void launch_inc_state_step(const GatherDims& g, const float* Xs, const float* mt, const float* sj, float* out,
-                           int negate, hipStream_t st) {
-    check_gather_dims(g);
+void launch_direct_inc_state_step(const GatherDims& g, const float* U, const float* mt, const float* sj, float* out,
+                                  int negate, hipStream_t st) {
     CLR_REQUIRE(mt != out, kErrInvalidArgument, "semi-Lagrangian step cannot run in place");
-    const int nb = nblocks_for(g.nl);
-    hipLaunchKernelGGL(k_inc_state_step, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), Xs, mt, sj, out,
-                       negate ? -1.0f : 1.0f, nb);
+    launch_direct(g, mt, 0, 1, EpiIncStep{reinterpret_cast<const float4*>(U), sj, out, negate ? -1.0f : 1.0f}, st);
 }
 
-void launch_adjoint_step(const GatherDims& g, const float* X4, const float* lam, float* out, hipStream_t st) {
-    check_gather_dims(g);
+void launch_direct_adjoint_step(const GatherDims& g, const float* U, const float* lam, float* out, hipStream_t st) {
     CLR_REQUIRE(lam != out, kErrInvalidArgument, "semi-Lagrangian step cannot run in place");
-    const int nb = nblocks_for(g.nl);
-    hipLaunchKernelGGL(k_adjoint_step, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g),
-                       reinterpret_cast<const float4*>(X4), lam, out, nb);
-}
-
-void launch_adjoint_final(const GatherDims& g, int nt, float ht, float hd, const float* X4, const float* lam,
-                          const float* gm, float* const b[3], float* lam_out, hipStream_t st) {
-    check_gather_dims(g);
-    CLR_REQUIRE(nt >= 1, kErrInvalidArgument, "nt must be >= 1");
-    const int nb = nblocks_for(g.nl);
-    FinalP p{lam, gm, {b[0], b[1], b[2]}, lam_out};
-    hipLaunchKernelGGL(k_adjoint_final, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), nt, ht, hd,
-                       reinterpret_cast<const float4*>(X4), p, nb);
+    launch_direct(g, lam, 0, 1, EpiAdjStep{reinterpret_cast<const float4*>(U), out}, st);
 }
 
 void launch_body_force_static(int64_t nl, float hd, const float* lam, const float* gm, float* const b[3],

@@ -65,40 +65,42 @@ void launch_gather3(const GatherDims& g, const float* const f[3], const float* X
 // RK2 characteristic, Euler predictor and velocity gather fused:
 //   X̃ = (x − σ·ht·v)/2π,  X = (x − σ·ht/2·(v(X̃) + v))/2π
 void launch_trajectory_rk2(const Dims& d, const float* const v[3], float sigma, float ht, float* X, hipStream_t st);
-// Adjoint-side per-point data: X4 = (X_a, c) with d = ∇·v, dX = d(X_a),
-// c = 1 + ht/2·(dX + (1 + ht·dX)·d) (the SL adjoint update
-// λ⁺ = λX + ht/2·(λX·dX + (λX + ht·λX·dX)·d) = c·λX).
-void launch_adjoint_coeff(const GatherDims& g, const float* Xa, const float* divv, float ht, float* X4,
+
+// Plan of a trajectory: U (4 floats per point) = (u1, u2, u3, 0), u = q·N + 3
+// after the periodic wrap.  Returns the per-axis stencil displacement bound D
+// in cells (synchronizes the stream); the LDS-staged gathers need
+// D <= lds_gather_max_displacement().  `scratch` is one device int.
+int launch_make_plan(const GatherDims& g, const float* X, float* U, int* scratch, hipStream_t st);
+int lds_gather_max_displacement();
+// Adjoint plan coefficient: U.w = c = 1 + ht/2·(dX + (1 + ht·dX)·d) with
+// d = ∇·v, dX = d(X_a); the SL adjoint update is then λ⁺ = c·λ(X_a).
+void launch_adjoint_coeff(const GatherDims& g, float* U, const float* divv, float ht, hipStream_t st);
+
+// Gathers along a plan.  Vector fields are nf equally spaced component pointers.
+//   gather:          out_d = f_d(X)
+//   inc_state_step:  out = ±(f(X) − s)   (m̃^{j+1} = m̃^j(X_s) − s_j; negated for λ̃⁰ = −m̃^nt)
+//   adjoint_step:    out = c·f(X)        (λ^{j+1} = c·λ^j(X_a))
+// LDS-staged versions (dbound = plan's D <= lds_gather_max_displacement()):
+void launch_lds_gather(int dbound, const GatherDims& g, const float* const f[], int nf, const float* U,
+                       float* const out[], hipStream_t st);
+void launch_lds_inc_state_step(int dbound, const GatherDims& g, const float* U, const float* mt, const float* sj,
+                               float* out, int negate, hipStream_t st);
+void launch_lds_adjoint_step(int dbound, const GatherDims& g, const float* U, const float* lam, float* out,
+                             hipStream_t st);
+// Direct versions (any displacement):
+void launch_direct_gather(const GatherDims& g, const float* const f[], int nf, const float* U, float* const out[],
                           hipStream_t st);
-// m̃^{j+1} = m̃^j(X_s) − s_j, negated when `negate` (λ̃⁰ = −m̃^nt).
-void launch_inc_state_step(const GatherDims& g, const float* Xs, const float* mt, const float* sj, float* out,
-                           int negate, hipStream_t st);
-// λ^{j+1} = c·λ^j(X_a)
-void launch_adjoint_step(const GatherDims& g, const float* X4, const float* lam, float* out, hipStream_t st);
-// Last adjoint step fused with the body force:
-//   λ^nt = c·λ^{nt−1}(X_a),  b_d = hd·Σ_{j=0..nt} w_j·λ^j·∂_d m^{nt−j},
-//   w_0 = w_nt = ht/2, w_j = ht otherwise.  lam holds λ^0..λ^{nt−1};
-//   λ^nt is written to lam_out when it is not null.
-void launch_adjoint_final(const GatherDims& g, int nt, float ht, float hd, const float* X4, const float* lam,
-                          const float* gm, float* const b[3], float* lam_out, hipStream_t st);
+void launch_direct_inc_state_step(const GatherDims& g, const float* U, const float* mt, const float* sj, float* out,
+                                  int negate, hipStream_t st);
+void launch_direct_adjoint_step(const GatherDims& g, const float* U, const float* lam, float* out, hipStream_t st);
+
+// b_d = hd·Σ_{j=0..nt} w_j·λ^j·∂_d m^{nt−j} (trapezoid weights); lam holds
+// λ^0..λ^nt, gm holds ∇m^0..∇m^nt (3 fields each).
+void launch_body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const float* gm, float* const b[3],
+                       hipStream_t st);
 // v = 0 body force: b_d = hd·λ·∂_d m
 void launch_body_force_static(int64_t nl, float hd, const float* lam, const float* gm, float* const b[3],
                               hipStream_t st);
-
-// LDS-staged gathers (gather_lds.hip).  `dbound` is the per-axis
-// characteristic displacement bound in cells (max_displacement); points
-// beyond it fall back to the direct gather.  Vector fields are passed as nf
-// equally spaced component pointers.
-int lds_gather_max_displacement();
-int max_displacement(const GatherDims& g, const float* X, int stride, int* scratch, hipStream_t st);
-void launch_lds_gather(int dbound, const GatherDims& g, const float* const f[], int nf, const float* X,
-                       float* const out[], hipStream_t st);
-void launch_lds_inc_state_step(int dbound, const GatherDims& g, const float* Xs, const float* mt, const float* sj,
-                               float* out, int negate, hipStream_t st);
-void launch_lds_adjoint_step(int dbound, const GatherDims& g, const float* X4, const float* lam, float* out,
-                             hipStream_t st);
-void launch_lds_adjoint_final(int dbound, const GatherDims& g, int nt, float ht, float hd, const float* X4,
-                              const float* lam, const float* gm, float* const b[3], float* lam_out, hipStream_t st);
 // s_j = ht/2·[(∇m^j)(X_s)·ṽX + ∇m^{j+1}·ṽ] for j = 0..nt−1 from a precomputed
 // ṽX = ṽ(X_s); writes m̃¹ = −s_0 (λ̃⁰ = s_0 when nt = 1) and s_1..s_{nt−1}.
 void launch_inc_state_rhs(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],

@@ -19,6 +19,15 @@ BETA = (1e-2, 1e-4, 0.0)
 NT = 4
 
 
+def assert_fp32_parity(gpu, ref32, ref64, tol=1e-5):
+    """The GPU FP32 result must be within `tol` of the FP64 oracle, or, where
+    FP32 conditioning puts that out of reach for any FP32 implementation (the
+    |w|⁴ multipliers of H2/H2SN amplify FFT round-off), no less accurate than
+    the reference's FP32 path (within 2× of its error against FP64)."""
+    e_gpu, e_ref = relerr(gpu, ref64), relerr(ref32, ref64)
+    assert e_gpu <= max(tol, 2.0 * e_ref), (e_gpu, e_ref)
+
+
 def _data(shape, seed=0, cfl=2.0):
     v = synth.bandlimited_velocity(shape, kmax=min(8, min(shape) // 4), cfl=cfl, nt=NT, seed=seed)
     mT = synth.gaussian_blobs(shape, count=8, seed=seed + 1)
@@ -64,11 +73,13 @@ def test_regularization_ops(norm, oracle):
     _, _, _, vt = _data(shape, seed=5)
     c = Claire(shape, nt=NT, regnorm=norm, beta=BETA)
     try:
-        assert relerr(c.reg_apply(vt), oracle.reg_apply(vt, norm, BETA)) < 1e-5
+        assert_fp32_parity(c.reg_apply(vt), oracle.reg_apply(vt, norm, BETA, prec="f32"),
+                           oracle.reg_apply(vt, norm, BETA))
         assert relerr(c.apply_inv_reg(vt), oracle.reg_inverse(vt, norm, BETA)) < 1e-5
         assert relerr(c.apply_inv_reg(vt, applysqrt=True), oracle.reg_inverse(vt, norm, BETA, applysqrt=True)) < 1e-5
         r = c.reg_functional(vt)
-        assert abs(r - oracle.reg_functional(vt, norm, BETA)) < 1e-5 * abs(r)
+        r32, r64 = oracle.reg_functional(vt, norm, BETA, prec="f32"), oracle.reg_functional(vt, norm, BETA)
+        assert abs(r - r64) <= max(1e-5 * abs(r64), 2.0 * abs(r32 - r64))
         lo, hi = c.reg_eigvals_inv()
         elo, ehi = oracle.reg_eigvals_inv(shape, norm, BETA, prec="f32")
         assert abs(lo - elo) <= 1e-6 * abs(elo) and abs(hi - ehi) <= 1e-6 * abs(ehi)
@@ -161,7 +172,8 @@ def test_hessian_other_norms(norm, oracle):
         H = c.hessian_matvec(vt)
         m32 = oracle.solve_state(v, mT, NT, prec="f32")
         Ho = oracle.hessian_matvec(v, m32, vt, NT, norm, (1e-3, 1e-4, 0.0), prec="f32")
-        assert relerr(H, Ho) < 1e-5
+        H64 = oracle.hessian_matvec(v, oracle.solve_state(v, mT, NT), vt, NT, norm, (1e-3, 1e-4, 0.0))
+        assert_fp32_parity(H, Ho, H64)
     finally:
         c.close()
 
