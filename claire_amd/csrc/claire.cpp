@@ -383,16 +383,21 @@ int SemiLagrangian::ComputeTrajectory(const VecField* v, const std::string& flag
         launch_trajectory_rk2(m_Opt->domain, vc, w == 0 ? 1.0f : -1.0f, m_Opt->GetTimeStepSize(), m_X[w].get(),
                               m_Opt->stream);
     }
-    m_U[w].resize(4 * static_cast<size_t>(m_Opt->domain.nl));
-    m_D[w] = launch_make_plan(gather_dims(m_Opt->domain), m_X[w].get(), m_U[w].get(), m_Scratch.get(), m_Opt->stream);
+    m_P[w].resize(3 * static_cast<size_t>(m_Opt->domain.nl));
+    {
+        // bytes: X read (12) + packed plan written (12) per point
+        KernelScope k(m_Opt->prof, "gather_plan", 24.0 * m_Opt->domain.nl);
+        m_D[w] = launch_make_plan(gather_dims(m_Opt->domain), m_X[w].get(), m_P[w].get(), m_Scratch.get(),
+                                  m_Opt->stream);
+    }
     m_Valid[w] = true;
     return 0;
 }
 
-float* SemiLagrangian::Plan(const std::string& flag) {
+TrajectoryPlan SemiLagrangian::Plan(const std::string& flag) const {
     const int w = Which(flag);
     CLR_REQUIRE(m_Valid[w], kErrState, "trajectory '" + flag + "' has not been computed");
-    return m_U[w].get();
+    return TrajectoryPlan{m_P[w].get(), m_X[w].get(), m_D[w]};
 }
 
 bool SemiLagrangian::HasTrajectory(const std::string& flag) const { return m_Valid[Which(flag)]; }
@@ -404,32 +409,26 @@ const float* SemiLagrangian::Trajectory(const std::string& flag) const {
 }
 
 void SemiLagrangian::Gather(float* xo, const float* xi, const std::string& flag) {
-    const float* U = Plan(flag);
+    const TrajectoryPlan tp = Plan(flag);
     const GatherDims g = gather_dims(m_Opt->domain);
     const float* in[1] = {xi};
     float* out[1] = {xo};
-    // bytes: plan 16 + gathered field 4 + output 4 per point
-    KernelScope k(m_Opt->prof, "gather_scalar", 24.0 * g.nl);
-    if (UseLds(flag))
-        launch_lds_gather(DisplacementBound(flag), g, in, 1, U, out, m_Opt->stream);
-    else
-        launch_direct_gather(g, in, 1, U, out, m_Opt->stream);
+    // bytes: plan 12 + gathered field 4 + output 4 per point
+    KernelScope k(m_Opt->prof, "gather_scalar", 20.0 * g.nl);
+    launch_plan_gather(tp, g, in, 1, out, m_Opt->stream);
 }
 
 void SemiLagrangian::Gather3(const float* const in[3], float* const out[3], const std::string& flag) {
-    const float* U = Plan(flag);
+    const TrajectoryPlan tp = Plan(flag);
     const GatherDims g = gather_dims(m_Opt->domain);
     const bool spaced = (in[1] - in[0] == in[2] - in[1]) && (out[1] - out[0] == out[2] - out[1]);
     if (!spaced) {
         for (int d = 0; d < 3; ++d) Gather(out[d], in[d], flag);
         return;
     }
-    // bytes: 3 × (plan 16 + gathered field 4 + output 4) per point
-    KernelScope k(m_Opt->prof, "gather_vector", 72.0 * g.nl);
-    if (UseLds(flag))
-        launch_lds_gather(DisplacementBound(flag), g, in, 3, U, out, m_Opt->stream);
-    else
-        launch_direct_gather(g, in, 3, U, out, m_Opt->stream);
+    // bytes: 3 × (plan 12 + gathered field 4 + output 4) per point
+    KernelScope k(m_Opt->prof, "gather_vector", 60.0 * g.nl);
+    launch_plan_gather(tp, g, in, 3, out, m_Opt->stream);
 }
 
 int SemiLagrangian::Interpolate(float* xo, const float* xi, const std::string& flag) {
@@ -495,6 +494,7 @@ CLAIRE::CLAIRE(const RegOpt& opt) : m_Opt(opt) {
     m_S.resize(nt * nl);
     m_Lam.resize((nt + 1) * nl);
     m_Scalar.resize(nl);
+    m_AdjCoeff.resize(nl);
     m_VtX.resize(3 * nl);
 }
 
@@ -551,9 +551,10 @@ void CLAIRE::EnsureTrajectories() {
     const float* vc[3] = {m_VelocityField->m_X1, m_VelocityField->m_X2, m_VelocityField->m_X3};
     m_Spectral->Div(m_Scalar.get(), vc);
     {
-        KernelScope k(m_Opt.prof, "gather_adjoint_coeff", 40.0 * m_Opt.domain.nl);
-        launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Plan("adjoint"), m_Scalar.get(), m_Opt.GetTimeStepSize(),
-                             m_Opt.stream);
+        // bytes: X 12 + div v gathered 4 + div v 4 + C 4 per point
+        KernelScope k(m_Opt.prof, "gather_adjoint_coeff", 24.0 * m_Opt.domain.nl);
+        launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Plan("adjoint").X, m_Scalar.get(),
+                             m_Opt.GetTimeStepSize(), m_AdjCoeff.get(), m_Opt.stream);
     }
     m_TrajValid = true;
 }
@@ -624,17 +625,13 @@ void CLAIRE::IncAdjointCore(const float* lt0, VecField* bt) {
         launch_body_force_static(nl, m_Opt.GetLebesgueMeasure(), m_Lam.get(), m_GradM.get(), b, m_Opt.stream);
         return;
     }
-    const float* U = m_SL->Plan("adjoint");
-    const bool lds = m_SL->UseLds("adjoint");
-    const int db = m_SL->DisplacementBound("adjoint");
+    const TrajectoryPlan tp = m_SL->Plan("adjoint");
     // λ^{j+1} = c·λ^j(X_a) (src/CLAIRE.cpp:2668-2673, 3955-3962), j = 0..nt−1
     for (int j = 0; j < nt; ++j) {
-        // bytes: plan 16 + gathered λ 4 + output 4 per point
+        // bytes: plan 12 + c 4 + gathered λ 4 + output 4 per point
         KernelScope k(m_Opt.prof, "adjoint_step", 24.0 * nl);
-        if (lds)
-            launch_lds_adjoint_step(db, g, U, m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl, m_Opt.stream);
-        else
-            launch_direct_adjoint_step(g, U, m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl, m_Opt.stream);
+        launch_plan_adjoint_step(tp, g, m_AdjCoeff.get(), m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl,
+                                 m_Opt.stream);
     }
     // bytes: λ^0..λ^nt 4(nt+1) + ∇m 12(nt+1) + b 12
     KernelScope k(m_Opt.prof, "body_force", (16.0 * (nt + 1) + 12.0) * nl);
@@ -648,7 +645,7 @@ void CLAIRE::IncStateCore(const VecField* vt, float* out) {
     const int nt = m_Opt.nt;
     EnsureGradientsAtStateTrajectory();
     const GatherDims g = gather_dims(m_Opt.domain);
-    const float* Us = m_SL->Plan("state");
+    const TrajectoryPlan ts = m_SL->Plan("state");
     const float* v[3] = {vt->m_X1, vt->m_X2, vt->m_X3};
     float* first = (nt == 1) ? out : m_IncState[0].get();
     float* vx[3] = {m_VtX.get(), m_VtX.get() + nl, m_VtX.get() + 2 * nl};
@@ -660,18 +657,13 @@ void CLAIRE::IncStateCore(const VecField* vt, float* out) {
         launch_inc_state_rhs(nl, nt, m_Opt.GetTimeStepSize(), vx, v, m_GradMX.get(), m_GradM.get(), m_S.get(), first,
                              m_Opt.stream);
     }
-    const bool lds = m_SL->UseLds("state");
-    const int db = m_SL->DisplacementBound("state");
     for (int j = 1; j < nt; ++j) {
         const float* src = m_IncState[(j - 1) % 2].get();
         const bool last = (j == nt - 1);
         float* dst = last ? out : m_IncState[j % 2].get();
-        // bytes: plan 16 + gathered m̃ 4 + s_j 4 + out 4
-        KernelScope k(m_Opt.prof, "inc_state_step", 28.0 * nl);
-        if (lds)
-            launch_lds_inc_state_step(db, g, Us, src, m_S.get() + j * nl, dst, last ? 1 : 0, m_Opt.stream);
-        else
-            launch_direct_inc_state_step(g, Us, src, m_S.get() + j * nl, dst, last ? 1 : 0, m_Opt.stream);
+        // bytes: plan 12 + gathered m̃ 4 + s_j 4 + out 4
+        KernelScope k(m_Opt.prof, "inc_state_step", 24.0 * nl);
+        launch_plan_inc_state_step(ts, g, src, m_S.get() + j * nl, dst, last ? 1 : 0, m_Opt.stream);
     }
 }
 

@@ -236,22 +236,19 @@ class SemiLagrangian {
     // Trajectory X (3 floats per point, normalized by 2π; the reference's m_X)
     const float* Trajectory(const std::string& flag) const;
     // Interpolation plan of the trajectory (Interp3_Plan after scatter):
-    // 4 floats per point, rescaled coordinates u = q·N + 3 and one per-point
-    // coefficient slot.
-    float* Plan(const std::string& flag);
+    // packed foot points (3 words per point), the raw foot points and the
+    // per-axis stencil displacement bound in cells.  Gathers along it use the
+    // LDS-staged kernels when the bound is at most
+    // lds_gather_max_displacement(), else the direct kernels.
+    claire_amd::TrajectoryPlan Plan(const std::string& flag) const;
     bool HasTrajectory(const std::string& flag) const;
-    // Per-axis stencil displacement bound (cells) of the plan; gathers along
-    // it use the LDS-staged kernels when it is at most lds_gather_max_displacement().
     int DisplacementBound(const std::string& flag) const { return m_D[Which(flag)]; }
-    bool UseLds(const std::string& flag) const {
-        return m_D[Which(flag)] <= claire_amd::lds_gather_max_displacement();
-    }
 
   private:
     int Which(const std::string& flag) const;
     const RegOpt* m_Opt;
     DeviceArray<float> m_X[2];
-    DeviceArray<float> m_U[2];
+    DeviceArray<uint32_t> m_P[2];
     DeviceArray<int> m_Scratch;
     bool m_Valid[2] = {false, false};
     int m_D[2] = {0, 0};
@@ -321,6 +318,7 @@ class CLAIRE {
     DeviceArray<float> m_S;              // inc-state decrements, nt·nl
     DeviceArray<float> m_Lam;            // adjoint time series λ^0..λ^nt, (nt+1)·nl
     DeviceArray<float> m_Scalar;         // nl work
+    DeviceArray<float> m_AdjCoeff;       // nl: adjoint update coefficient at every point
     DeviceArray<float> m_VtX;            // ṽ(X_s), 3·nl
 
     bool m_HaveVelocity = false, m_HaveIncVelocity = false, m_HaveTemplate = false, m_HaveReference = false;

@@ -3,9 +3,7 @@
 // Query-point handling restated from the reference:
 //   * periodic wrap into (−1/N, 1) (Interp3_Plan::fast_scatter, src/Interp3_Plan.cpp:340-362);
 //   * index space u = q·N + 3 with ghost width 3 (rescale_xyz, src/interp3.cpp:44-47);
-//     like Interp3_Plan, which wraps and rescales the query points once in
-//     scatter() and reuses them in every interpolate(), a trajectory is turned
-//     once into a "plan" of rescaled points u (one float4 per point);
+//     a trajectory is turned once into a packed plan (below);
 //   * stencil base floor(u) − 1, weights c_k·Π_{l≠k}(t − l) in the FP32 product
 //     order of optimized_interp3_ghost_xyz_p (src/interp3.cpp:2508-2533);
 //   * ghost index j ↔ grid index (j − 3) mod N (the ghost layer of
@@ -53,15 +51,20 @@ __device__ __forceinline__ float index_coord(float q, int n, float hh) {
 // Grid index of the first stencil node before the periodic wrap: floor(u) − 4.
 __device__ __forceinline__ int stencil_start(float u) { return __float2int_rd(u) - 4; }
 
-__device__ __forceinline__ void setup_axis_u(float u, int n, Axis& A) {
-    const int gi = __float2int_rd(u) - 1;
-    const float x = u - static_cast<float>(gi);
+// Tricubic Lagrange weights at t in the FP32 product order of
+// optimized_interp3_ghost_xyz_p (src/interp3.cpp:2508-2533).
+__device__ __forceinline__ void lagrange_weights(float x, float w[4]) {
     const float c0 = static_cast<float>(-1.0 / 6.0), c1 = 0.5f, c2 = -0.5f, c3 = static_cast<float>(1.0 / 6.0);
     const float x0 = x, x1 = x - 1.0f, x2 = x - 2.0f, x3 = x - 3.0f;
-    A.w[0] = ((c0 * x1) * x2) * x3;
-    A.w[1] = ((c1 * x0) * x2) * x3;
-    A.w[2] = ((c2 * x0) * x1) * x3;
-    A.w[3] = ((c3 * x0) * x1) * x2;
+    w[0] = ((c0 * x1) * x2) * x3;
+    w[1] = ((c1 * x0) * x2) * x3;
+    w[2] = ((c2 * x0) * x1) * x3;
+    w[3] = ((c3 * x0) * x1) * x2;
+}
+
+__device__ __forceinline__ void setup_axis_u(float u, int n, Axis& A) {
+    const int gi = __float2int_rd(u) - 1;
+    lagrange_weights(u - static_cast<float>(gi), A.w);
     int i0 = gi - 3;
     if (i0 < -2 || i0 > n) {  // only for non-finite coordinates
         i0 %= n;
@@ -138,49 +141,68 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
 }
 
 // ---------------------------------------------------------------------------
-// Epilogues of the trajectory-based gathers: the query comes from a plan
-// (u1, u2, u3, aux) and the epilogue decides what to do with f(X).
+// Packed plan.  Like Interp3_Plan, which wraps and rescales the query points
+// once in scatter() and reuses them in every interpolate(), a trajectory is
+// turned once into a plan; here one 32-bit word per axis and point:
+//   bits  0..22  mantissa of t = u − (floor(u) − 1) ∈ [1, 2), the local
+//                coordinate the reference's weights are evaluated at
+//                (src/interp3.cpp:2508-2533).  t is exact: u − floor(u) + 1
+//                is a float subtraction without rounding for u ≥ 2;
+//   bits 23..31  signed offset o = (floor(u) − 4) − i of the first stencil
+//                node from the point's own grid index i, taken modulo N
+//                into [−N/2, N/2); nodes are (i + o + a) mod N, a = 0..3.
+// Decoding costs two integer instructions per axis and reproduces the
+// reference's floor/weights bit for bit.  N <= 512 keeps o in 9 bits.
 // ---------------------------------------------------------------------------
-struct QPre {
-    float u[3];
-    float aux;   // per-point coefficient from the plan (adjoint update c)
-    float aux2;  // per-point operand loaded with the query (inc-state s_j)
+constexpr int kPlanMaxN = 512;
+
+struct __attribute__((packed, aligned(4))) PlanPt {
+    uint32_t w[3];
 };
 
+__device__ __forceinline__ float plan_t(uint32_t w) { return __uint_as_float((w & 0x7FFFFFu) | 0x3F800000u); }
+__device__ __forceinline__ int plan_off(uint32_t w) { return static_cast<int>(w) >> 23; }
+
+// Encodes one axis; returns false when u is not a finite, wrapped coordinate.
+__device__ __forceinline__ bool plan_encode(float u, int idx, int n, uint32_t& w, int& off) {
+    if (!(u >= 0.0f && u < static_cast<float>(n + 8))) return false;
+    const int fu = __float2int_rd(u);
+    const float t = u - static_cast<float>(fu - 1);
+    int o = (fu - 4) - idx;
+    o = ((o + n / 2) % n + n) % n - n / 2;
+    off = o;
+    w = (__float_as_uint(t) & 0x7FFFFFu) | (static_cast<uint32_t>(o) << 23);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Epilogues of the trajectory-based gathers: `aux(i)` loads the per-point
+// operand the update needs (issued with the query point), `store` writes
+// the update of f(X).
+// ---------------------------------------------------------------------------
 struct EpiPlain {  // out = f(X)
-    const float4* U;
     float* out;
     int64_t ozstride;
     __device__ void select(int z) { out += z * ozstride; }
-    __device__ QPre load(int64_t i) const {
-        const float4 a = U[i];
-        return {{a.x, a.y, a.z}, 0.f, 0.f};
-    }
-    __device__ void store(int64_t i, float v, const QPre&) const { out[i] = v; }
+    __device__ float aux(int64_t) const { return 0.0f; }
+    __device__ void store(int64_t i, float v, float) const { out[i] = v; }
 };
 
 struct EpiIncStep {  // out = sign·(f(X) − s)   (src/CLAIRE.cpp:3228-3273)
-    const float4* U;
     const float* s;
     float* out;
     float sign;
     __device__ void select(int) {}
-    __device__ QPre load(int64_t i) const {
-        const float4 a = U[i];
-        return {{a.x, a.y, a.z}, 0.f, s[i]};
-    }
-    __device__ void store(int64_t i, float v, const QPre& p) const { out[i] = sign * (v - p.aux2); }
+    __device__ float aux(int64_t i) const { return s[i]; }
+    __device__ void store(int64_t i, float v, float sv) const { out[i] = sign * (v - sv); }
 };
 
 struct EpiAdjStep {  // out = c·f(X)   (src/CLAIRE.cpp:2668-2673, 3955-3962)
-    const float4* U;
+    const float* c;
     float* out;
     __device__ void select(int) {}
-    __device__ QPre load(int64_t i) const {
-        const float4 a = U[i];
-        return {{a.x, a.y, a.z}, a.w, 0.f};
-    }
-    __device__ void store(int64_t i, float v, const QPre& p) const { out[i] = p.aux * v; }
+    __device__ float aux(int64_t i) const { return c[i]; }
+    __device__ void store(int64_t i, float v, float cv) const { out[i] = cv * v; }
 };
 
 }  // namespace gather

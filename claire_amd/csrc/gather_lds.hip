@@ -30,21 +30,27 @@ namespace {
 using namespace gather;
 
 constexpr int kLT2 = 16;        // tile extent in x2
-constexpr int kLT3 = 64;        // tile extent in x3 (one wave per tile row)
+constexpr int kLT3 = 64;        // tile extent in x3
 constexpr int kLC1 = 32;        // planes per block
 constexpr int kLThreads = 512;  // 8 waves; each thread evaluates 2 points per plane
-constexpr int kRowsPerPass = kLThreads / kLT3;
-constexpr int kPts = kLT2 / kRowsPerPass;
-static_assert(kPts == 2, "the contraction below is written for two points per thread");
+constexpr int kPts = 2;
+static_assert(kLT2 * kLT3 == kLThreads * kPts, "one point pair per thread and plane");
 
+// LDS plane geometry for displacement bound D.  A plane holds E2 rows of E3
+// valid columns at a row pitch that is a multiple of 32 dwords: a wave reads
+// two tile rows of 32 points each (one row per half-wave), so lanes whose
+// stencils start on different rows or planes still hit different banks
+// unless their columns collide mod 32.
 template <int D>
 struct LdsGeo {
     static constexpr int HLO = D + 1, HHI = D + 2;
     static constexpr int E2 = kLT2 + HLO + HHI, E3 = kLT3 + HLO + HHI;
-    static constexpr int P = E2 * E3;          // floats per LDS plane
+    static constexpr int PITCH = (E3 + 31) / 32 * 32;
+    static constexpr int P = E2 * PITCH;       // dwords per LDS plane
+    static constexpr int NV = E2 * E3;         // valid elements per plane
     static constexpr int WIN = HLO + HHI + 1;  // planes read while evaluating one output plane
     static constexpr int W = WIN + 1;          // ring slots (one spare for the prefetch)
-    static constexpr int FILL = (P + kLThreads - 1) / kLThreads;
+    static constexpr int FILL = (NV + kLThreads - 1) / kLThreads;
 };
 
 __device__ __forceinline__ int pmod(int a, int n) {
@@ -52,44 +58,18 @@ __device__ __forceinline__ int pmod(int a, int n) {
     return r < 0 ? r + n : r;
 }
 
-__device__ __forceinline__ int wrap_once(int a, int n) {
-    a += (a < 0) ? n : 0;
-    a -= (a >= n) ? n : 0;
-    return a;
-}
-
-// Stencil of one point in LDS coordinates: four plane bases and the weights.
-struct LdsStencil {
-    int base[4];
-    float w1[4], w2[4], w3[4];
-};
-
-template <int D>
-__device__ __forceinline__ void lds_stencil(const QPre& p, const GatherDims& g, int x1, int x1s, int r2, int r3,
-                                            LdsStencil& L) {
-    using G = LdsGeo<D>;
-    Stencil S;
-    setup_stencil_u(p.u, g, S);
-    // Offsets inside the window.  The plan's displacement bound makes them lie
-    // in range; the clamps only keep LDS addresses inside the ring.
-    const int d1 = min(wrap_once(S.a1.idx[0] - (x1 - G::HLO), g.n1), G::WIN - 4);
-    const int l2 = min(wrap_once(S.a2.idx[0] - r2, g.n2), G::E2 - 4);
-    const int l3 = min(wrap_once(S.a3.idx[0] - r3, g.n3), G::E3 - 4);
-    const int cs = (x1 - x1s) % G::W;  // ring slot of plane x1 − HLO
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        int slot = cs + d1 + a;
-        slot -= (slot >= G::W) ? G::W : 0;
-        L.base[a] = slot * G::P + l2 * G::E3 + l3;
-        L.w1[a] = S.a1.w[a];
-        L.w2[a] = S.a2.w[a];
-        L.w3[a] = S.a3.w[a];
-    }
+// Keeps the compiler from splitting an LDS base address into a register part
+// plus a constant too large for the ds_read2 offset fields; the stencil's row
+// and column offsets then all fit the instruction's immediates.
+__device__ __forceinline__ int opaque(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
 }
 
 template <int D, class Epi>
-__global__ __launch_bounds__(kLThreads) void k_gather_lds(GatherDims g, const float* __restrict__ fbase,
-                                                          int64_t fzstride, Epi epi, int nblocks) {
+__global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(D <= 2 ? 4 : 2))) void k_gather_lds(
+    GatherDims g, const PlanPt* __restrict__ plan, const float* __restrict__ fbase, int64_t fzstride, Epi epi,
+    int nblocks) {
     using G = LdsGeo<D>;
     __shared__ float lds[G::W * G::P];
     const int tid = threadIdx.x;
@@ -107,43 +87,56 @@ __global__ __launch_bounds__(kLThreads) void k_gather_lds(GatherDims g, const fl
     const int r3 = t3 * kLT3 - G::HLO;
     const int64_t plane = static_cast<int64_t>(g.n2) * g.n3;
 
-    // Offsets (within a global plane) of the tile elements this thread loads;
-    // elements past the plane size reload element 0 (never stored).
-    int foff[G::FILL];
+    // Offsets (within a global plane and within an LDS plane) of the tile
+    // elements this thread loads; elements past the tile reload element
+    // NV−1 (never stored).
+    int foff[G::FILL], loff[G::FILL];
 #pragma unroll
     for (int k = 0; k < G::FILL; ++k) {
-        const int e = min(tid + kLThreads * k, G::P - 1);
+        const int e = min(tid + kLThreads * k, G::NV - 1);
         const int e2 = e / G::E3;
         const int e3 = e - e2 * G::E3;
         foff[k] = pmod(r2 + e2, g.n2) * g.n3 + pmod(r3 + e3, g.n3);
+        loff[k] = e2 * G::PITCH + e3;
     }
-    const int base_u = x1s - G::HLO;  // unwrapped index of the plane in ring slot 0
 
-    // This thread's two output points within a plane (invalid ones load point
-    // 0 of the plane and skip the store).
-    const int x3 = t3 * kLT3 + tid % kLT3;
-    int off[kPts];
+    // This thread's two output points within a plane: wave w covers tile rows
+    // 2w and 2w+1, 32 columns each, and the two points are 32 columns apart.
+    // Invalid points load point 0 of the plane and skip the store.
+    const int lane = tid & 63;
+    const int lx2 = 2 * (tid >> 6) + (lane >> 5);
+    const int x2 = t2 * kLT2 + lx2;
+    int off[kPts], rc[kPts];
     bool valid[kPts];
 #pragma unroll
     for (int k = 0; k < kPts; ++k) {
-        const int x2 = t2 * kLT2 + tid / kLT3 + kRowsPerPass * k;
+        const int lx3 = (lane & 31) + 32 * k;
+        const int x3 = t3 * kLT3 + lx3;
         valid[k] = x2 < g.n2 && x3 < g.n3;
         off[k] = valid[k] ? x2 * g.n3 + x3 : 0;
+        rc[k] = (lx2 + G::HLO) * G::PITCH + lx3 + G::HLO;
     }
 
-    // prologue: planes x1s−HLO … x1s+HHI and the query points of plane x1s
+    // prologue: planes x1s−HLO … x1s+HHI into ring slots 0 … WIN−1, and the
+    // query points of plane x1s
     for (int k = 0; k < G::WIN; ++k) {
-        const float* src = f + pmod(base_u + k, g.n1) * plane;
+        const float* src = f + pmod(x1s - G::HLO + k, g.n1) * plane;
         float* dst = lds + k * G::P;
 #pragma unroll
         for (int q = 0; q < G::FILL; ++q)
-            if (tid + kLThreads * q < G::P) dst[tid + kLThreads * q] = src[foff[q]];
+            if (tid + kLThreads * q < G::NV) dst[loff[q]] = src[foff[q]];
     }
-    QPre cur[kPts];
+    PlanPt cur[kPts];
+    float caux[kPts];
 #pragma unroll
-    for (int k = 0; k < kPts; ++k) cur[k] = epi.load(static_cast<int64_t>(x1s) * plane + off[k]);
+    for (int k = 0; k < kPts; ++k) {
+        const int64_t i = static_cast<int64_t>(x1s) * plane + off[k];
+        cur[k] = plan[i];
+        caux[k] = epi.aux(i);
+    }
     __syncthreads();
 
+    int cs = 0;  // ring slot of plane x1 − HLO
     for (int x1 = x1s; x1 < x1e; ++x1) {
         // Prefetch for the next step, unconditionally (past the chunk end it
         // reloads a plane of this chunk; the values are never used).
@@ -154,40 +147,75 @@ __global__ __launch_bounds__(kLThreads) void k_gather_lds(GatherDims g, const fl
 #pragma unroll
             for (int q = 0; q < G::FILL; ++q) pre[q] = src[foff[q]];
         }
-        QPre nxt[kPts];
+        PlanPt nxt[kPts];
+        float naux[kPts];
 #pragma unroll
-        for (int k = 0; k < kPts; ++k) nxt[k] = epi.load(static_cast<int64_t>(xn) * plane + off[k]);
+        for (int k = 0; k < kPts; ++k) {
+            const int64_t i = static_cast<int64_t>(xn) * plane + off[k];
+            nxt[k] = plan[i];
+            naux[k] = epi.aux(i);
+        }
 
-        LdsStencil s0, s1;
-        lds_stencil<D>(cur[0], g, x1, x1s, r2, r3, s0);
-        lds_stencil<D>(cur[1], g, x1, x1s, r2, r3, s1);
-        float v0 = 0.f, v1 = 0.f;
+        // Stencils: weights and the LDS address of row (a, b = 0) of each point.
+        // The plan's bound keeps every offset within the ring; the clamps only
+        // pin LDS addresses inside the allocation.
+        float w1[kPts][4], w2[kPts][4], w3[kPts][4];
+        int base[kPts][4], base2[kPts][4];
+#pragma unroll
+        for (int k = 0; k < kPts; ++k) {
+            lagrange_weights(plan_t(cur[k].w[0]), w1[k]);
+            lagrange_weights(plan_t(cur[k].w[1]), w2[k]);
+            lagrange_weights(plan_t(cur[k].w[2]), w3[k]);
+            const int lb = min(max(rc[k] + plan_off(cur[k].w[1]) * G::PITCH + plan_off(cur[k].w[2]), 0),
+                               G::P - 3 * G::PITCH - 4);
+            const int s0 = min(max(cs + G::HLO + plan_off(cur[k].w[0]), 0), 2 * G::W - 4);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                int sa = s0 + a;
+                sa -= (sa >= G::W) ? G::W : 0;
+                // rows 0-1 and 2-3 from two bases: 3·PITCH + 3 exceeds the
+                // 8-bit dword offsets of ds_read2_b32
+                base[k][a] = opaque(sa * G::P + lb);
+                base2[k][a] = opaque(base[k][a] + 2 * G::PITCH);
+            }
+        }
+        float v[kPts];
+#pragma unroll
+        for (int k = 0; k < kPts; ++k) v[k] = 0.f;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            float sa0 = 0.f, sa1 = 0.f;
+            float sa[kPts];
+#pragma unroll
+            for (int k = 0; k < kPts; ++k) sa[k] = 0.f;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const float* p0 = lds + s0.base[a] + b * G::E3;
-                const float* p1 = lds + s1.base[a] + b * G::E3;
-                const float r0 = s0.w3[0] * p0[0] + s0.w3[1] * p0[1] + s0.w3[2] * p0[2] + s0.w3[3] * p0[3];
-                const float r1 = s1.w3[0] * p1[0] + s1.w3[1] * p1[1] + s1.w3[2] * p1[2] + s1.w3[3] * p1[3];
-                sa0 += s0.w2[b] * r0;
-                sa1 += s1.w2[b] * r1;
+#pragma unroll
+                for (int k = 0; k < kPts; ++k) {
+                    const float* p = lds + (b < 2 ? base[k][a] + b * G::PITCH : base2[k][a] + (b - 2) * G::PITCH);
+                    const float r = w3[k][0] * p[0] + w3[k][1] * p[1] + w3[k][2] * p[2] + w3[k][3] * p[3];
+                    sa[k] += w2[k][b] * r;
+                }
             }
-            v0 += s0.w1[a] * sa0;
-            v1 += s1.w1[a] * sa1;
+#pragma unroll
+            for (int k = 0; k < kPts; ++k) v[k] += w1[k][a] * sa[k];
         }
         const int64_t pb = static_cast<int64_t>(x1) * plane;
-        if (valid[0]) epi.store(pb + off[0], v0, cur[0]);
-        if (valid[1]) epi.store(pb + off[1], v1, cur[1]);
+#pragma unroll
+        for (int k = 0; k < kPts; ++k)
+            if (valid[k]) epi.store(pb + off[k], v[k], caux[k]);
 
-        // land the prefetched plane in the spare ring slot
-        float* dst = lds + ((x1 - x1s + G::WIN) % G::W) * G::P;
+        // land the prefetched plane (x1 + HHI + 1) in the spare ring slot
+        const int spare = (cs == 0) ? G::W - 1 : cs - 1;
+        float* dst = lds + spare * G::P;
 #pragma unroll
         for (int q = 0; q < G::FILL; ++q)
-            if (tid + kLThreads * q < G::P) dst[tid + kLThreads * q] = pre[q];
+            if (tid + kLThreads * q < G::NV) dst[loff[q]] = pre[q];
 #pragma unroll
-        for (int k = 0; k < kPts; ++k) cur[k] = nxt[k];
+        for (int k = 0; k < kPts; ++k) {
+            cur[k] = nxt[k];
+            caux[k] = naux[k];
+        }
+        cs = (cs + 1 == G::W) ? 0 : cs + 1;
         __syncthreads();
     }
 }
@@ -266,50 +294,57 @@ __global__ __launch_bounds__(256) void k_body_force(int64_t nl4, int nt, float h
 }
 
 template <int D, class Epi>
-void launch_lds(const GatherDims& g, const float* f, int64_t fzstride, int nz, const Epi& epi, hipStream_t st) {
+void launch_lds(const GatherDims& g, const uint32_t* P, const float* f, int64_t fzstride, int nz, const Epi& epi,
+                hipStream_t st) {
     const int nt3 = (g.n3 + kLT3 - 1) / kLT3;
     const int nt2 = (g.n2 + kLT2 - 1) / kLT2;
     const int nch = (g.n1 + kLC1 - 1) / kLC1;
     const int nb = nt3 * nt2 * nch;
-    hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g, f, fzstride, epi, nb);
+    hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
+                       reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb);
 }
 
 template <class Epi>
-void dispatch(int dbound, const GatherDims& g, const float* f, int64_t fzstride, int nz, const Epi& epi,
-              hipStream_t st) {
+void dispatch(int dbound, const GatherDims& g, const uint32_t* P, const float* f, int64_t fzstride, int nz,
+              const Epi& epi, hipStream_t st) {
     CLR_REQUIRE(g.n1 >= 4 && g.n2 >= 4 && g.n3 >= 4, kErrInvalidArgument, "gather needs n >= 4 per axis");
     CLR_REQUIRE(g.nl < (int64_t(1) << 31), kErrInvalidArgument, "grid too large for 32-bit stencil offsets");
     CLR_REQUIRE(dbound >= 0 && dbound <= 4, kErrInvalidArgument, "displacement bound too large for the LDS gather");
     if (dbound <= 2)
-        launch_lds<2>(g, f, fzstride, nz, epi, st);
+        launch_lds<2>(g, P, f, fzstride, nz, epi, st);
     else
-        launch_lds<4>(g, f, fzstride, nz, epi, st);
+        launch_lds<4>(g, P, f, fzstride, nz, epi, st);
 }
+
+bool use_lds(const TrajectoryPlan& tp) { return tp.dbound >= 0 && tp.dbound <= lds_gather_max_displacement(); }
 
 }  // namespace
 
 int lds_gather_max_displacement() { return 4; }
 
-void launch_lds_gather(int dbound, const GatherDims& g, const float* const f[], int nf, const float* U,
-                       float* const out[], hipStream_t st) {
+void launch_plan_gather(const TrajectoryPlan& tp, const GatherDims& g, const float* const f[], int nf,
+                        float* const out[], hipStream_t st) {
+    if (!use_lds(tp)) return launch_direct_gather(g, tp.X, f, nf, out, st);
     const int64_t fz = nf > 1 ? (f[1] - f[0]) : 0;
     const int64_t oz = nf > 1 ? (out[1] - out[0]) : 0;
     for (int d = 2; d < nf; ++d)
         CLR_REQUIRE(f[d] - f[d - 1] == fz && out[d] - out[d - 1] == oz, kErrInvalidArgument,
                     "vector gather needs equally spaced components");
-    dispatch(dbound, g, f[0], fz, nf, EpiPlain{reinterpret_cast<const float4*>(U), out[0], oz}, st);
+    dispatch(tp.dbound, g, tp.P, f[0], fz, nf, EpiPlain{out[0], oz}, st);
 }
 
-void launch_lds_inc_state_step(int dbound, const GatherDims& g, const float* U, const float* mt, const float* sj,
-                               float* out, int negate, hipStream_t st) {
+void launch_plan_inc_state_step(const TrajectoryPlan& tp, const GatherDims& g, const float* mt, const float* sj,
+                                float* out, int negate, hipStream_t st) {
+    if (!use_lds(tp)) return launch_direct_inc_state_step(g, tp.X, mt, sj, out, negate, st);
     CLR_REQUIRE(mt != out, kErrInvalidArgument, "semi-Lagrangian step cannot run in place");
-    dispatch(dbound, g, mt, 0, 1, EpiIncStep{reinterpret_cast<const float4*>(U), sj, out, negate ? -1.0f : 1.0f}, st);
+    dispatch(tp.dbound, g, tp.P, mt, 0, 1, EpiIncStep{sj, out, negate ? -1.0f : 1.0f}, st);
 }
 
-void launch_lds_adjoint_step(int dbound, const GatherDims& g, const float* U, const float* lam, float* out,
-                             hipStream_t st) {
+void launch_plan_adjoint_step(const TrajectoryPlan& tp, const GatherDims& g, const float* C, const float* lam,
+                              float* out, hipStream_t st) {
+    if (!use_lds(tp)) return launch_direct_adjoint_step(g, tp.X, C, lam, out, st);
     CLR_REQUIRE(lam != out, kErrInvalidArgument, "semi-Lagrangian step cannot run in place");
-    dispatch(dbound, g, lam, 0, 1, EpiAdjStep{reinterpret_cast<const float4*>(U), out}, st);
+    dispatch(tp.dbound, g, tp.P, lam, 0, 1, EpiAdjStep{C, out}, st);
 }
 
 void launch_body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const float* gm, float* const b[3],

@@ -5,9 +5,9 @@
 // exchange accfft_get_ghost_xyz (src/ghost.cpp:555-612) on one GPU.  The
 // arithmetic per point is in gather_device.hpp.  Kernels here:
 //   * the fused RK2 characteristic (Euler predictor kept in registers);
-//   * the plan: wrapped, rescaled query points u = q·N + 3 stored once per
-//     trajectory as float4, together with the exact per-axis displacement
-//     bound that decides whether the LDS-staged gathers can be used;
+//   * the packed plan of a trajectory (12 B per point, gather_device.hpp),
+//     together with the exact per-axis displacement bound that decides
+//     whether the LDS-staged gathers can be used;
 //   * direct gathers (one thread per point, stencil rows read from global
 //     memory as unaligned 16-byte loads) for arbitrary query points and for
 //     trajectories whose displacement exceeds the LDS window.
@@ -102,12 +102,13 @@ __global__ __launch_bounds__(kThreads) void k_traj_rk2(GatherDims g, HH hh, Traj
     }
 }
 
-// Plan of a trajectory: U[i] = (u1, u2, u3, 0) with u = wrap(q)·N + 3, and the
-// smallest D such that every stencil lies within [i − D − 1, i + D + 2] per
-// axis (the window of the LDS-staged gathers), computed from the same floor
-// that the gathers use.
+// Packed plan of a trajectory (gather_device.hpp) and the smallest D such
+// that every stencil lies within [i − D − 1, i + D + 2] per axis (the window
+// of the LDS-staged gathers).  Points that cannot be encoded (non-finite
+// coordinates) set D to 1 << 20, which routes every gather along this
+// trajectory to the direct kernels on the raw foot points.
 __global__ __launch_bounds__(kThreads) void k_make_plan(GatherDims g, HH hh, const float* __restrict__ X,
-                                                        float4* __restrict__ U, int* __restrict__ dmax, int nblocks) {
+                                                        PlanPt* __restrict__ P, int* __restrict__ dmax, int nblocks) {
     __shared__ int sh[kThreads];
     const int64_t i = point_index(nblocks);
     int need = 0;
@@ -116,20 +117,19 @@ __global__ __launch_bounds__(kThreads) void k_make_plan(GatherDims g, HH hh, con
         const int idx[3] = {static_cast<int>(i / plane), static_cast<int>((i / g.n3) % g.n2),
                             static_cast<int>(i % g.n3)};
         const int n[3] = {g.n1, g.n2, g.n3};
-        float u[3];
+        PlanPt pt;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            u[a] = index_coord(X[3 * i + a], n[a], hh.h[a]);
-            if (!(u[a] >= 0.0f && u[a] < static_cast<float>(n[a] + 8))) {
-                need = 1 << 20;  // non-finite or not wrapped: direct gathers only
+            const float u = index_coord(X[3 * i + a], n[a], hh.h[a]);
+            int lo = 0;
+            if (!plan_encode(u, idx[a], n[a], pt.w[a], lo)) {
+                need = 1 << 20;
+                pt.w[a] = 0u;  // t = 1, o = 0: never read
                 continue;
             }
-            int lo = stencil_start(u[a]) - idx[a];
-            // nearest periodic representative in [−n/2, n/2)
-            lo = ((lo + n[a] / 2) % n[a] + n[a]) % n[a] - n[a] / 2;
             need = max(need, max(-lo - 1, lo + 1));
         }
-        U[i] = make_float4(u[0], u[1], u[2], 0.0f);
+        P[i] = pt;
     }
     sh[threadIdx.x] = need;
     __syncthreads();
@@ -143,40 +143,44 @@ __global__ __launch_bounds__(kThreads) void k_make_plan(GatherDims g, HH hh, con
 __global__ void k_zero_int(int* p) { *p = 0; }
 
 // Adjoint update coefficient c = 1 + ht/2·(dX + (1 + ht·dX)·d), d = ∇·v,
-// dX = d(X_a), stored in the w component of the adjoint plan.  The SL adjoint
-// step λ⁺ = λX + ht/2·(λX·dX + (λX + ht·λX·dX)·d) (src/CLAIRE.cpp:2668-2673,
-// 3955-3962) is then λ⁺ = c·λX.
-__global__ __launch_bounds__(kThreads) void k_adjoint_coeff(GatherDims g, float4* __restrict__ U,
-                                                            const float* __restrict__ divv, float ht, int nblocks) {
+// dX = d(X_a).  The SL adjoint step λ⁺ = λX + ht/2·(λX·dX + (λX + ht·λX·dX)·d)
+// (src/CLAIRE.cpp:2668-2673, 3955-3962) is then λ⁺ = c·λX.
+__global__ __launch_bounds__(kThreads) void k_adjoint_coeff(GatherDims g, HH hh, const float* __restrict__ X,
+                                                            const float* __restrict__ divv, float ht,
+                                                            float* __restrict__ C, int nblocks) {
     const int64_t i = point_index(nblocks);
     if (i >= g.nl) return;
-    float4 a = U[i];
-    const float u[3] = {a.x, a.y, a.z};
+    float q[3];
+    load_q3(X, i, q);
     Stencil S;
-    setup_stencil_u(u, g, S);
+    setup_stencil(q, g, hh.h, S);
     const float* fs[1] = {divv};
     float dx[1];
     eval_stencil<1>(fs, S, g, dx);
     const float d = divv[i];
     const float hthalf = 0.5f * ht;
-    a.w = 1.0f + hthalf * (dx[0] + (1.0f + ht * dx[0]) * d);
-    U[i] = a;
+    C[i] = 1.0f + hthalf * (dx[0] + (1.0f + ht * dx[0]) * d);
 }
 
+// Gather along a trajectory straight from the raw foot points (any
+// displacement).
 template <class Epi>
-__global__ __launch_bounds__(kThreads) void k_gather_direct(GatherDims g, const float* __restrict__ fbase,
-                                                            int64_t fzstride, Epi epi, int nblocks) {
+__global__ __launch_bounds__(kThreads) void k_gather_direct(GatherDims g, HH hh, const float* __restrict__ X,
+                                                            const float* __restrict__ fbase, int64_t fzstride, Epi epi,
+                                                            int nblocks) {
     const int64_t i = point_index(nblocks);
     if (i >= g.nl) return;
     const float* f = fbase + static_cast<int64_t>(blockIdx.y) * fzstride;
     epi.select(blockIdx.y);
-    const QPre p = epi.load(i);
+    const float aux = epi.aux(i);
+    float q[3];
+    load_q3(X, i, q);
     Stencil S;
-    setup_stencil_u(p.u, g, S);
+    setup_stencil(q, g, hh.h, S);
     const float* fs[1] = {f};
     float r[1];
     eval_stencil<1>(fs, S, g, r);
-    epi.store(i, r[0], p);
+    epi.store(i, r[0], aux);
 }
 
 __global__ __launch_bounds__(kThreads) void k_body_force_static(int64_t nl, float hd, const float* __restrict__ lam,
@@ -198,10 +202,11 @@ void check_gather_dims(const GatherDims& g) {
 }
 
 template <class Epi>
-void launch_direct(const GatherDims& g, const float* f, int64_t fz, int nz, const Epi& e, hipStream_t st) {
+void launch_direct(const GatherDims& g, const float* X, const float* f, int64_t fz, int nz, const Epi& e,
+                   hipStream_t st) {
     check_gather_dims(g);
     const int nb = nblocks_for(g.nl);
-    hipLaunchKernelGGL((k_gather_direct<Epi>), dim3(nb, nz), dim3(kThreads), 0, st, g, f, fz, e, nb);
+    hipLaunchKernelGGL((k_gather_direct<Epi>), dim3(nb, nz), dim3(kThreads), 0, st, g, make_hh(g), X, f, fz, e, nb);
 }
 
 }  // namespace
@@ -234,11 +239,13 @@ void launch_trajectory_rk2(const Dims& d, const float* const v[3], float sigma, 
     hipLaunchKernelGGL(k_traj_rk2, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), tp, p, X, nb);
 }
 
-int launch_make_plan(const GatherDims& g, const float* X, float* U, int* scratch, hipStream_t st) {
+int launch_make_plan(const GatherDims& g, const float* X, uint32_t* P, int* scratch, hipStream_t st) {
     check_gather_dims(g);
+    CLR_REQUIRE(g.n1 <= kPlanMaxN && g.n2 <= kPlanMaxN && g.n3 <= kPlanMaxN, kErrUnsupportedSize,
+                "trajectory plans support at most 512 points per axis");
     const int nb = nblocks_for(g.nl);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, scratch);
-    hipLaunchKernelGGL(k_make_plan, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), X, reinterpret_cast<float4*>(U),
+    hipLaunchKernelGGL(k_make_plan, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), X, reinterpret_cast<PlanPt*>(P),
                        scratch, nb);
     int h = 0;
     CLR_HIP_CHECK(hipMemcpyAsync(&h, scratch, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -246,31 +253,33 @@ int launch_make_plan(const GatherDims& g, const float* X, float* U, int* scratch
     return h;
 }
 
-void launch_adjoint_coeff(const GatherDims& g, float* U, const float* divv, float ht, hipStream_t st) {
+void launch_adjoint_coeff(const GatherDims& g, const float* X, const float* divv, float ht, float* C,
+                          hipStream_t st) {
     check_gather_dims(g);
     const int nb = nblocks_for(g.nl);
-    hipLaunchKernelGGL(k_adjoint_coeff, dim3(nb), dim3(kThreads), 0, st, g, reinterpret_cast<float4*>(U), divv, ht, nb);
+    hipLaunchKernelGGL(k_adjoint_coeff, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), X, divv, ht, C, nb);
 }
 
-void launch_direct_gather(const GatherDims& g, const float* const f[], int nf, const float* U, float* const out[],
+void launch_direct_gather(const GatherDims& g, const float* X, const float* const f[], int nf, float* const out[],
                           hipStream_t st) {
     const int64_t fz = nf > 1 ? (f[1] - f[0]) : 0;
     const int64_t oz = nf > 1 ? (out[1] - out[0]) : 0;
     for (int d = 2; d < nf; ++d)
         CLR_REQUIRE(f[d] - f[d - 1] == fz && out[d] - out[d - 1] == oz, kErrInvalidArgument,
                     "vector gather needs equally spaced components");
-    launch_direct(g, f[0], fz, nf, EpiPlain{reinterpret_cast<const float4*>(U), out[0], oz}, st);
+    launch_direct(g, X, f[0], fz, nf, EpiPlain{out[0], oz}, st);
 }
 
-void launch_direct_inc_state_step(const GatherDims& g, const float* U, const float* mt, const float* sj, float* out,
+void launch_direct_inc_state_step(const GatherDims& g, const float* X, const float* mt, const float* sj, float* out,
                                   int negate, hipStream_t st) {
     CLR_REQUIRE(mt != out, kErrInvalidArgument, "semi-Lagrangian step cannot run in place");
-    launch_direct(g, mt, 0, 1, EpiIncStep{reinterpret_cast<const float4*>(U), sj, out, negate ? -1.0f : 1.0f}, st);
+    launch_direct(g, X, mt, 0, 1, EpiIncStep{sj, out, negate ? -1.0f : 1.0f}, st);
 }
 
-void launch_direct_adjoint_step(const GatherDims& g, const float* U, const float* lam, float* out, hipStream_t st) {
+void launch_direct_adjoint_step(const GatherDims& g, const float* X, const float* C, const float* lam, float* out,
+                                hipStream_t st) {
     CLR_REQUIRE(lam != out, kErrInvalidArgument, "semi-Lagrangian step cannot run in place");
-    launch_direct(g, lam, 0, 1, EpiAdjStep{reinterpret_cast<const float4*>(U), out}, st);
+    launch_direct(g, X, lam, 0, 1, EpiAdjStep{C, out}, st);
 }
 
 void launch_body_force_static(int64_t nl, float hd, const float* lam, const float* gm, float* const b[3],

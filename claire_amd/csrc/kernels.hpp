@@ -47,9 +47,8 @@ class FFTPlan3D {
 };
 
 // ---------------------------------------------------------------------------
-// Semi-Lagrangian gathers (interp.hip).  X: normalized query points,
-// interleaved xyz (3 floats per grid point).  X4: xyz + one per-point
-// coefficient (4 floats per point).
+// Semi-Lagrangian gathers (interp.hip, gather_lds.hip).  X: normalized query
+// points, interleaved xyz (3 floats per grid point).
 // ---------------------------------------------------------------------------
 struct GatherDims {
     int n1, n2, n3;
@@ -66,33 +65,47 @@ void launch_gather3(const GatherDims& g, const float* const f[3], const float* X
 //   X̃ = (x − σ·ht·v)/2π,  X = (x − σ·ht/2·(v(X̃) + v))/2π
 void launch_trajectory_rk2(const Dims& d, const float* const v[3], float sigma, float ht, float* X, hipStream_t st);
 
-// Plan of a trajectory: U (4 floats per point) = (u1, u2, u3, 0), u = q·N + 3
-// after the periodic wrap.  Returns the per-axis stencil displacement bound D
-// in cells (synchronizes the stream); the LDS-staged gathers need
-// D <= lds_gather_max_displacement().  `scratch` is one device int.
-int launch_make_plan(const GatherDims& g, const float* X, float* U, int* scratch, hipStream_t st);
+// Packed plan of a trajectory: 3 words per point (12 B), one per axis, the
+// fraction t ∈ [1,2) of the rescaled coordinate u = q·N + 3 and the offset of
+// the first stencil node from the point (layout in gather_device.hpp).
+// Returns the per-axis stencil displacement bound D in cells (synchronizes
+// the stream); 1 << 20 when a foot point is not finite.  n <= 512 per axis.
+// `scratch` is one device int.
+int launch_make_plan(const GatherDims& g, const float* X, uint32_t* P, int* scratch, hipStream_t st);
+// The LDS-staged gathers need D <= lds_gather_max_displacement().
 int lds_gather_max_displacement();
-// Adjoint plan coefficient: U.w = c = 1 + ht/2·(dX + (1 + ht·dX)·d) with
-// d = ∇·v, dX = d(X_a); the SL adjoint update is then λ⁺ = c·λ(X_a).
-void launch_adjoint_coeff(const GatherDims& g, float* U, const float* divv, float ht, hipStream_t st);
+// Adjoint update coefficient per point: C = 1 + ht/2·(dX + (1 + ht·dX)·d)
+// with d = ∇·v, dX = d(X_a); the SL adjoint update is then λ⁺ = C·λ(X_a).
+void launch_adjoint_coeff(const GatherDims& g, const float* X, const float* divv, float ht, float* C,
+                          hipStream_t st);
 
-// Gathers along a plan.  Vector fields are nf equally spaced component pointers.
+// A trajectory as the gathers see it: the packed plan, the raw foot points
+// (3 floats per point, /2π) and the plan's displacement bound.
+struct TrajectoryPlan {
+    const uint32_t* P;
+    const float* X;
+    int dbound;
+};
+
+// Gathers along a trajectory.  Vector fields are nf equally spaced component
+// pointers.  They run the LDS-staged kernels when dbound allows, else the
+// direct kernels on the raw foot points.
 //   gather:          out_d = f_d(X)
 //   inc_state_step:  out = ±(f(X) − s)   (m̃^{j+1} = m̃^j(X_s) − s_j; negated for λ̃⁰ = −m̃^nt)
-//   adjoint_step:    out = c·f(X)        (λ^{j+1} = c·λ^j(X_a))
-// LDS-staged versions (dbound = plan's D <= lds_gather_max_displacement()):
-void launch_lds_gather(int dbound, const GatherDims& g, const float* const f[], int nf, const float* U,
-                       float* const out[], hipStream_t st);
-void launch_lds_inc_state_step(int dbound, const GatherDims& g, const float* U, const float* mt, const float* sj,
-                               float* out, int negate, hipStream_t st);
-void launch_lds_adjoint_step(int dbound, const GatherDims& g, const float* U, const float* lam, float* out,
-                             hipStream_t st);
-// Direct versions (any displacement):
-void launch_direct_gather(const GatherDims& g, const float* const f[], int nf, const float* U, float* const out[],
+//   adjoint_step:    out = C·f(X)        (λ^{j+1} = C·λ^j(X_a))
+void launch_plan_gather(const TrajectoryPlan& tp, const GatherDims& g, const float* const f[], int nf,
+                        float* const out[], hipStream_t st);
+void launch_plan_inc_state_step(const TrajectoryPlan& tp, const GatherDims& g, const float* mt, const float* sj,
+                                float* out, int negate, hipStream_t st);
+void launch_plan_adjoint_step(const TrajectoryPlan& tp, const GatherDims& g, const float* C, const float* lam,
+                              float* out, hipStream_t st);
+// Direct versions on the raw foot points X (any displacement):
+void launch_direct_gather(const GatherDims& g, const float* X, const float* const f[], int nf, float* const out[],
                           hipStream_t st);
-void launch_direct_inc_state_step(const GatherDims& g, const float* U, const float* mt, const float* sj, float* out,
+void launch_direct_inc_state_step(const GatherDims& g, const float* X, const float* mt, const float* sj, float* out,
                                   int negate, hipStream_t st);
-void launch_direct_adjoint_step(const GatherDims& g, const float* U, const float* lam, float* out, hipStream_t st);
+void launch_direct_adjoint_step(const GatherDims& g, const float* X, const float* C, const float* lam, float* out,
+                                hipStream_t st);
 
 // b_d = hd·Σ_{j=0..nt} w_j·λ^j·∂_d m^{nt−j} (trapezoid weights); lam holds
 // λ^0..λ^nt, gm holds ∇m^0..∇m^nt (3 fields each).
