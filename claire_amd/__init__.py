@@ -64,6 +64,23 @@ def _vp(p):
     return ctypes.c_void_p(p)
 
 
+class GNOptions(ctypes.Structure):
+    """clr_gn_options (include/claire_amd.h)."""
+    _fields_ = [("maxit", ctypes.c_int), ("gttol", ctypes.c_double), ("gatol", ctypes.c_double),
+                ("krylov_maxit", ctypes.c_int), ("krylov_rtol", ctypes.c_double), ("forcing", ctypes.c_int),
+                ("precond", ctypes.c_int), ("linesearch", ctypes.c_int)]
+
+
+class GNReport(ctypes.Structure):
+    """clr_gn_report (include/claire_amd.h)."""
+    _fields_ = [(n, ctypes.c_int) for n in ("iterations", "reason", "matvecs", "pcg_iterations",
+                                            "objective_evals", "gradient_evals")] + \
+               [(n, ctypes.c_double) for n in ("J0", "D0", "gnorm0", "J", "D", "R", "gnorm")]
+
+
+GN_REASONS = {0: "continue", 1: "gttol", 2: "gatol", 3: "steptol", -1: "maxits", -2: "nan", -3: "linesearch"}
+
+
 class DeviceArray:
     """Device allocation owned by Python (clr_device_malloc/free)."""
 
@@ -218,6 +235,28 @@ class Claire:
         g = self.buf("g", (3,) + self.nx)
         _check(lib().clr_evaluate_gradient(self.ctx, vp, _vp(g.ptr)))
         return g.download()
+
+    def solve_gn(self, v0=None, hist_len=128, **options):
+        """Gauss–Newton–Krylov registration (clr_solve_gn) from v0 (zero by
+        default).  Options override the reference defaults (maxit, gttol,
+        gatol, krylov_maxit, krylov_rtol, forcing, precond, linesearch).
+        Returns (v, report dict, history rows (iter, J, D, R, ‖g‖, ‖g‖/‖g₀‖,
+        pcg_its, step))."""
+        opt = GNOptions()
+        _check(lib().clr_gn_options_default(ctypes.byref(opt)))
+        for k, val in options.items():
+            if not hasattr(opt, k):
+                raise TypeError(f"unknown option {k}")
+            setattr(opt, k, val)
+        v = self.buf("gn_v", (3,) + self.nx)
+        v.upload(np.zeros((3,) + self.nx, np.float32) if v0 is None else v0)
+        rep = GNReport()
+        hist = np.zeros((hist_len, 8), dtype=np.float64)
+        _check(lib().clr_solve_gn(self.ctx, _vp(v.ptr), ctypes.byref(opt), ctypes.byref(rep),
+                                  hist.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(hist_len)))
+        r = {name: getattr(rep, name) for name, _ in GNReport._fields_}
+        r["reason"] = GN_REASONS.get(r["reason"], r["reason"])
+        return v.download(), r, hist[: min(r["iterations"] + 1, hist_len)]
 
     # -- regularization -----------------------------------------------------
     def apply_inv_reg(self, x, applysqrt=False):

@@ -6,6 +6,7 @@
 
 #include "../../include/claire_amd.h"
 #include "claire.hpp"
+#include "solver.hpp"
 
 using claire_amd::Error;
 
@@ -233,6 +234,40 @@ int clr_evaluate_gradient(clr_ctx* ctx, const float* v, float* g) {
         need(g, "g");
         C(ctx).EvaluateGradient(g, v);
         check_hip_launch();
+    });
+}
+
+int clr_gn_options_default(clr_gn_options* opt) {
+    return guard([&] {
+        need(opt, "opt");
+        const reg::GNOptions d;
+        *opt = clr_gn_options{d.maxit, d.gttol, d.gatol, d.krylov_maxit, d.krylov_rtol, d.forcing, d.precond,
+                              d.linesearch};
+    });
+}
+
+int clr_solve_gn(clr_ctx* ctx, float* v, const clr_gn_options* opt, clr_gn_report* report, double* history,
+                 int history_len) {
+    return guard([&] {
+        need(v, "v");
+        need(opt, "opt");
+        need(report, "report");
+        CLR_REQUIRE(history_len >= 0, claire_amd::kErrInvalidArgument, "negative history length");
+        reg::GNOptions o;
+        o.maxit = opt->maxit;
+        o.gttol = opt->gttol;
+        o.gatol = opt->gatol;
+        o.krylov_maxit = opt->krylov_maxit;
+        o.krylov_rtol = opt->krylov_rtol;
+        o.forcing = opt->forcing;
+        o.precond = opt->precond;
+        o.linesearch = opt->linesearch;
+        reg::Optimizer optimizer(&C(ctx));
+        reg::GNReport r;
+        optimizer.Run(v, o, r, history, history ? history_len : 0);
+        check_hip_launch();
+        *report = clr_gn_report{r.iterations, r.reason, r.matvecs, r.pcg_iterations, r.objective_evals,
+                                r.gradient_evals, r.J0, r.D0, r.gnorm0, r.J, r.D, r.R, r.gnorm};
     });
 }
 

@@ -153,10 +153,29 @@ __device__ __forceinline__ void twiddle_butterfly(c32 (&v)[V], int t, const c32*
     }
 }
 
+// LDS layouts of the exchanges: lay(n, c) is the slot of element n of the
+// block's column (or row pair) c.
+//   ColLayout: columns interleaved with an odd pitch; the lanes of a wave run
+//              over c, so any row access hits distinct banks.
+//   RowLayout: each row contiguous with one pad slot per 16 elements; the
+//              lanes of a wave run over n, and the radix-16 strides n = 16t + r
+//              and n = t + 16r both map the 16 lanes of a row to 16 distinct
+//              bank pairs.
+struct ColLayout {
+    int LD;
+    __device__ __forceinline__ int operator()(int n, int c) const { return n * LD + c; }
+};
+
+template <int N>
+struct RowLayout {
+    static constexpr int CP = N + N / 16;
+    __device__ __forceinline__ int operator()(int n, int c) const { return c * CP + n + (n >> 4); }
+};
+
 // Write stage outputs (span NS, radix R) to LDS, then load the inputs of the
-// next radix-R2 stage.  s[n·LD + c] holds element n of the block's column c.
-template <int N, int V, int NS, int R, int R2>
-__device__ __forceinline__ void exchange(c32 (&v)[V], c32* s, int LD, int c, int t) {
+// next radix-R2 stage.
+template <int N, int V, int NS, int R, int R2, class Lay>
+__device__ __forceinline__ void exchange(c32 (&v)[V], c32* s, const Lay& lay, int c, int t) {
     constexpr int T = N / V;
     __syncthreads();
 #pragma unroll
@@ -164,14 +183,14 @@ __device__ __forceinline__ void exchange(c32 (&v)[V], c32* s, int LD, int c, int
         const int j = t + T * q;
         const int base = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-        for (int r = 0; r < R; ++r) s[(base + r * NS) * LD + c] = v[q * R + r];
+        for (int r = 0; r < R; ++r) s[lay(base + r * NS, c)] = v[q * R + r];
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < V / R2; ++q) {
         const int j = t + T * q;
 #pragma unroll
-        for (int r = 0; r < R2; ++r) v[q * R2 + r] = s[(j + r * (N / R2)) * LD + c];
+        for (int r = 0; r < R2; ++r) v[q * R2 + r] = s[lay(j + r * (N / R2), c)];
     }
 }
 
@@ -180,26 +199,29 @@ struct Stages;
 
 template <int N, int V, int DIR, int NS, int R>
 struct Stages<N, V, DIR, NS, Radices<R>> {
-    __device__ __forceinline__ static void run(c32 (&v)[V], c32*, int, int, int t, const c32* tw) {
+    template <class Lay>
+    __device__ __forceinline__ static void run(c32 (&v)[V], c32*, const Lay&, int, int t, const c32* tw) {
         twiddle_butterfly<N, V, DIR, NS, R>(v, t, tw);
     }
 };
 
 template <int N, int V, int DIR, int NS, int R, int R2, int... RS>
 struct Stages<N, V, DIR, NS, Radices<R, R2, RS...>> {
-    __device__ __forceinline__ static void run(c32 (&v)[V], c32* s, int LD, int c, int t, const c32* tw) {
+    template <class Lay>
+    __device__ __forceinline__ static void run(c32 (&v)[V], c32* s, const Lay& lay, int c, int t, const c32* tw) {
         twiddle_butterfly<N, V, DIR, NS, R>(v, t, tw);
-        exchange<N, V, NS, R, R2>(v, s, LD, c, t);
-        Stages<N, V, DIR, NS * R, Radices<R2, RS...>>::run(v, s, LD, c, t, tw);
+        exchange<N, V, NS, R, R2>(v, s, lay, c, t);
+        Stages<N, V, DIR, NS * R, Radices<R2, RS...>>::run(v, s, lay, c, t, tw);
     }
 };
 
-template <int N, int DIR>
-__device__ __forceinline__ void fft_regs(c32 (&v)[Plan<N>::V], c32* s, int LD, int c, int t, const c32* tw) {
+template <int N, int DIR, class Lay>
+__device__ __forceinline__ void fft_regs(c32 (&v)[Plan<N>::V], c32* s, const Lay& lay, int c, int t,
+                                         const c32* tw) {
     if constexpr (DIR < 0)
-        Stages<N, Plan<N>::V, DIR, 1, typename Plan<N>::Fwd>::run(v, s, LD, c, t, tw);
+        Stages<N, Plan<N>::V, DIR, 1, typename Plan<N>::Fwd>::run(v, s, lay, c, t, tw);
     else
-        Stages<N, Plan<N>::V, DIR, 1, typename Plan<N>::Inv>::run(v, s, LD, c, t, tw);
+        Stages<N, Plan<N>::V, DIR, 1, typename Plan<N>::Inv>::run(v, s, lay, c, t, tw);
 }
 
 // Block geometry: B columns (or row pairs) per block, T threads per column.
@@ -209,8 +231,9 @@ struct Geo {
     static constexpr int T = N / V;
     static constexpr int B = (T >= 32) ? 8 : ((T >= 16) ? 16 : (256 / T));
     static constexpr int THREADS = B * T;
-    static constexpr int LD = B + 1;  // LDS row pitch (odd: spreads banks)
-    static constexpr int LDS_ELEMS = N * LD;
+    static constexpr int LD = B + 1;  // ColLayout pitch (odd: spreads banks)
+    static constexpr int ROW_ELEMS = B * RowLayout<N>::CP;
+    static constexpr int LDS_ELEMS = (N * LD > ROW_ELEMS) ? N * LD : ROW_ELEMS;
 };
 
 __device__ __forceinline__ int wavenum(int k, int n) { return k > n / 2 ? k - n : (k == n / 2 ? 0 : k); }
@@ -275,20 +298,21 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_row_r2c(const float* __rest
             v[q * R0 + r] = {va ? pa[n] : 0.f, vb ? pb[n] : 0.f};
         }
     }
-    fft_regs<N, -1>(v, s, G::LD, c, t, tw);
+    const RowLayout<N> lay;
+    fft_regs<N, -1>(v, s, lay, c, t, tw);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < V / RL; ++q) {
         const int j = t + T * q;
 #pragma unroll
-        for (int r = 0; r < RL; ++r) s[(j + r * (N / RL)) * G::LD + c] = v[q * RL + r];
+        for (int r = 0; r < RL; ++r) s[lay(j + r * (N / RL), c)] = v[q * RL + r];
     }
     __syncthreads();
     c32* oa = out + ra * NH;
     c32* ob = out + rb * NH;
     for (int k = t; k < NH; k += T) {
-        const c32 zk = s[k * G::LD + c];
-        const c32 zm = s[((N - k) & (N - 1)) * G::LD + c];
+        const c32 zk = s[lay(k, c)];
+        const c32 zm = s[lay((N - k) & (N - 1), c)];
         if (va) oa[k] = {0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y)};
         if (vb) ob[k] = {0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x)};
     }
@@ -337,7 +361,7 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_row_c2r(const c32* __restri
             v[q * R0 + r] = {A.x - Bv.y, A.y + Bv.x};
         }
     }
-    fft_regs<N, +1>(v, s, G::LD, c, t, tw);
+    fft_regs<N, +1>(v, s, RowLayout<N>{}, c, t, tw);
     float* oa = out + ra * N;
     float* ob = out + rb * N;
     const float* aa = add ? add + ra * N : nullptr;
@@ -378,7 +402,12 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_col(const c32* __restrict__
     constexpr int V = G::V, T = G::T, R0 = First<L>::value, RL = Last<L>::value;
     const int c = threadIdx.x % G::B;
     const int t = threadIdx.x / G::B;
-    const int64_t cg = static_cast<int64_t>(blockIdx.x) * G::B + c;
+    // Blocks are dealt round-robin over the 8 XCDs; remap them so each XCD
+    // owns one contiguous range of columns and the cache lines shared by
+    // neighbouring column groups (rows are not 128-byte aligned) stay in one L2.
+    const unsigned nb = gridDim.x;
+    const unsigned b = (nb % 8 == 0) ? (blockIdx.x & 7u) * (nb >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int64_t cg = static_cast<int64_t>(b) * G::B + c;
     const bool valid = cg < geo.ncols;
     const int64_t base = valid ? col_base<N>(geo, cg) : 0;
     c32 v[V];
@@ -389,7 +418,7 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_col(const c32* __restrict__
         for (int r = 0; r < R0; ++r)
             v[q * R0 + r] = valid ? in[base + static_cast<int64_t>(j + r * (N / R0)) * geo.estride] : c32{0.f, 0.f};
     }
-    fft_regs<N, DIR>(v, s, G::LD, c, t, tw);
+    fft_regs<N, DIR>(v, s, ColLayout{G::LD}, c, t, tw);
     if (valid) {
 #pragma unroll
         for (int q = 0; q < V / RL; ++q) {
@@ -437,12 +466,12 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_x1_fused(Ptr3 ptr, ColGeom 
             for (int r = 0; r < R0; ++r)
                 v[q * R0 + r] = valid ? src[base + static_cast<int64_t>(j + r * (N / R0)) * geo.estride] : c32{0.f, 0.f};
         }
-        fft_regs<N, -1>(v, s, G::LD, c, t, tw);
+        fft_regs<N, -1>(v, s, ColLayout{G::LD}, c, t, tw);
     };
     // After the forward pass v[q·RL + r] = X[k1], k1 = j + r·N/RL; the
     // inverse plan starts with radix RL and expects the same layout.
     auto inv_store = [&](c32 (&v)[V], c32* dst) {
-        fft_regs<N, +1>(v, s, G::LD, c, t, tw);
+        fft_regs<N, +1>(v, s, ColLayout{G::LD}, c, t, tw);
         constexpr int RI = Last<typename Plan<N>::Inv>::value;
         if (valid) {
 #pragma unroll

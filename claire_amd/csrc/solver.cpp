@@ -1,0 +1,176 @@
+// solver.cpp -- Gauss–Newton–Krylov driver on the device (see solver.hpp).
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace reg {
+
+using namespace claire_amd;
+
+Optimizer::Optimizer(CLAIRE* problem)
+    : m_Prob(problem), m_N(3 * problem->Opt().domain.nl), m_Stream(problem->Opt().stream) {
+    const size_t n = static_cast<size_t>(m_N);
+    m_G.resize(n);
+    m_D.resize(n);
+    m_R.resize(n);
+    m_Z.resize(n);
+    m_P.resize(n);
+    m_W.resize(n);
+    m_Vt.resize(n);
+}
+
+double Optimizer::Dot(const float* a, const float* b) { return m_Prob->Red()->Dot(a, b, m_N); }
+
+// KSPCG on H d = −g, zero initial guess, unpreconditioned residual
+// norm: converged when ‖r‖ <= max(rtol·‖b‖, 1e-16), diverged when
+// ‖r‖ >= 1e6·‖b‖, at most krylov_maxit − 1 iterations
+// (KSPSetTolerances, src/Optimizer.cpp:298-307); stops on a non-positive
+// curvature pᵀHp (indefinite operator).  Returns the iteration count; the
+// solution is in m_D.
+int Optimizer::Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& rep) {
+    float* d = m_D.get();
+    float* r = m_R.get();
+    float* z = m_Z.get();
+    float* p = m_P.get();
+    float* w = m_W.get();
+    launch_fill(d, 0.0f, m_N, m_Stream);
+    launch_copy(g, r, m_N, m_Stream);
+    launch_scale(r, -1.0f, m_N, m_Stream);
+    const double bnorm = std::sqrt(Dot(r, r));
+    const double tol = std::max(rtol * bnorm, 1e-16);
+    const int kmax = std::max(0, opt.krylov_maxit - 1);
+    if (!(bnorm > tol)) return 0;
+    auto precond = [&](const float* in, float* out) {
+        if (opt.precond)
+            m_Prob->ApplyInvRegularizationOperator(out, in, false);
+        else
+            launch_copy(in, out, m_N, m_Stream);
+    };
+    precond(r, z);
+    double rz = Dot(r, z);
+    int its = 0;
+    for (int k = 1; k <= kmax; ++k) {
+        if (k == 1) launch_copy(z, p, m_N, m_Stream);
+        m_Prob->HessianMatVec(w, p, true);
+        rep.matvecs++;
+        its = k;
+        const double pw = Dot(p, w);
+        if (!(pw > 0.0)) break;
+        const double a = rz / pw;
+        launch_axpy(static_cast<float>(a), p, d, m_N, m_Stream);
+        launch_axpy(static_cast<float>(-a), w, r, m_N, m_Stream);
+        const double rnorm = std::sqrt(Dot(r, r));
+        if (rnorm <= tol || !(rnorm < 1e6 * bnorm)) break;
+        precond(r, z);
+        const double rz_new = Dot(r, z);
+        const double beta = rz_new / rz;
+        rz = rz_new;
+        // p = z + β·p
+        launch_waxpy(static_cast<float>(beta), p, z, p, m_N, m_Stream);
+    }
+    return its;
+}
+
+int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, int hist_len) {
+    CLR_REQUIRE(v != nullptr, kErrInvalidArgument, "null velocity");
+    CLR_REQUIRE(opt.maxit >= 1 && opt.krylov_maxit >= 1, kErrInvalidArgument, "iteration limits must be >= 1");
+    rep = GNReport{};
+    float* g = m_G.get();
+    float* d = m_D.get();
+    float* vt = m_Vt.get();
+
+    // ‖g₀‖ at v = 0 (InitializeOptimization, src/CLAIRE.cpp:320-335)
+    m_Zero.resize(static_cast<size_t>(m_N));
+    launch_fill(m_Zero.get(), 0.0f, m_N, m_Stream);
+    m_Prob->EvaluateGradient(g, m_Zero.get());
+    rep.gradient_evals++;
+    rep.gnorm0 = std::sqrt(Dot(g, g));
+    const double g0 = rep.gnorm0 > 0.0 ? rep.gnorm0 : 1.0;
+
+    double J, D, R;
+    m_Prob->EvaluateObjective(&J, &D, &R, v);
+    rep.objective_evals++;
+    m_Prob->EvaluateGradient(g, nullptr);  // same velocity: reuses the state
+    rep.gradient_evals++;
+    double gnorm = std::sqrt(Dot(g, g));
+    rep.J0 = J;
+    rep.D0 = D;
+    double kg0 = -1.0;  // ‖g‖ at the first Krylov solve (KrylovMethod.g0norm)
+    double step = 1.0;
+    int iter = 0;
+    for (;;) {
+        double* h = (hist && iter < hist_len) ? hist + 8 * iter : nullptr;
+        if (h) {
+            h[0] = iter; h[1] = J; h[2] = D; h[3] = R; h[4] = gnorm; h[5] = gnorm / g0; h[6] = 0; h[7] = 0;
+        }
+        // CheckConvergenceGrad (src/TaoInterface.cpp:421-480)
+        if (!std::isfinite(J) || !std::isfinite(gnorm)) { rep.reason = kGNNan; break; }
+        if (step < 1.0 / 1024.0) { rep.reason = kGNSteptol; break; }
+        if (gnorm < opt.gttol * g0) { rep.reason = kGNGttol; break; }
+        if (gnorm < opt.gatol) { rep.reason = kGNGatol; break; }
+        if (iter > opt.maxit - 1) { rep.reason = kGNMaxits; break; }
+
+        // Eisenstat–Walker forcing (src/KrylovInterface.cpp:120-160)
+        if (kg0 < 0.0) kg0 = gnorm;
+        double rtol = opt.krylov_rtol;
+        if (opt.forcing != 0 && kg0 > 0.0) {
+            const double q = gnorm / kg0;
+            rtol = std::max(1e-16, std::min(0.5, opt.forcing == 1 ? q : std::sqrt(q)));
+        }
+        // the current velocity is v (set by the last objective/gradient call)
+        const int its = Pcg(g, rtol, opt, rep);
+        rep.pcg_iterations += its;
+        if (h) h[6] = its;
+
+        // descent check, Armijo line search, steepest-descent fallback
+        double gd = Dot(g, d);
+        bool newton = true;
+        if (!(gd < 0.0)) {
+            launch_copy(g, d, m_N, m_Stream);
+            launch_scale(d, -1.0f, m_N, m_Stream);
+            gd = -gnorm * gnorm;
+            newton = false;
+        }
+        double alpha = 1.0, Jt = 0.0, Dt = 0.0, Rt = 0.0;
+        bool ok = false;
+        for (;;) {
+            alpha = 1.0;
+            const int trials = opt.linesearch ? 30 : 1;
+            for (int k = 0; k < trials; ++k) {
+                launch_waxpy(static_cast<float>(alpha), d, v, vt, m_N, m_Stream);
+                m_Prob->EvaluateObjective(&Jt, &Dt, &Rt, vt);
+                rep.objective_evals++;
+                if (!opt.linesearch || (std::isfinite(Jt) && Jt <= J + 1e-4 * alpha * gd)) {
+                    ok = true;
+                    break;
+                }
+                alpha *= 0.5;
+            }
+            if (ok || !newton) break;
+            launch_copy(g, d, m_N, m_Stream);
+            launch_scale(d, -1.0f, m_N, m_Stream);
+            gd = -gnorm * gnorm;
+            newton = false;
+        }
+        if (!ok) {
+            rep.reason = kGNLineSearch;
+            m_Prob->SetControlVariable(v);
+            break;
+        }
+        launch_copy(vt, v, m_N, m_Stream);
+        J = Jt; D = Dt; R = Rt;
+        m_Prob->EvaluateGradient(g, nullptr);  // at the accepted trial velocity
+        rep.gradient_evals++;
+        gnorm = std::sqrt(Dot(g, g));
+        step = alpha;
+        if (h) h[7] = alpha;
+        ++iter;
+    }
+    rep.iterations = iter;
+    rep.J = J; rep.D = D; rep.R = R; rep.gnorm = gnorm;
+    CLR_HIP_CHECK(hipStreamSynchronize(m_Stream));
+    return 0;
+}
+
+}  // namespace reg

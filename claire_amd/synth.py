@@ -100,3 +100,29 @@ def random_field(nx, seed=0, smooth=4, dtype=np.float32):
     v = bandlimited_velocity(nx, kmax=smooth, cfl=1.0, nt=1, seed=seed, dtype=np.float64)[0]
     v /= np.abs(v).max()
     return v.astype(dtype)
+
+
+def rescale(x, xminout=0.0, xmaxout=1.0):
+    """Intensity rescaling of one image, restated from Rescale with nc = 1
+    (src/CLAIREUtils.cpp:691-720): shift by −min only when min < −1e-6, then
+    scale by xmaxout/(max − min) when max − min > 1e-6."""
+    x = np.asarray(x)
+    xmin, xmax = float(x.min()), float(x.max())
+    y = x - xmin if -xmin > 1e-6 else x.copy()
+    scale = xmaxout / (xmax - xmin) if abs(xmax - xmin) > 1e-6 else 1.0
+    return (y * scale).astype(x.dtype)
+
+
+def smooth_template_problem(nx, dtype=np.float32):
+    """The reference's synthetic problem 4 (CLAIREBase::SetupSyntheticProb,
+    src/CLAIREBase.cpp:1094-1095, 1123-1139, 1209): template
+    mT = (sin²x1 + sin²x2 + sin²x3)/3 rescaled to [0, 1] and the velocity
+    v = (sin x3·cos x2·sin x2, sin x1·cos x3·sin x3, sin x2·cos x1·sin x1).
+    The reference image is mT transported by v (nt steps), rescaled."""
+    x1, x2, x3 = coords(nx, np.float64)
+    shape = tuple(nx)
+    mT = np.broadcast_to((np.sin(x1) ** 2 + np.sin(x2) ** 2 + np.sin(x3) ** 2) / 3.0, shape)
+    v = np.stack([np.broadcast_to(np.sin(x3) * np.cos(x2) * np.sin(x2), shape),
+                  np.broadcast_to(np.sin(x1) * np.cos(x3) * np.sin(x3), shape),
+                  np.broadcast_to(np.sin(x2) * np.cos(x1) * np.sin(x1), shape)])
+    return rescale(mT.astype(dtype)), v.astype(dtype)

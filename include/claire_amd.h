@@ -117,6 +117,48 @@ int clr_evaluate_gradient(clr_ctx* ctx, const float* v, float* g);
  * out may alias in. */
 int clr_apply_inv_reg(clr_ctx* ctx, const float* in, float* out, int applysqrt);
 
+/* Gauss–Newton–Krylov registration on the device: replaces Optimizer::Run
+ * + TaoSolve (src/Optimizer.cpp:448-527) with the reference's default
+ * setup: TAO "nls" with the matrix-free Hessian (MatShell, src/Optimizer.cpp:
+ * 433-437), KSPCG on the unpreconditioned residual with at most
+ * krylov_maxit − 1 iterations (src/Optimizer.cpp:298-311), the INVREG
+ * preconditioner (src/Preconditioner.cpp:521-573), Eisenstat–Walker forcing
+ * (PreKrylovSolve, src/KrylovInterface.cpp:79-175), Armijo line search and
+ * the gradient stopping test CheckConvergenceGrad (src/TaoInterface.cpp:
+ * 390-512) with ‖g₀‖ taken at v = 0 (src/CLAIRE.cpp:320-335).
+ * Template and reference images must be set.  v (device, 3·nl) holds the
+ * initial guess on entry and the solution on exit.  history (host, may be
+ * NULL) receives 8 doubles per iterate, up to history_len iterates:
+ * iter, J, D, R, ‖g‖, ‖g‖/‖g₀‖, PCG iterations and step length of the step
+ * taken from that iterate.  Inner products are FP64. */
+typedef struct clr_gn_options {
+    int maxit;           /* OptPara.maxiter (50) */
+    double gttol;        /* OptPara.tol[2] (5e-2): stop when ‖g‖ < gttol·‖g₀‖ */
+    double gatol;        /* OptPara.tol[0] (1e-6): stop when ‖g‖ < gatol */
+    int krylov_maxit;    /* KrylovMethod.maxiter (50) */
+    double krylov_rtol;  /* KrylovMethod.tol[0] (1e-12), used when forcing = 0 */
+    int forcing;         /* 0 none, 1 quadratic, 2 superlinear (default) */
+    int precond;         /* 0 none, 1 inverse regularization (default) */
+    int linesearch;      /* 0 unit step, 1 Armijo (default) */
+} clr_gn_options;
+
+#define CLR_GN_GTTOL 1      /* ‖g‖ < gttol·‖g₀‖ (TAO_CONVERGED_GTTOL) */
+#define CLR_GN_GATOL 2      /* ‖g‖ < gatol (TAO_CONVERGED_GATOL) */
+#define CLR_GN_STEPTOL 3    /* line-search step < 2^-10 (TAO_CONVERGED_STEPTOL) */
+#define CLR_GN_MAXITS (-1)  /* iteration limit (TAO_DIVERGED_MAXITS) */
+#define CLR_GN_NAN (-2)     /* objective or ‖g‖ not finite (TAO_DIVERGED_NAN) */
+#define CLR_GN_LINESEARCH (-3)  /* no step satisfied the Armijo condition */
+
+typedef struct clr_gn_report {
+    int iterations, reason, matvecs, pcg_iterations, objective_evals, gradient_evals;
+    double J0, D0, gnorm0, J, D, R, gnorm;
+} clr_gn_report;
+
+/* Fills the reference's defaults (RegOpt::Initialize, src/RegOpt.cpp:1068-1180). */
+int clr_gn_options_default(clr_gn_options* opt);
+int clr_solve_gn(clr_ctx* ctx, float* v, const clr_gn_options* opt, clr_gn_report* report, double* history,
+                 int history_len);
+
 /* Regularization::EvaluateGradient == HessianMatVec (3·nl → 3·nl). */
 int clr_reg_apply(clr_ctx* ctx, const float* v, float* out);
 /* Regularization::EvaluateFunctional */

@@ -33,6 +33,7 @@
 #include <stdexcept>
 #include <vector>
 #include <algorithm>
+#include <array>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -801,6 +802,191 @@ void objective(const Grid<T>& g, int nt, int norm, const T* beta, const T* const
     *D = distance_sl2(g, nt, m.data(), mR);
     *R = velocity_is_zero(g, v) ? 0.0 : reg_functional(g, norm, beta, v);
     *J = *D + *R;
+}
+
+// ---------------------------------------------------------------------------
+// Gauss–Newton–Krylov solve, the reference's default optimizer:
+//   TAO "nls" with a matrix-free Hessian (src/Optimizer.cpp:235-444),
+//   KSPCG with the unpreconditioned residual norm, zero initial guess,
+//   at most maxiter−1 iterations (src/Optimizer.cpp:298-311), preconditioned
+//   by the inverse regularization operator (PCSHELL → ApplySpectralPrecond,
+//   src/Preconditioner.cpp:521-573, applysqrt = false);
+//   Eisenstat–Walker forcing (PreKrylovSolve, src/KrylovInterface.cpp:79-175):
+//   rtol = max(1e-16, min(0.5, (‖g‖/‖g₀‖)^{1/2})) (superlinear) or the ratio
+//   itself (quadratic), ‖g₀‖ the norm at the first Krylov solve;
+//   Armijo backtracking (TaoLineSearch "armijo": σ = 1e-4, step halving,
+//   at most 30 trials, initial step 1);
+//   stopping test CheckConvergenceGrad (src/TaoInterface.cpp:390-512) with
+//   ‖g₀‖ the gradient norm at v = 0 (InitializeOptimization,
+//   src/CLAIRE.cpp:320-335), step < 2^-10 → stop, iter > maxit−1 → stop.
+// If the Krylov direction is not a descent direction, or its line search
+// fails, the step falls back to steepest descent −g (TAO NLS gradient step).
+// Inner products accumulate in double.
+// history: 8 doubles per iterate: iter, J, D, R, ‖g‖, ‖g‖/‖g₀‖, PCG
+// iterations and step length of the step taken from this iterate.
+// ---------------------------------------------------------------------------
+struct GNOptions {
+    int maxit = 50;            // OptPara.maxiter (src/RegOpt.cpp, default 50)
+    double gttol = 5e-2;       // OptPara.tol[2]
+    double gatol = 1e-6;       // OptPara.tol[0]
+    int krylov_maxit = 50;     // KrylovMethod.maxiter
+    double krylov_rtol = 1e-12;  // KrylovMethod.tol[0] (used without forcing)
+    int forcing = 2;           // 0 none, 1 quadratic, 2 superlinear (SLFS default)
+    int precond = 1;           // 0 none, 1 inverse regularization (INVREG default)
+    int linesearch = 1;        // 0 unit step, 1 Armijo (ARMIJOLS default)
+};
+
+struct GNReport {
+    int iterations = 0, reason = 0, matvecs = 0, pcg_iterations = 0, objective_evals = 0, gradient_evals = 0;
+    double J0 = 0, D0 = 0, gnorm0 = 0, J = 0, D = 0, R = 0, gnorm = 0;
+};
+
+enum GNReason { kGNContinue = 0, kGNGttol = 1, kGNGatol = 2, kGNSteptol = 3, kGNMaxits = -1, kGNNan = -2,
+                kGNLineSearch = -3 };
+
+template <class T>
+double dot3(const Grid<T>& g, const T* const a[3], const T* const b[3]) {
+    double s = 0.0;
+    for (int d = 0; d < 3; ++d)
+        for (int64_t i = 0; i < g.nl; ++i) s += static_cast<double>(a[d][i]) * static_cast<double>(b[d][i]);
+    return s;
+}
+
+template <class T>
+void gn_solve(const Grid<T>& g, int nt, int norm, const T* beta, T* const v[3], const T* mT, const T* mR,
+              const GNOptions& o, GNReport& rep, double* hist, int hist_len) {
+    const int64_t nl = g.nl;
+    const T ht = T(1) / static_cast<T>(nt);
+    auto field3 = [&]() { return std::vector<T>(3 * nl); };
+    auto ptrs = [&](std::vector<T>& a) { return std::array<T*, 3>{a.data(), a.data() + nl, a.data() + 2 * nl}; };
+    std::vector<T> gv = field3(), d = field3(), r = field3(), z = field3(), p = field3(), w = field3(), vt = field3();
+    auto G = ptrs(gv), Dd = ptrs(d), Rr = ptrs(r), Z = ptrs(z), P = ptrs(p), W = ptrs(w), VT = ptrs(vt);
+    auto cst = [](const std::array<T*, 3>& a) { return std::array<const T*, 3>{a[0], a[1], a[2]}; };
+    const std::array<const T*, 3> V{v[0], v[1], v[2]};
+
+    // ‖g₀‖ at v = 0 (InitializeOptimization)
+    {
+        std::vector<T> zero = field3();
+        auto Zr = ptrs(zero);
+        gradient(g, nt, norm, beta, cst(Zr).data(), mT, mR, G.data());
+        rep.gnorm0 = std::sqrt(dot3(g, cst(G).data(), cst(G).data()));
+        rep.gradient_evals++;
+    }
+    double J, D, R;
+    objective(g, nt, norm, beta, V.data(), mT, mR, &J, &D, &R);
+    rep.objective_evals++;
+    gradient(g, nt, norm, beta, V.data(), mT, mR, G.data());
+    rep.gradient_evals++;
+    double gnorm = std::sqrt(dot3(g, cst(G).data(), cst(G).data()));
+    rep.J0 = J;
+    rep.D0 = D;
+    const double g0 = rep.gnorm0 > 0.0 ? rep.gnorm0 : 1.0;
+    double kg0 = -1.0;  // ‖g‖ at the first Krylov solve
+    double step = 1.0;
+    int iter = 0;
+    std::vector<T> m((nt + 1) * nl), Xs(3 * nl), Xa(3 * nl);
+    for (;;) {
+        double* h = (iter < hist_len && hist) ? hist + 8 * iter : nullptr;
+        if (h) {
+            h[0] = iter; h[1] = J; h[2] = D; h[3] = R; h[4] = gnorm; h[5] = gnorm / g0; h[6] = 0; h[7] = 0;
+        }
+        // CheckConvergenceGrad
+        if (!std::isfinite(J) || !std::isfinite(gnorm)) { rep.reason = kGNNan; break; }
+        if (step < 1.0 / 1024.0) { rep.reason = kGNSteptol; break; }
+        if (gnorm < o.gttol * g0) { rep.reason = kGNGttol; break; }
+        if (gnorm < o.gatol) { rep.reason = kGNGatol; break; }
+        if (iter > o.maxit - 1) { rep.reason = kGNMaxits; break; }
+
+        // Newton system H d = −g at the current iterate
+        solve_state(g, nt, V.data(), mT, m.data());
+        trajectory_rk2(g, V.data(), ht, +1, Xs.data());
+        trajectory_rk2(g, V.data(), ht, -1, Xa.data());
+        if (kg0 < 0.0) kg0 = gnorm;
+        double rtol = o.krylov_rtol;
+        if (o.forcing != 0 && kg0 > 0.0) {
+            const double q = gnorm / kg0;
+            rtol = std::max(1e-16, std::min(0.5, o.forcing == 1 ? q : std::sqrt(q)));
+        }
+        auto apply_pc = [&](const std::array<const T*, 3>& in, const std::array<T*, 3>& out) {
+            if (o.precond) {
+                reg_inverse(g, norm, beta, in.data(), out.data(), false);
+            } else {
+                for (int c = 0; c < 3; ++c) std::copy(in[c], in[c] + nl, out[c]);
+            }
+        };
+        // KSPCG, x0 = 0, b = −g
+        for (size_t i = 0; i < d.size(); ++i) { d[i] = T(0); r[i] = -gv[i]; }
+        const double bnorm = std::sqrt(dot3(g, cst(Rr).data(), cst(Rr).data()));
+        const double tol = std::max(rtol * bnorm, 1e-16);
+        const int kmax = std::max(0, o.krylov_maxit - 1);
+        int its = 0;
+        double rnorm = bnorm;
+        if (rnorm > tol) {
+            apply_pc(cst(Rr), Z);
+            double rz = dot3(g, cst(Rr).data(), cst(Z).data());
+            for (int k = 1; k <= kmax; ++k) {
+                if (k == 1) p = z;  // later directions are updated at the end of the iteration
+                hessian_matvec_x(g, nt, norm, beta, V.data(), Xs.data(), Xa.data(), m.data(), cst(P).data(), W.data(),
+                                 true);
+                rep.matvecs++;
+                its = k;
+                const double pw = dot3(g, cst(P).data(), cst(W).data());
+                if (!(pw > 0.0)) break;  // indefinite or breakdown
+                const double a = rz / pw;
+                for (size_t i = 0; i < d.size(); ++i) {
+                    d[i] += static_cast<T>(a) * p[i];
+                    r[i] -= static_cast<T>(a) * w[i];
+                }
+                rnorm = std::sqrt(dot3(g, cst(Rr).data(), cst(Rr).data()));
+                if (rnorm <= tol || !(rnorm < 1e6 * bnorm)) break;
+                apply_pc(cst(Rr), Z);
+                const double rz_new = dot3(g, cst(Rr).data(), cst(Z).data());
+                const double bta = rz_new / rz;
+                rz = rz_new;
+                for (size_t i = 0; i < p.size(); ++i) p[i] = z[i] + static_cast<T>(bta) * p[i];
+            }
+        }
+        rep.pcg_iterations += its;
+        if (h) h[6] = its;
+
+        // descent check, line search, gradient-step fallback
+        double gd = dot3(g, cst(G).data(), cst(Dd).data());
+        bool newton = true;
+        if (!(gd < 0.0)) {
+            for (size_t i = 0; i < d.size(); ++i) d[i] = -gv[i];
+            gd = -gnorm * gnorm;
+            newton = false;
+        }
+        double alpha = 1.0, Jt = 0.0, Dt = 0.0, Rt = 0.0;
+        bool ok = false;
+        for (;;) {
+            alpha = 1.0;
+            const int trials = o.linesearch ? 30 : 1;
+            for (int k = 0; k < trials; ++k) {
+                for (int c = 0; c < 3; ++c)
+                    for (int64_t i = 0; i < nl; ++i) VT[c][i] = v[c][i] + static_cast<T>(alpha) * Dd[c][i];
+                objective(g, nt, norm, beta, cst(VT).data(), mT, mR, &Jt, &Dt, &Rt);
+                rep.objective_evals++;
+                if (!o.linesearch || (std::isfinite(Jt) && Jt <= J + 1e-4 * alpha * gd)) { ok = true; break; }
+                alpha *= 0.5;
+            }
+            if (ok || !newton) break;
+            for (size_t i = 0; i < d.size(); ++i) d[i] = -gv[i];
+            gd = -gnorm * gnorm;
+            newton = false;
+        }
+        if (!ok) { rep.reason = kGNLineSearch; break; }
+        for (int c = 0; c < 3; ++c) std::copy(VT[c], VT[c] + nl, v[c]);
+        J = Jt; D = Dt; R = Rt;
+        gradient(g, nt, norm, beta, V.data(), mT, mR, G.data());
+        rep.gradient_evals++;
+        gnorm = std::sqrt(dot3(g, cst(G).data(), cst(G).data()));
+        step = alpha;
+        if (h) h[7] = alpha;
+        ++iter;
+    }
+    rep.iterations = iter;
+    rep.J = J; rep.D = D; rep.R = R; rep.gnorm = gnorm;
 }
 
 }  // namespace oracle

@@ -220,6 +220,34 @@ void oracle_set_num_threads(int n) {
             const T* v[3] = {v1, v2, v3};                                                                           \
             oracle::objective(g, nt, norm, b, v, mT, mR, J, D, R);                                                  \
         });                                                                                                        \
+    }                                                                                                              \
+    /* iopts: maxit, krylov_maxit, forcing, precond, linesearch; dopts: gttol, gatol, krylov_rtol;              */ \
+    /* irep: iterations, reason, matvecs, pcg_iterations, objective_evals, gradient_evals;                      */ \
+    /* drep: J0, D0, gnorm0, J, D, R, gnorm; v is the initial guess on entry and the solution on exit.          */ \
+    int oracle_gn_solve_##SFX(const int* nx, int nt, int norm, const double* beta, T* v1, T* v2, T* v3,          \
+                              const T* mT, const T* mR, const int* iopts, const double* dopts, int* irep,          \
+                              double* drep, double* hist, int hist_len) {                                          \
+        return guard([&] {                                                                                         \
+            oracle::Grid<T> g(nx);                                                                                  \
+            const T b[3] = {static_cast<T>(beta[0]), static_cast<T>(beta[1]), static_cast<T>(beta[2])};           \
+            T* v[3] = {v1, v2, v3};                                                                                 \
+            oracle::GNOptions o;                                                                                    \
+            o.maxit = iopts[0];                                                                                     \
+            o.krylov_maxit = iopts[1];                                                                              \
+            o.forcing = iopts[2];                                                                                   \
+            o.precond = iopts[3];                                                                                   \
+            o.linesearch = iopts[4];                                                                                \
+            o.gttol = dopts[0];                                                                                     \
+            o.gatol = dopts[1];                                                                                     \
+            o.krylov_rtol = dopts[2];                                                                               \
+            oracle::GNReport r;                                                                                     \
+            oracle::gn_solve(g, nt, norm, b, v, mT, mR, o, r, hist, hist_len);                                      \
+            const int ir[6] = {r.iterations, r.reason, r.matvecs, r.pcg_iterations, r.objective_evals,             \
+                               r.gradient_evals};                                                                   \
+            const double dr[7] = {r.J0, r.D0, r.gnorm0, r.J, r.D, r.R, r.gnorm};                                    \
+            std::copy(ir, ir + 6, irep);                                                                            \
+            std::copy(dr, dr + 7, drep);                                                                            \
+        });                                                                                                        \
     }
 
 ORACLE_DEFINE(double, f64)

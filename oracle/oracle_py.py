@@ -236,3 +236,27 @@ def hessian_bench(v, m, vt, nt, regnorm, beta, reps=1, prec="f32"):
           _p(v[0]), _p(v[1]), _p(v[2]), _p(m), _p(vt[0]), _p(vt[1]), _p(vt[2]), _p(h[0]), _p(h[1]), _p(h[2]),
           ctypes.c_int(reps), ctypes.byref(sec))
     return h, sec.value
+
+
+GN_REASONS = {0: "continue", 1: "gttol", 2: "gatol", 3: "steptol", -1: "maxits", -2: "nan", -3: "linesearch"}
+
+
+def gn_solve(v0, mT, mR, nt, regnorm, beta, maxit=50, gttol=5e-2, gatol=1e-6, krylov_maxit=50,
+             krylov_rtol=1e-12, forcing=2, precond=1, linesearch=1, prec="f64", hist_len=128):
+    """Gauss–Newton–Krylov registration (the reference's default TAO NLS +
+    KSPCG + INVREG + Eisenstat–Walker + Armijo).  Returns (v, report, history)
+    with history rows (iter, J, D, R, ‖g‖, ‖g‖/‖g₀‖, pcg_its, step)."""
+    v = np.ascontiguousarray(np.array(v0, dtype=_dt(prec), copy=True))
+    mT, mR = _c(mT, prec), _c(mR, prec)
+    iopts = (ctypes.c_int * 5)(maxit, krylov_maxit, forcing, precond, linesearch)
+    dopts = (ctypes.c_double * 3)(gttol, gatol, krylov_rtol)
+    irep = (ctypes.c_int * 6)()
+    drep = (ctypes.c_double * 7)()
+    hist = np.zeros((hist_len, 8), dtype=np.float64)
+    _call("oracle_gn_solve", prec, _nx(mT.shape), ctypes.c_int(nt), ctypes.c_int(REG[regnorm]), _beta(beta),
+          _p(v[0]), _p(v[1]), _p(v[2]), _p(mT), _p(mR), iopts, dopts, irep, drep, _p(hist), ctypes.c_int(hist_len))
+    rep = dict(zip(["iterations", "reason", "matvecs", "pcg_iterations", "objective_evals", "gradient_evals"],
+                   list(irep)))
+    rep.update(zip(["J0", "D0", "gnorm0", "J", "D", "R", "gnorm"], list(drep)))
+    rep["reason"] = GN_REASONS.get(rep["reason"], rep["reason"])
+    return v, rep, hist[: min(rep["iterations"] + 1, hist_len)]

@@ -252,3 +252,19 @@ def test_hessian_is_derivative_of_gradient_at_zero_residual(oracle):
               - oracle.gradient(v - eps * vt, mT, mR, nt, "h1sn", beta)) / (2 * eps)
         errs.append(relerr(H, fd))
     assert errs[0] < 5e-2 and errs[1] < 5e-3 and errs[1] < errs[0] / 4
+
+
+def test_gauss_newton_krylov_invariants(oracle):
+    """GN–Krylov on the reference's synthetic problem 4 at 16³: the objective
+    decreases monotonically, ‖g‖ falls below gttol·‖g₀‖ (the stop reason),
+    and the FP32 twin takes the same Newton and PCG iterations as FP64."""
+    shape, nt, beta = (16, 16, 16), 4, (1e-2, 1e-4, 0.0)
+    mT, vstar = synth.smooth_template_problem(shape, np.float64)
+    mR = synth.rescale(oracle.solve_state(vstar, mT, nt)[nt])
+    v, rep, hist = oracle.gn_solve(np.zeros((3,) + shape), mT, mR, nt, "h1sn", beta, maxit=10)
+    assert rep["reason"] == "gttol" and rep["gnorm"] < 5e-2 * rep["gnorm0"]
+    assert np.all(np.diff(hist[:, 1]) < 0)  # J_k decreasing
+    v32, rep32, hist32 = oracle.gn_solve(np.zeros((3,) + shape), mT, mR, nt, "h1sn", beta, maxit=10, prec="f32")
+    assert rep32["iterations"] == rep["iterations"] and rep32["pcg_iterations"] == rep["pcg_iterations"]
+    assert abs(rep32["J"] - rep["J"]) < 1e-5 * rep["J"]
+    assert relerr(v32, v) < 1e-4
