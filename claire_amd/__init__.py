@@ -124,7 +124,7 @@ class DeviceArray:
 class Claire:
     """One registration problem (GN, semi-Lagrangian, nc = 1) on the GPU."""
 
-    def __init__(self, nx, nt=4, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0), stream=None):
+    def __init__(self, nx, nt=4, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0), stream=None, precision="fp32"):
         self.nx = tuple(int(n) for n in nx)
         self.nt = int(nt)
         self.nl = int(np.prod(self.nx))
@@ -135,6 +135,8 @@ class Claire:
                                 ctypes.c_int(REG[regnorm]), (ctypes.c_double * 3)(*b[:3]), _vp(stream or 0)))
         self.ctx = ctx
         self._bufs = {}
+        if precision != "fp32":
+            self.set_precision(precision)
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -235,6 +237,11 @@ class Claire:
         g = self.buf("g", (3,) + self.nx)
         _check(lib().clr_evaluate_gradient(self.ctx, vp, _vp(g.ptr)))
         return g.download()
+
+    def set_precision(self, precision):
+        """"fp32" (the reference's path) or "mixed" (FP16 storage of the
+        Newton-invariant gradient fields in the Hessian matvec)."""
+        _check(lib().clr_set_precision(self.ctx, ctypes.c_int({"fp32": 0, "mixed": 1}[precision])))
 
     def solve_gn(self, v0=None, hist_len=128, **options):
         """Gauss–Newton–Krylov registration (clr_solve_gn) from v0 (zero by

@@ -237,6 +237,10 @@ int clr_evaluate_gradient(clr_ctx* ctx, const float* v, float* g) {
     });
 }
 
+int clr_set_precision(clr_ctx* ctx, int precision) {
+    return guard([&] { C(ctx).SetPrecision(precision); });
+}
+
 int clr_gn_options_default(clr_gn_options* opt) {
     return guard([&] {
         need(opt, "opt");

@@ -504,7 +504,8 @@ size_t CLAIRE::DeviceBytes() const {
     const size_t nl = static_cast<size_t>(m_Opt.domain.nl), nt = static_cast<size_t>(m_Opt.nt);
     const size_t floats = 5 * 3 * nl + 2 * nl + (nt + 1) * nl + nl + 3 * nl + 4 * nl + (nt + 1) * 3 * nl +
                           nt * 3 * nl + nt * nl + nt * nl + nl + 2 * 3 * nl;
-    return floats * 4 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
+    const size_t halves = m_Opt.precision == 1 ? (2 * nt + 1) * 3 * nl : 0;
+    return floats * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
 }
 
 void CLAIRE::RequireVelocity() const { CLR_REQUIRE(m_HaveVelocity, kErrState, "velocity field not set"); }
@@ -512,7 +513,7 @@ void CLAIRE::RequireVelocity() const { CLR_REQUIRE(m_HaveVelocity, kErrState, "v
 int CLAIRE::SetControlVariable(const float* v) {
     m_VelocityField->SetComponents(v);
     m_HaveVelocity = true;
-    m_TrajValid = m_StateValid = m_GradMValid = m_GradMXValid = false;
+    m_TrajValid = m_StateValid = m_GradMValid = m_GradMXValid = m_HalfValid = false;
     // CLAIREBase::IsVelocityZero (src/CLAIREBase.cpp:578-605): ∞-norm == 0
     m_VelocityIsZero = m_Red->AbsMax(m_VelocityField->m_X1, 3 * m_Opt.domain.nl) == 0.0;
     return 0;
@@ -529,7 +530,7 @@ int CLAIRE::SetTemplateImage(const float* mT) {
     CLR_HIP_CHECK(hipMemcpyAsync(m_TemplateImage.get(), mT, sizeof(float) * m_Opt.domain.nl, hipMemcpyDeviceToDevice,
                                  m_Opt.stream));
     m_HaveTemplate = true;
-    m_StateValid = m_GradMValid = m_GradMXValid = false;
+    m_StateValid = m_GradMValid = m_GradMXValid = m_HalfValid = false;
     return 0;
 }
 
@@ -574,7 +575,7 @@ int CLAIRE::SolveStateEquation() {
             m_SL->Gather(m + (j + 1) * nl, m + j * nl, "state");
     }
     m_StateValid = true;
-    m_GradMValid = m_GradMXValid = false;
+    m_GradMValid = m_GradMXValid = m_HalfValid = false;
     return 0;
 }
 
@@ -588,7 +589,7 @@ void CLAIRE::EnsureStateGradients() {
         m_Spectral->Grad(g, m_State.get() + j * nl);
     }
     m_GradMValid = true;
-    m_GradMXValid = false;
+    m_GradMXValid = m_HalfValid = false;
 }
 
 void CLAIRE::EnsureGradientsAtStateTrajectory() {
@@ -606,15 +607,34 @@ void CLAIRE::EnsureGradientsAtStateTrajectory() {
     m_GradMXValid = true;
 }
 
+void CLAIRE::EnsureHalfGradients() {
+    EnsureGradientsAtStateTrajectory();
+    if (m_HalfValid) return;
+    const int64_t nl = m_Opt.domain.nl;
+    const int nt = m_Opt.nt;
+    m_GradM16.resize(static_cast<size_t>((nt + 1) * 3 * nl));
+    m_GradMX16.resize(static_cast<size_t>(nt * 3 * nl));
+    launch_to_half(m_GradM.get(), m_GradM16.get(), (nt + 1) * 3 * nl, m_Opt.stream);
+    launch_to_half(m_GradMX.get(), m_GradMX16.get(), nt * 3 * nl, m_Opt.stream);
+    m_HalfValid = true;
+}
+
+int CLAIRE::SetPrecision(int precision) {
+    CLR_REQUIRE(precision == 0 || precision == 1, kErrInvalidArgument, "precision must be 0 (FP32) or 1 (mixed)");
+    m_Opt.precision = precision;
+    return 0;
+}
+
 int CLAIRE::PrepareHessian() {
     EnsureTrajectories();
     EnsureGradientsAtStateTrajectory();
+    if (m_Opt.precision == 1) EnsureHalfGradients();
     CLR_HIP_CHECK(hipStreamSynchronize(m_Opt.stream));
     return 0;
 }
 
 // λ^0 given in m_Lam slot 0; writes b (×hd) and, when lam_out is set, λ at t = 0.
-void CLAIRE::IncAdjointCore(const float* lt0, VecField* bt) {
+void CLAIRE::IncAdjointCore(const float* lt0, VecField* bt, bool half) {
     const int64_t nl = m_Opt.domain.nl;
     const int nt = m_Opt.nt;
     const GatherDims g = gather_dims(m_Opt.domain);
@@ -633,6 +653,13 @@ void CLAIRE::IncAdjointCore(const float* lt0, VecField* bt) {
         launch_plan_adjoint_step(tp, g, m_AdjCoeff.get(), m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl,
                                  m_Opt.stream);
     }
+    if (half) {
+        // bytes: λ^0..λ^nt 4(nt+1) + ∇m (half) 6(nt+1) + b 12
+        KernelScope k(m_Opt.prof, "body_force", (10.0 * (nt + 1) + 12.0) * nl);
+        launch_body_force_half(nl, nt, m_Opt.GetTimeStepSize(), m_Opt.GetLebesgueMeasure(), m_Lam.get(),
+                               m_GradM16.get(), b, m_Opt.stream);
+        return;
+    }
     // bytes: λ^0..λ^nt 4(nt+1) + ∇m 12(nt+1) + b 12
     KernelScope k(m_Opt.prof, "body_force", (16.0 * (nt + 1) + 12.0) * nl);
     launch_body_force(nl, nt, m_Opt.GetTimeStepSize(), m_Opt.GetLebesgueMeasure(), m_Lam.get(), m_GradM.get(), b,
@@ -640,7 +667,7 @@ void CLAIRE::IncAdjointCore(const float* lt0, VecField* bt) {
 }
 
 // Incremental state: writes λ̃^0 = −m̃^nt into out (or m̃^nt when !negate_last).
-void CLAIRE::IncStateCore(const VecField* vt, float* out) {
+void CLAIRE::IncStateCore(const VecField* vt, float* out, bool half) {
     const int64_t nl = m_Opt.domain.nl;
     const int nt = m_Opt.nt;
     EnsureGradientsAtStateTrajectory();
@@ -651,7 +678,12 @@ void CLAIRE::IncStateCore(const VecField* vt, float* out) {
     float* vx[3] = {m_VtX.get(), m_VtX.get() + nl, m_VtX.get() + 2 * nl};
     // ṽ(X_s) (src/CLAIRE.cpp:3221)
     m_SL->Gather3(v, vx, "state");
-    {
+    if (half) {
+        // bytes: ṽX 12 + ṽ 12 + (∇m^j)(X_s) 6nt + ∇m^{j+1} 6nt (half) + s_j 4nt
+        KernelScope k(m_Opt.prof, "inc_state_rhs", (24.0 + 16.0 * nt) * nl);
+        launch_inc_state_rhs_half(nl, nt, m_Opt.GetTimeStepSize(), vx, v, m_GradMX16.get(), m_GradM16.get(),
+                                  m_S.get(), first, m_Opt.stream);
+    } else {
         // bytes: ṽX 12 + ṽ 12 + (∇m^j)(X_s) 12nt + ∇m^{j+1} 12nt + s_j 4nt
         KernelScope k(m_Opt.prof, "inc_state_rhs", (24.0 + 28.0 * nt) * nl);
         launch_inc_state_rhs(nl, nt, m_Opt.GetTimeStepSize(), vx, v, m_GradMX.get(), m_GradM.get(), m_S.get(), first,
@@ -671,7 +703,7 @@ int CLAIRE::SolveIncStateEquation() {
     CLR_REQUIRE(m_HaveIncVelocity, kErrState, "incremental velocity not set");
     if (!m_StateValid) SolveStateEquation();
     // λ̃⁰ = −m̃^nt lands in m_Lam; m̃^nt = −λ̃⁰ is kept for the API.
-    IncStateCore(m_IncVelocityField.get(), m_Lam.get());
+    IncStateCore(m_IncVelocityField.get(), m_Lam.get(), false);
     const int64_t nl = m_Opt.domain.nl;
     float* mt = m_IncState[0].get();
     launch_copy(m_Lam.get(), mt, nl, m_Opt.stream);
@@ -688,7 +720,7 @@ int CLAIRE::SolveIncAdjointFrom(const float* mtilde) {
     // λ̃⁰ = −m̃ (src/DistanceMeasureSL2.cpp:266-272)
     launch_copy(mtilde, m_Lam.get(), nl, m_Opt.stream);
     launch_scale(m_Lam.get(), -1.0f, nl, m_Opt.stream);
-    IncAdjointCore(m_Lam.get(), m_IncBodyForce.get());
+    IncAdjointCore(m_Lam.get(), m_IncBodyForce.get(), false);
     // λ̃ at t = 0: the last slot (constant in time when v = 0)
     launch_copy(m_VelocityIsZero ? m_Lam.get() : m_Lam.get() + m_Opt.nt * nl, m_IncAdjointT0.get(), nl, m_Opt.stream);
     return 0;
@@ -711,9 +743,11 @@ int CLAIRE::HessianMatVec(float* Hvtilde, const float* vtilde, bool scale) {
     CLR_REQUIRE(Hvtilde != vt->m_X1, kErrInvalidArgument, "HessianMatVec cannot run in place");
     VecField hv(&m_Opt, Hvtilde);
     // \tilde{m} (src/CLAIRE.cpp:1345) with λ̃⁰ = −m̃^nt fused into the last step
-    IncStateCore(vt, m_Lam.get());
+    const bool half = m_Opt.precision == 1;
+    if (half) EnsureHalfGradients();
+    IncStateCore(vt, m_Lam.get(), half);
     // \tilde{λ} and \tilde{b}·hd (src/CLAIRE.cpp:1348, 3307-3491)
-    IncAdjointCore(m_Lam.get(), m_IncBodyForce.get());
+    IncAdjointCore(m_Lam.get(), m_IncBodyForce.get(), half);
     // Hṽ = β·A[ṽ] + b̃ (src/CLAIRE.cpp:1356-1361); 1/hd when !scale (src/CLAIRE.cpp:1283-1286)
     const float a = scale ? 1.0f : 1.0f / m_Opt.GetLebesgueMeasure();
     m_Regularization->ApplyFused(&hv, vt, a, m_IncBodyForce.get(), a);
@@ -728,7 +762,7 @@ int CLAIRE::SolveAdjointEquation() {
     const int64_t nl = m_Opt.domain.nl;
     // λ¹ = mR − m^nt (src/DistanceMeasureSL2.cpp:198-205)
     launch_sub(m_ReferenceImage.get(), m_State.get() + m_Opt.nt * nl, m_Lam.get(), nl, m_Opt.stream);
-    IncAdjointCore(m_Lam.get(), m_BodyForce.get());
+    IncAdjointCore(m_Lam.get(), m_BodyForce.get(), false);
     launch_copy(m_VelocityIsZero ? m_Lam.get() : m_Lam.get() + m_Opt.nt * nl, m_Adjoint.get(), nl, m_Opt.stream);
     return 0;
 }

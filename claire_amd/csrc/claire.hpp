@@ -119,6 +119,11 @@ struct RegOpt {
     float beta[3] = {1e-2f, 1e-4f, 0.0f};
     hipStream_t stream = nullptr;
     Profiler* prof = nullptr;
+    // 0: plain FP32.  1: mixed form of the Hessian matvec -- the
+    // Newton-invariant gradient fields ∇m^j and (∇m^j)(X_s) read by the
+    // streaming kernels are stored as IEEE half (FP32 arithmetic); objective,
+    // gradient and reductions are unchanged.
+    int precision = 0;
     // include/RegOpt.hpp:485-500, src/RegOpt.cpp:2505-2512
     float GetLebesgueMeasure() const { return domain.hd; }
     float GetTimeStepSize() const { return 1.0f / static_cast<float>(nt); }
@@ -290,6 +295,8 @@ class CLAIRE {
     size_t DeviceBytes() const;
     // Newton-invariant data used by the Hessian (computed on demand).
     int PrepareHessian();
+    // 0 plain FP32, 1 mixed (RegOpt::precision).
+    int SetPrecision(int precision);
     void EnsureTrajectories();
     int SolveIncAdjointFrom(const float* mtilde);
 
@@ -297,8 +304,9 @@ class CLAIRE {
     void RequireVelocity() const;
     void EnsureStateGradients();
     void EnsureGradientsAtStateTrajectory();
-    void IncStateCore(const VecField* vt, float* lt0_out);
-    void IncAdjointCore(const float* lt0, VecField* bt);
+    void IncStateCore(const VecField* vt, float* lt0_out, bool half);
+    void IncAdjointCore(const float* lt0, VecField* bt, bool half);
+    void EnsureHalfGradients();
 
     RegOpt m_Opt;
     std::unique_ptr<Reducer> m_Red;
@@ -320,6 +328,9 @@ class CLAIRE {
     DeviceArray<float> m_Scalar;         // nl work
     DeviceArray<float> m_AdjCoeff;       // nl: adjoint update coefficient at every point
     DeviceArray<float> m_VtX;            // ṽ(X_s), 3·nl
+    DeviceArray<uint16_t> m_GradM16;     // half copies of m_GradM / m_GradMX (mixed form)
+    DeviceArray<uint16_t> m_GradMX16;
+    bool m_HalfValid = false;
 
     bool m_HaveVelocity = false, m_HaveIncVelocity = false, m_HaveTemplate = false, m_HaveReference = false;
     bool m_VelocityIsZero = false;

@@ -224,27 +224,39 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(D <= 
 //   s_j = ht/2·[(∇m^j)(X_s)·ṽ(X_s) + ∇m^{j+1}·ṽ],  j = 0..nt−1
 // (the two decrements of src/CLAIRE.cpp:3253-3273);
 // writes m̃¹ = −s_0 (λ̃⁰ = s_0 when nt = 1) and s_1..s_{nt−1}.
+// 4 consecutive values as float4: FP32 directly, FP16 converted.
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld4(const float* a, int64_t i) { return reinterpret_cast<const float4*>(a)[i]; }
+__device__ __forceinline__ float4 ld4(const _Float16* a, int64_t i) {
+    const h4 h = reinterpret_cast<const h4*>(a)[i];
+    return make_float4(static_cast<float>(h.x), static_cast<float>(h.y), static_cast<float>(h.z),
+                       static_cast<float>(h.w));
+}
+
+template <class G>
 struct RhsP {
     const float* vx[3];
     const float* vt[3];
-    const float* gX;
-    const float* gm;
+    const G* gX;
+    const G* gm;
     float* s;
     float* mt1;
 };
 
-__global__ __launch_bounds__(256) void k_inc_state_rhs(int64_t nl4, int nt, float hthalf, RhsP p) {
+// G = float (plain) or _Float16 (mixed form: the Newton-invariant gradient
+// fields stored in half precision, accumulated in FP32).
+template <class G>
+__global__ __launch_bounds__(256) void k_inc_state_rhs(int64_t nl4, int nt, float hthalf, RhsP<G> p) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= nl4) return;
     const int64_t nl = nl4 * 4;
-    auto ld = [&](const float* a) { return reinterpret_cast<const float4*>(a)[i]; };
-    const float4 x0 = ld(p.vx[0]), x1 = ld(p.vx[1]), x2 = ld(p.vx[2]);
-    const float4 v0 = ld(p.vt[0]), v1 = ld(p.vt[1]), v2 = ld(p.vt[2]);
+    const float4 x0 = ld4(p.vx[0], i), x1 = ld4(p.vx[1], i), x2 = ld4(p.vx[2], i);
+    const float4 v0 = ld4(p.vt[0], i), v1 = ld4(p.vt[1], i), v2 = ld4(p.vt[2], i);
     for (int j = 0; j < nt; ++j) {
-        const float* gx = p.gX + static_cast<int64_t>(3 * j) * nl;
-        const float* gn = p.gm + static_cast<int64_t>(3 * (j + 1)) * nl;
-        const float4 a0 = ld(gx), a1 = ld(gx + nl), a2 = ld(gx + 2 * nl);
-        const float4 c0 = ld(gn), c1 = ld(gn + nl), c2 = ld(gn + 2 * nl);
+        const G* gx = p.gX + static_cast<int64_t>(3 * j) * nl;
+        const G* gn = p.gm + static_cast<int64_t>(3 * (j + 1)) * nl;
+        const float4 a0 = ld4(gx, i), a1 = ld4(gx + nl, i), a2 = ld4(gx + 2 * nl, i);
+        const float4 c0 = ld4(gn, i), c1 = ld4(gn + nl, i), c2 = ld4(gn + 2 * nl, i);
         float4 s;
         s.x = hthalf * (a0.x * x0.x + a1.x * x1.x + a2.x * x2.x) + hthalf * (c0.x * v0.x + c1.x * v1.x + c2.x * v2.x);
         s.y = hthalf * (a0.y * x0.y + a1.y * x1.y + a2.y * x2.y) + hthalf * (c0.y * v0.y + c1.y * v1.y + c2.y * v2.y);
@@ -263,25 +275,26 @@ __global__ __launch_bounds__(256) void k_inc_state_rhs(int64_t nl4, int nt, floa
 // w_0 = w_nt = ht/2, w_j = ht, accumulated in the reference's order
 // (src/CLAIRE.cpp:2668-2709 and 3940-3999, scaled by hd at :2341/:3471).
 // Streamed with 16-byte accesses; lam holds λ^0..λ^nt.
+template <class G>
 struct BodyP {
     const float* lam;
-    const float* gm;
+    const G* gm;
     float* b[3];
 };
 
-__global__ __launch_bounds__(256) void k_body_force(int64_t nl4, int nt, float ht, float hd, BodyP p) {
+template <class G>
+__global__ __launch_bounds__(256) void k_body_force(int64_t nl4, int nt, float ht, float hd, BodyP<G> p) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= nl4) return;
     const int64_t nl = nl4 * 4;
-    auto ld = [&](const float* a) { return reinterpret_cast<const float4*>(a)[i]; };
     float4 s[3] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
     for (int j = 0; j <= nt; ++j) {
         const float w = (j == 0 || j == nt) ? 0.5f * ht : ht;
-        const float4 l = ld(p.lam + static_cast<int64_t>(j) * nl);
-        const float* gj = p.gm + static_cast<int64_t>(3 * (nt - j)) * nl;
+        const float4 l = ld4(p.lam + static_cast<int64_t>(j) * nl, i);
+        const G* gj = p.gm + static_cast<int64_t>(3 * (nt - j)) * nl;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const float4 gd = ld(gj + d * nl);
+            const float4 gd = ld4(gj + d * nl, i);
             s[d].x += w * gd.x * l.x;
             s[d].y += w * gd.y * l.y;
             s[d].z += w * gd.z * l.z;
@@ -291,6 +304,20 @@ __global__ __launch_bounds__(256) void k_body_force(int64_t nl4, int nt, float h
 #pragma unroll
     for (int d = 0; d < 3; ++d)
         reinterpret_cast<float4*>(p.b[d])[i] = make_float4(s[d].x * hd, s[d].y * hd, s[d].z * hd, s[d].w * hd);
+}
+
+// FP32 → FP16 (round to nearest even), 4 values per thread.
+__global__ __launch_bounds__(256) void k_to_half(const float* __restrict__ in, _Float16* __restrict__ out,
+                                                 int64_t n4) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n4) return;
+    const float4 a = reinterpret_cast<const float4*>(in)[i];
+    h4 h;
+    h.x = static_cast<_Float16>(a.x);
+    h.y = static_cast<_Float16>(a.y);
+    h.z = static_cast<_Float16>(a.z);
+    h.w = static_cast<_Float16>(a.w);
+    reinterpret_cast<h4*>(out)[i] = h;
 }
 
 template <int D, class Epi>
@@ -347,23 +374,53 @@ void launch_plan_adjoint_step(const TrajectoryPlan& tp, const GatherDims& g, con
     dispatch(tp.dbound, g, tp.P, lam, 0, 1, EpiAdjStep{C, out}, st);
 }
 
-void launch_body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const float* gm, float* const b[3],
-                       hipStream_t st) {
+template <class G>
+void body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const G* gm, float* const b[3],
+                hipStream_t st) {
     CLR_REQUIRE(nl % 4 == 0, kErrInvalidArgument, "grid size must be a multiple of 4 points");
     const int64_t nl4 = nl / 4;
-    BodyP p{lam, gm, {b[0], b[1], b[2]}};
+    BodyP<G> p{lam, gm, {b[0], b[1], b[2]}};
     const int nb = static_cast<int>((nl4 + 255) / 256);
-    hipLaunchKernelGGL(k_body_force, dim3(nb), dim3(256), 0, st, nl4, nt, ht, hd, p);
+    hipLaunchKernelGGL((k_body_force<G>), dim3(nb), dim3(256), 0, st, nl4, nt, ht, hd, p);
+}
+
+template <class G>
+void inc_state_rhs(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3], const G* gX,
+                   const G* gm, float* s, float* mt1, hipStream_t st) {
+    CLR_REQUIRE(nl % 4 == 0, kErrInvalidArgument, "grid size must be a multiple of 4 points");
+    CLR_REQUIRE(nt >= 1, kErrInvalidArgument, "nt must be >= 1");
+    const int64_t nl4 = nl / 4;
+    RhsP<G> p{{vx[0], vx[1], vx[2]}, {vt[0], vt[1], vt[2]}, gX, gm, s, mt1};
+    const int nb = static_cast<int>((nl4 + 255) / 256);
+    hipLaunchKernelGGL((k_inc_state_rhs<G>), dim3(nb), dim3(256), 0, st, nl4, nt, 0.5f * ht, p);
+}
+
+void launch_body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const float* gm, float* const b[3],
+                       hipStream_t st) {
+    body_force(nl, nt, ht, hd, lam, gm, b, st);
+}
+
+void launch_body_force_half(int64_t nl, int nt, float ht, float hd, const float* lam, const uint16_t* gm,
+                            float* const b[3], hipStream_t st) {
+    body_force(nl, nt, ht, hd, lam, reinterpret_cast<const _Float16*>(gm), b, st);
 }
 
 void launch_inc_state_rhs(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
                           const float* gX, const float* gm, float* s, float* mt1, hipStream_t st) {
-    CLR_REQUIRE(nl % 4 == 0, kErrInvalidArgument, "grid size must be a multiple of 4 points");
-    CLR_REQUIRE(nt >= 1, kErrInvalidArgument, "nt must be >= 1");
-    const int64_t nl4 = nl / 4;
-    RhsP p{{vx[0], vx[1], vx[2]}, {vt[0], vt[1], vt[2]}, gX, gm, s, mt1};
-    const int nb = static_cast<int>((nl4 + 255) / 256);
-    hipLaunchKernelGGL(k_inc_state_rhs, dim3(nb), dim3(256), 0, st, nl4, nt, 0.5f * ht, p);
+    inc_state_rhs(nl, nt, ht, vx, vt, gX, gm, s, mt1, st);
+}
+
+void launch_inc_state_rhs_half(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
+                               const uint16_t* gX, const uint16_t* gm, float* s, float* mt1, hipStream_t st) {
+    inc_state_rhs(nl, nt, ht, vx, vt, reinterpret_cast<const _Float16*>(gX), reinterpret_cast<const _Float16*>(gm),
+                  s, mt1, st);
+}
+
+void launch_to_half(const float* in, uint16_t* out, int64_t n, hipStream_t st) {
+    CLR_REQUIRE(n % 4 == 0, kErrInvalidArgument, "length must be a multiple of 4");
+    const int64_t n4 = n / 4;
+    const int nb = static_cast<int>((n4 + 255) / 256);
+    hipLaunchKernelGGL(k_to_half, dim3(nb), dim3(256), 0, st, in, reinterpret_cast<_Float16*>(out), n4);
 }
 
 }  // namespace claire_amd

@@ -111,6 +111,10 @@ void launch_direct_adjoint_step(const GatherDims& g, const float* X, const float
 // λ^0..λ^nt, gm holds ∇m^0..∇m^nt (3 fields each).
 void launch_body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const float* gm, float* const b[3],
                        hipStream_t st);
+// Mixed form: the same with the gradient fields stored as IEEE half
+// (uint16_t bit patterns), accumulated in FP32.
+void launch_body_force_half(int64_t nl, int nt, float ht, float hd, const float* lam, const uint16_t* gm,
+                            float* const b[3], hipStream_t st);
 // v = 0 body force: b_d = hd·λ·∂_d m
 void launch_body_force_static(int64_t nl, float hd, const float* lam, const float* gm, float* const b[3],
                               hipStream_t st);
@@ -118,6 +122,11 @@ void launch_body_force_static(int64_t nl, float hd, const float* lam, const floa
 // ṽX = ṽ(X_s); writes m̃¹ = −s_0 (λ̃⁰ = s_0 when nt = 1) and s_1..s_{nt−1}.
 void launch_inc_state_rhs(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
                           const float* gX, const float* gm, float* s, float* mt1, hipStream_t st);
+
+void launch_inc_state_rhs_half(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
+                               const uint16_t* gX, const uint16_t* gm, float* s, float* mt1, hipStream_t st);
+// FP32 → IEEE half, round to nearest even (n a multiple of 4).
+void launch_to_half(const float* in, uint16_t* out, int64_t n, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Element-wise kernels and reductions (pointwise.hip)

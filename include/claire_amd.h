@@ -65,6 +65,15 @@ int clr_device_synchronize(void);
 int clr_create(clr_ctx** ctx, const int nx[3], int nt, int regnorm, const double beta[3], void* stream);
 int clr_destroy(clr_ctx* ctx);
 int clr_synchronize(clr_ctx* ctx);
+/* Arithmetic form of the Hessian matvec.  CLR_PRECISION_FP32 (default) is
+ * the reference's single-precision path.  CLR_PRECISION_MIXED stores the
+ * Newton-invariant gradient fields ∇m^j and (∇m^j)(X_s) that the streaming
+ * kernels read as IEEE half (FP32 arithmetic, FP64 reductions unchanged);
+ * it has no counterpart in the reference, whose precision is fixed by the
+ * PETSc build (ScalarType = PetscReal, include/CLAIREUtils.hpp:58-67). */
+#define CLR_PRECISION_FP32 0
+#define CLR_PRECISION_MIXED 1
+int clr_set_precision(clr_ctx* ctx, int precision);
 /* Device memory held by the context (bytes). */
 int clr_device_bytes(clr_ctx* ctx, size_t* bytes);
 

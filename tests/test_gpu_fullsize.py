@@ -104,3 +104,28 @@ def test_hessian_accuracy_vs_fp64_128(norm, oracle):
     e_gpu, e_cpu = relerr(H, H64), relerr(H32, H64)
     print(f"relerr vs FP64: gpu {e_gpu:.3e}  reference FP32 path {e_cpu:.3e}  gpu vs ref {relerr(H, H32):.3e}")
     assert e_gpu <= 2.0 * e_cpu + 1e-6
+
+
+@pytest.mark.parametrize("norm", ["h1sn", "h2"])
+def test_mixed_hessian_accuracy_vs_fp64_128(norm, oracle):
+    """Mixed form (FP16 storage of the Newton-invariant gradient fields)
+    scored against the FP64 oracle at 128³, next to the plain FP32 path."""
+    shape = (128, 128, 128)
+    print(f"\n[gpu case] mixed accuracy 128^3 {norm}", flush=True)
+    beta = (1e-3, 1e-4, 0.0) if norm == "h2" else (1e-2, 1e-4, 0.0)
+    v = synth.bandlimited_velocity(shape, kmax=8, cfl=2.0, nt=NT, seed=0)
+    mT = synth.gaussian_blobs(shape, count=24, seed=1)
+    vt = synth.bandlimited_velocity(shape, kmax=8, cfl=1.0, nt=NT, seed=2)
+    c = Claire(shape, nt=NT, regnorm=norm, beta=beta)
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        H = c.hessian_matvec(vt)
+        c.set_precision("mixed")
+        Hm = c.hessian_matvec(vt)
+    finally:
+        c.close()
+    H64 = oracle.hessian_matvec(v, oracle.solve_state(v, mT, NT), vt, NT, norm, beta)
+    e_fp32, e_mixed = relerr(H, H64), relerr(Hm, H64)
+    print(f"relerr vs FP64: gpu fp32 {e_fp32:.3e}  gpu mixed {e_mixed:.3e}")
+    assert e_mixed <= max(1e-4, 2.0 * e_fp32)
