@@ -238,6 +238,23 @@ class Claire:
         _check(lib().clr_evaluate_gradient(self.ctx, vp, _vp(g.ptr)))
         return g.download()
 
+    def det_def_grad(self, inverse=False):
+        """det(∇y) (or det(∇y⁻¹)) of the current velocity; returns
+        (J, (min, max, mean))."""
+        J = self.buf("jac", self.nx)
+        st = (ctypes.c_double * 3)()
+        _check(lib().clr_det_def_grad(self.ctx, ctypes.c_int(int(inverse)), _vp(J.ptr), st))
+        return J.download(), tuple(st)
+
+    def check_bounds(self, v, inverse=False, bound=0.25):
+        """CLAIREBase::CheckBounds: (bound reached, (min, max, mean))."""
+        d = self._in("v", v, (3,) + self.nx)
+        reached = ctypes.c_int()
+        st = (ctypes.c_double * 3)()
+        _check(lib().clr_check_bounds(self.ctx, _vp(d.ptr), ctypes.c_int(int(inverse)), ctypes.c_double(bound),
+                                      ctypes.byref(reached), st))
+        return bool(reached.value), tuple(st)
+
     def set_precision(self, precision):
         """"fp32" (the reference's path) or "mixed" (FP16 storage of the
         Newton-invariant gradient fields in the Hessian matvec)."""

@@ -1,4 +1,5 @@
 // capi.cpp -- the extern "C" boundary (include/claire_amd.h).
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -233,6 +234,27 @@ int clr_evaluate_gradient(clr_ctx* ctx, const float* v, float* g) {
     return guard([&] {
         need(g, "g");
         C(ctx).EvaluateGradient(g, v);
+        check_hip_launch();
+    });
+}
+
+int clr_det_def_grad(clr_ctx* ctx, int inverse, float* jac, double* stats) {
+    return guard([&] {
+        need(stats, "stats");
+        C(ctx).ComputeDetDefGrad(inverse != 0, jac, stats);
+        check_hip_launch();
+    });
+}
+
+int clr_check_bounds(clr_ctx* ctx, const float* v, int inverse, double bound, int* reached, double* stats) {
+    return guard([&] {
+        need(v, "v");
+        need(reached, "reached");
+        double st[3];
+        C(ctx).SetControlVariable(v);
+        C(ctx).ComputeDetDefGrad(inverse != 0, nullptr, st);
+        *reached = (st[0] <= bound || 1.0 / st[1] <= bound) ? 1 : 0;
+        if (stats) std::copy(st, st + 3, stats);
         check_hip_launch();
     });
 }

@@ -109,6 +109,30 @@ double Reducer::AbsMax(const float* a, IntType n) {
     return fetch();
 }
 
+double Reducer::Min(const float* a, IntType n) {
+    {
+        KernelScope k(m_Opt->prof, "reduce_min", 4.0 * n);
+        launch_min(a, n, m_Partial.get(), m_Result.get(), m_Opt->stream);
+    }
+    return fetch();
+}
+
+double Reducer::Max(const float* a, IntType n) {
+    {
+        KernelScope k(m_Opt->prof, "reduce_max", 4.0 * n);
+        launch_max(a, n, m_Partial.get(), m_Result.get(), m_Opt->stream);
+    }
+    return fetch();
+}
+
+double Reducer::Sum(const float* a, IntType n) {
+    {
+        KernelScope k(m_Opt->prof, "reduce_sum", 4.0 * n);
+        launch_sum(a, n, m_Partial.get(), m_Result.get(), m_Opt->stream);
+    }
+    return fetch();
+}
+
 double Reducer::SqDist(const float* a, const float* b, IntType n) {
     {
         KernelScope k(m_Opt->prof, "reduce_sqdist", 8.0 * n);
@@ -555,7 +579,7 @@ void CLAIRE::EnsureTrajectories() {
         // bytes: X 12 + div v gathered 4 + div v 4 + C 4 per point
         KernelScope k(m_Opt.prof, "gather_adjoint_coeff", 24.0 * m_Opt.domain.nl);
         launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Plan("adjoint").X, m_Scalar.get(),
-                             m_Opt.GetTimeStepSize(), m_AdjCoeff.get(), m_Opt.stream);
+                             m_Opt.GetTimeStepSize(), 1.0f, m_AdjCoeff.get(), m_Opt.stream);
     }
     m_TrajValid = true;
 }
@@ -617,6 +641,40 @@ void CLAIRE::EnsureHalfGradients() {
     launch_to_half(m_GradM.get(), m_GradM16.get(), (nt + 1) * 3 * nl, m_Opt.stream);
     launch_to_half(m_GradMX.get(), m_GradMX16.get(), nt * 3 * nl, m_Opt.stream);
     m_HalfValid = true;
+}
+
+int CLAIRE::ComputeDetDefGrad(bool inverse, float* jac, double stats[3]) {
+    // DeformationFields::ComputeDetDefGrad / ComputeDetDefGradSL
+    // (src/DeformationFields.cpp:315-388, 636-740): J⁰ = 1, then nt SL steps
+    // along the state trajectory J^{j+1} = C·J^j(X_s) with
+    // C = 1 + ht/2·(α·dX + α·(1 + ht·α·dX)·d), d = ∇·v, dX = d(X_s), α = ±1.
+    RequireVelocity();
+    EnsureTrajectories();
+    const int64_t nl = m_Opt.domain.nl;
+    const int nt = m_Opt.nt;
+    const GatherDims g = gather_dims(m_Opt.domain);
+    DeviceArray<float> divv(static_cast<size_t>(nl)), coeff(static_cast<size_t>(nl));
+    const float* vc[3] = {m_VelocityField->m_X1, m_VelocityField->m_X2, m_VelocityField->m_X3};
+    m_Spectral->Div(divv.get(), vc);
+    const TrajectoryPlan tp = m_SL->Plan("state");
+    {
+        KernelScope k(m_Opt.prof, "gather_det_coeff", 24.0 * nl);
+        launch_adjoint_coeff(g, tp.X, divv.get(), m_Opt.GetTimeStepSize(), inverse ? -1.0f : 1.0f, coeff.get(),
+                             m_Opt.stream);
+    }
+    float* buf[2] = {m_IncState[0].get(), m_IncState[1].get()};
+    launch_fill(buf[0], 1.0f, nl, m_Opt.stream);
+    for (int j = 0; j < nt; ++j) {
+        KernelScope k(m_Opt.prof, "det_step", 24.0 * nl);
+        launch_plan_adjoint_step(tp, g, coeff.get(), buf[j % 2], buf[(j + 1) % 2], m_Opt.stream);
+    }
+    const float* J = buf[nt % 2];
+    stats[0] = m_Red->Min(J, nl);
+    stats[1] = m_Red->Max(J, nl);
+    stats[2] = m_Red->Sum(J, nl) / static_cast<double>(nl);
+    if (jac) launch_copy(J, jac, nl, m_Opt.stream);
+    CLR_HIP_CHECK(hipStreamSynchronize(m_Opt.stream));
+    return 0;
 }
 
 int CLAIRE::SetPrecision(int precision) {

@@ -137,6 +137,9 @@ class Reducer {
     double Dot(const float* a, const float* b, IntType n);
     double AbsMax(const float* a, IntType n);
     double SqDist(const float* a, const float* b, IntType n);
+    double Min(const float* a, IntType n);
+    double Max(const float* a, IntType n);
+    double Sum(const float* a, IntType n);
 
   private:
     double fetch();
@@ -297,6 +300,10 @@ class CLAIRE {
     int PrepareHessian();
     // 0 plain FP32, 1 mixed (RegOpt::precision).
     int SetPrecision(int precision);
+    // det(∇y) (inverse = false) or det(∇y⁻¹) by SL transport
+    // (DeformationFields::ComputeDetDefGrad); jac may be null; stats receive
+    // min, max and mean.
+    int ComputeDetDefGrad(bool inverse, float* jac, double stats[3]);
     void EnsureTrajectories();
     int SolveIncAdjointFrom(const float* mtilde);
 

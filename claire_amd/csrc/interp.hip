@@ -142,11 +142,13 @@ __global__ __launch_bounds__(kThreads) void k_make_plan(GatherDims g, HH hh, con
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 
-// Adjoint update coefficient c = 1 + ht/2·(dX + (1 + ht·dX)·d), d = ∇·v,
-// dX = d(X_a).  The SL adjoint step λ⁺ = λX + ht/2·(λX·dX + (λX + ht·λX·dX)·d)
-// (src/CLAIRE.cpp:2668-2673, 3955-3962) is then λ⁺ = c·λX.
+// Update coefficient c = 1 + ht/2·(α·dX + α·(1 + ht·α·dX)·d), d = ∇·v,
+// dX = d(X).  With α = 1 on the adjoint trajectory, the SL adjoint step
+// λ⁺ = λX + ht/2·(λX·dX + (λX + ht·λX·dX)·d) (src/CLAIRE.cpp:2668-2673,
+// 3955-3962) is λ⁺ = c·λX; with α = ±1 on the state trajectory, the SL
+// step of det(∇y^{±1}) (src/DeformationFields.cpp:714-721) is J⁺ = c·J(X).
 __global__ __launch_bounds__(kThreads) void k_adjoint_coeff(GatherDims g, HH hh, const float* __restrict__ X,
-                                                            const float* __restrict__ divv, float ht,
+                                                            const float* __restrict__ divv, float ht, float alpha,
                                                             float* __restrict__ C, int nblocks) {
     const int64_t i = point_index(nblocks);
     if (i >= g.nl) return;
@@ -159,7 +161,8 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_coeff(GatherDims g, HH hh,
     eval_stencil<1>(fs, S, g, dx);
     const float d = divv[i];
     const float hthalf = 0.5f * ht;
-    C[i] = 1.0f + hthalf * (dx[0] + (1.0f + ht * dx[0]) * d);
+    const float adx = alpha * dx[0];
+    C[i] = 1.0f + hthalf * (adx + alpha * (1.0f + ht * adx) * d);
 }
 
 // Gather along a trajectory straight from the raw foot points (any
@@ -253,11 +256,11 @@ int launch_make_plan(const GatherDims& g, const float* X, uint32_t* P, int* scra
     return h;
 }
 
-void launch_adjoint_coeff(const GatherDims& g, const float* X, const float* divv, float ht, float* C,
+void launch_adjoint_coeff(const GatherDims& g, const float* X, const float* divv, float ht, float alpha, float* C,
                           hipStream_t st) {
     check_gather_dims(g);
     const int nb = nblocks_for(g.nl);
-    hipLaunchKernelGGL(k_adjoint_coeff, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), X, divv, ht, C, nb);
+    hipLaunchKernelGGL(k_adjoint_coeff, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), X, divv, ht, alpha, C, nb);
 }
 
 void launch_direct_gather(const GatherDims& g, const float* X, const float* const f[], int nf, float* const out[],

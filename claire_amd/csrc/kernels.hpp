@@ -74,9 +74,11 @@ void launch_trajectory_rk2(const Dims& d, const float* const v[3], float sigma, 
 int launch_make_plan(const GatherDims& g, const float* X, uint32_t* P, int* scratch, hipStream_t st);
 // The LDS-staged gathers need D <= lds_gather_max_displacement().
 int lds_gather_max_displacement();
-// Adjoint update coefficient per point: C = 1 + ht/2·(dX + (1 + ht·dX)·d)
-// with d = ∇·v, dX = d(X_a); the SL adjoint update is then λ⁺ = C·λ(X_a).
-void launch_adjoint_coeff(const GatherDims& g, const float* X, const float* divv, float ht, float* C,
+// Update coefficient per point: C = 1 + ht/2·(α·dX + α·(1 + ht·α·dX)·d)
+// with d = ∇·v, dX = d(X).  α = 1 on the adjoint trajectory gives the SL
+// adjoint update λ⁺ = C·λ(X_a); α = ±1 on the state trajectory gives the
+// SL update of det(∇y^{±1}), J⁺ = C·J(X_s).
+void launch_adjoint_coeff(const GatherDims& g, const float* X, const float* divv, float ht, float alpha, float* C,
                           hipStream_t st);
 
 // A trajectory as the gathers see it: the packed plan, the raw foot points
@@ -147,5 +149,8 @@ constexpr int kReduceBlocks = 1024;
 void launch_dot(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st);
 void launch_absmax(const float* a, int64_t n, double* partial, double* result, hipStream_t st);
 void launch_sqdist(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st);
+void launch_min(const float* a, int64_t n, double* partial, double* result, hipStream_t st);
+void launch_max(const float* a, int64_t n, double* partial, double* result, hipStream_t st);
+void launch_sum(const float* a, int64_t n, double* partial, double* result, hipStream_t st);
 
 }  // namespace claire_amd

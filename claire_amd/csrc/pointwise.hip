@@ -46,45 +46,52 @@ __global__ __launch_bounds__(kThreads) void k_sub(const float* __restrict__ x, c
     if (i < n) z[i] = x[i] - y[i];
 }
 
-__device__ __forceinline__ double block_sum(double v, double* sh) {
+enum RedOp { kRedDot = 0, kRedAbsMax = 1, kRedSqDist = 2, kRedMin = 3, kRedMax = 4, kRedSum = 5 };
+
+template <int OP>
+__device__ __forceinline__ double red_init() {
+    if constexpr (OP == kRedMin) return INFINITY;
+    if constexpr (OP == kRedMax) return -INFINITY;
+    return 0.0;
+}
+
+template <int OP>
+__device__ __forceinline__ double red_combine(double x, double y) {
+    if constexpr (OP == kRedMin) return fmin(x, y);
+    if constexpr (OP == kRedMax || OP == kRedAbsMax) return fmax(x, y);
+    return x + y;
+}
+
+template <int OP>
+__device__ double block_reduce(double v, double* sh) {
     sh[threadIdx.x] = v;
     __syncthreads();
     for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        if (threadIdx.x < s) sh[threadIdx.x] = red_combine<OP>(sh[threadIdx.x], sh[threadIdx.x + s]);
         __syncthreads();
     }
     return sh[0];
 }
-
-__device__ __forceinline__ double block_max(double v, double* sh) {
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-        __syncthreads();
-    }
-    return sh[0];
-}
-
-enum RedOp { kRedDot = 0, kRedAbsMax = 1, kRedSqDist = 2 };
 
 template <int OP>
 __global__ __launch_bounds__(kThreads) void k_reduce_partial(const float* __restrict__ a, const float* __restrict__ b,
                                                              int64_t n, double* __restrict__ partial) {
     __shared__ double sh[kThreads];
-    double acc = 0.0;
+    double acc = red_init<OP>();
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
         if constexpr (OP == kRedDot) {
             acc += static_cast<double>(a[i]) * static_cast<double>(b[i]);
         } else if constexpr (OP == kRedAbsMax) {
             acc = fmax(acc, fabs(static_cast<double>(a[i])));
-        } else {
+        } else if constexpr (OP == kRedSqDist) {
             const double d = static_cast<double>(a[i] - b[i]);
             acc += d * d;
+        } else {
+            acc = red_combine<OP>(acc, static_cast<double>(a[i]));
         }
     }
-    const double r = (OP == kRedAbsMax) ? block_max(acc, sh) : block_sum(acc, sh);
+    const double r = block_reduce<OP>(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
@@ -92,9 +99,9 @@ template <int OP>
 __global__ __launch_bounds__(kThreads) void k_reduce_final(const double* __restrict__ partial, int np,
                                                            double* __restrict__ result) {
     __shared__ double sh[kThreads];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < np; i += kThreads) acc = (OP == kRedAbsMax) ? fmax(acc, partial[i]) : acc + partial[i];
-    const double r = (OP == kRedAbsMax) ? block_max(acc, sh) : block_sum(acc, sh);
+    double acc = red_init<OP>();
+    for (int i = threadIdx.x; i < np; i += kThreads) acc = red_combine<OP>(acc, partial[i]);
+    const double r = block_reduce<OP>(acc, sh);
     if (threadIdx.x == 0) result[0] = r;
 }
 
@@ -132,6 +139,15 @@ void launch_absmax(const float* a, int64_t n, double* partial, double* result, h
 }
 void launch_sqdist(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st) {
     reduce<kRedSqDist>(a, b, n, partial, result, st);
+}
+void launch_min(const float* a, int64_t n, double* partial, double* result, hipStream_t st) {
+    reduce<kRedMin>(a, nullptr, n, partial, result, st);
+}
+void launch_max(const float* a, int64_t n, double* partial, double* result, hipStream_t st) {
+    reduce<kRedMax>(a, nullptr, n, partial, result, st);
+}
+void launch_sum(const float* a, int64_t n, double* partial, double* result, hipStream_t st) {
+    reduce<kRedSum>(a, nullptr, n, partial, result, st);
 }
 
 }  // namespace claire_amd

@@ -65,6 +65,17 @@ int clr_device_synchronize(void);
 int clr_create(clr_ctx** ctx, const int nx[3], int nt, int regnorm, const double beta[3], void* stream);
 int clr_destroy(clr_ctx* ctx);
 int clr_synchronize(clr_ctx* ctx);
+/* Determinant of the deformation gradient of the current velocity by SL
+ * transport: DeformationFields::ComputeDetDefGrad + ComputeDetDefGradSL
+ * (src/DeformationFields.cpp:315-388, 636-740); inverse != 0 gives
+ * det(∇y⁻¹) (RegFlags.invdefgrad).  jac (device, nl) may be NULL; stats
+ * (host, 3 doubles) receive min, max and mean (Monitor.detdgradmin/max/mean). */
+int clr_det_def_grad(clr_ctx* ctx, int inverse, float* jac, double* stats);
+/* CLAIREBase::CheckBounds (src/CLAIREBase.cpp:1386-1446): sets the velocity
+ * v (device, 3·nl), computes det(∇y^{±1}) and reports whether
+ * min(J) <= bound or 1/max(J) <= bound (default bound 0.25, src/RegOpt.cpp:1254). */
+int clr_check_bounds(clr_ctx* ctx, const float* v, int inverse, double bound, int* reached, double* stats);
+
 /* Arithmetic form of the Hessian matvec.  CLR_PRECISION_FP32 (default) is
  * the reference's single-precision path.  CLR_PRECISION_MIXED stores the
  * Newton-invariant gradient fields ∇m^j and (∇m^j)(X_s) that the streaming

@@ -804,6 +804,45 @@ void objective(const Grid<T>& g, int nt, int norm, const T* beta, const T* const
     *J = *D + *R;
 }
 
+// Determinant of the deformation gradient by semi-Lagrangian transport,
+// DeformationFields::ComputeDetDefGrad + ComputeDetDefGradSL
+// (src/DeformationFields.cpp:315-388, 636-740): J⁰ = 1 and for j = 0..nt−1
+//   JX = J^j(X_s), rhs0 = α·JX·(∇·v)(X_s), rhs1 = α·(JX + ht·rhs0)·∇·v,
+//   J^{j+1} = JX + ht/2·(rhs0 + rhs1),   α = −1 for det(∇y⁻¹) (inverse).
+// Also min, max and mean of J (VecMin/VecMax/VecSum/ng).
+template <class T>
+void det_def_grad(const Grid<T>& g, int nt, const T* const v[3], bool inverse, T* jac, double* jmin, double* jmax,
+                  double* jmean) {
+    const int64_t nl = g.nl;
+    const T ht = T(1) / static_cast<T>(nt);
+    const T hthalf = static_cast<T>(0.5) * ht;
+    const T alpha = inverse ? T(-1) : T(1);
+    std::vector<T> X(3 * nl), divv(nl), divvx(nl), jx(nl);
+    std::fill(jac, jac + nl, T(1));
+    trajectory_rk2(g, v, ht, +1, X.data());
+    div(g, v[0], v[1], v[2], divv.data());
+    interp(g, divv.data(), X.data(), divvx.data());
+    for (int j = 0; j < nt; ++j) {
+        interp(g, jac, X.data(), jx.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            const T jacX = jx[i];
+            const T rhs0 = alpha * jacX * divvx[i];
+            const T rhs1 = alpha * (jacX + ht * rhs0) * divv[i];
+            jac[i] = jacX + hthalf * (rhs0 + rhs1);
+        }
+    }
+    double mn = jac[0], mx = jac[0], sum = 0.0;
+    for (int64_t i = 0; i < nl; ++i) {
+        mn = std::min(mn, static_cast<double>(jac[i]));
+        mx = std::max(mx, static_cast<double>(jac[i]));
+        sum += jac[i];
+    }
+    *jmin = mn;
+    *jmax = mx;
+    *jmean = sum / static_cast<double>(nl);
+}
+
 // ---------------------------------------------------------------------------
 // Gauss–Newton–Krylov solve, the reference's default optimizer:
 //   TAO "nls" with a matrix-free Hessian (src/Optimizer.cpp:235-444),

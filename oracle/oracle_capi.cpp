@@ -221,6 +221,14 @@ void oracle_set_num_threads(int n) {
             oracle::objective(g, nt, norm, b, v, mT, mR, J, D, R);                                                  \
         });                                                                                                        \
     }                                                                                                              \
+    int oracle_det_def_grad_##SFX(const int* nx, int nt, const T* v1, const T* v2, const T* v3, int inverse,      \
+                                  T* jac, double* stats) {                                                         \
+        return guard([&] {                                                                                         \
+            oracle::Grid<T> g(nx);                                                                                  \
+            const T* v[3] = {v1, v2, v3};                                                                           \
+            oracle::det_def_grad(g, nt, v, inverse != 0, jac, stats, stats + 1, stats + 2);                         \
+        });                                                                                                        \
+    }                                                                                                              \
     /* iopts: maxit, krylov_maxit, forcing, precond, linesearch; dopts: gttol, gatol, krylov_rtol;              */ \
     /* irep: iterations, reason, matvecs, pcg_iterations, objective_evals, gradient_evals;                      */ \
     /* drep: J0, D0, gnorm0, J, D, R, gnorm; v is the initial guess on entry and the solution on exit.          */ \

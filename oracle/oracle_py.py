@@ -260,3 +260,14 @@ def gn_solve(v0, mT, mR, nt, regnorm, beta, maxit=50, gttol=5e-2, gatol=1e-6, kr
     rep.update(zip(["J0", "D0", "gnorm0", "J", "D", "R", "gnorm"], list(drep)))
     rep["reason"] = GN_REASONS.get(rep["reason"], rep["reason"])
     return v, rep, hist[: min(rep["iterations"] + 1, hist_len)]
+
+
+def det_def_grad(v, nt, inverse=False, prec="f64"):
+    """det(∇y) (or det(∇y⁻¹)) by SL transport; returns (J, (min, max, mean))."""
+    v = _c(v, prec)
+    n = v.shape[1:]
+    jac = np.empty(n, dtype=_dt(prec))
+    stats = (ctypes.c_double * 3)()
+    _call("oracle_det_def_grad", prec, _nx(n), ctypes.c_int(nt), _p(v[0]), _p(v[1]), _p(v[2]),
+          ctypes.c_int(int(inverse)), _p(jac), stats)
+    return jac, tuple(stats)

@@ -245,3 +245,26 @@ def test_lds_gather_matches_direct_gather(cfl):
             assert relerr(a, b) < 1e-6
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_det_def_grad(shape, inverse, oracle):
+    """det(∇y^{±1}) by SL on the GPU against the FP32 oracle, with the
+    min/max/mean statistics and CLAIREBase::CheckBounds."""
+    print(f"\n[gpu case] det_def_grad {shape} inverse={inverse}", flush=True)
+    v = synth.bandlimited_velocity(shape, kmax=3, cfl=1.5, nt=NT, seed=60)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA)
+    try:
+        c.set_velocity(v)
+        J, st = c.det_def_grad(inverse)
+        reached, st2 = c.check_bounds(v, inverse, bound=0.25)
+        reached_tight, _ = c.check_bounds(v, inverse, bound=min(st[0], 1.0 / st[1]) * 1.01)
+    finally:
+        c.close()
+    Jr, str_ = oracle.det_def_grad(v, NT, inverse, prec="f32")
+    assert relerr(J, Jr) < 1e-5
+    assert np.allclose(st, str_, rtol=1e-5, atol=0)
+    assert np.allclose(st2, st, rtol=1e-6, atol=0)
+    assert (not reached) == (min(st[0], 1.0 / st[1]) > 0.25)
+    assert reached_tight

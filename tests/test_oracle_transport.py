@@ -268,3 +268,41 @@ def test_gauss_newton_krylov_invariants(oracle):
     assert rep32["iterations"] == rep["iterations"] and rep32["pcg_iterations"] == rep["pcg_iterations"]
     assert abs(rep32["J"] - rep["J"]) < 1e-5 * rep["J"]
     assert relerr(v32, v) < 1e-4
+
+
+def _flow_determinant(shape, A, sign, steps=400):
+    """exp(sign·∫₀¹ (∇·v)(y(s)) ds) along dy/ds = −v(y), y(0) = x, for the
+    analytic v = A·(sin x1 cos x2, sin x2 cos x3, sin x3 cos x1), by RK4."""
+    x = np.stack(np.meshgrid(*[np.arange(n) * 2 * np.pi / n for n in shape], indexing="ij")).reshape(3, -1)
+
+    def vel(y):
+        return A * np.stack([np.sin(y[0]) * np.cos(y[1]), np.sin(y[1]) * np.cos(y[2]), np.sin(y[2]) * np.cos(y[0])])
+
+    def dv(y):
+        return A * (np.cos(y[0]) * np.cos(y[1]) + np.cos(y[1]) * np.cos(y[2]) + np.cos(y[2]) * np.cos(y[0]))
+
+    h, y, integral = 1.0 / steps, x.copy(), 0.0
+    for _ in range(steps):
+        k1 = -vel(y)
+        k2 = -vel(y + h / 2 * k1)
+        k3 = -vel(y + h / 2 * k2)
+        k4 = -vel(y + h * k3)
+        integral = integral + h / 6 * (dv(y) + 2 * dv(y + h / 2 * k1) + 2 * dv(y + h / 2 * k2) + dv(y + h * k3))
+        y = y + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    return np.exp(sign * integral).reshape(shape)
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_det_def_grad_converges_to_flow_determinant(oracle, inverse):
+    """det(∇y) by SL (src/DeformationFields.cpp:636-740) converges at second
+    order in ht to the exact determinant of the characteristic flow,
+    exp(±∫∇·v) along the back-traced path; v = 0 gives J = 1 up to the
+    round-off of the foot points (x/2π is not exactly i/N)."""
+    shape, A = (32, 32, 32), 0.5
+    x1, x2, x3 = _grid(shape)
+    v = A * np.stack([np.sin(x1) * np.cos(x2), np.sin(x2) * np.cos(x3), np.sin(x3) * np.cos(x1)])
+    exact = _flow_determinant(shape, A, -1.0 if inverse else 1.0)
+    errs = [np.abs(oracle.det_def_grad(v, nt, inverse)[0] - exact).max() for nt in (4, 8, 16)]
+    assert errs[2] < 2e-2 and errs[1] < errs[0] / 3 and errs[2] < errs[1] / 3
+    J0, st = oracle.det_def_grad(np.zeros((3,) + shape), 4, inverse)
+    assert np.abs(J0 - 1.0).max() < 1e-12 and np.allclose(st, 1.0, rtol=0, atol=1e-12)
