@@ -64,6 +64,32 @@ def _vp(p):
     return ctypes.c_void_p(p)
 
 
+def nifti_read(path):
+    """Reads a NIfTI-1 volume (.nii / .nii.gz): returns (array of shape
+    (n1, n2, n3) = (nz, ny, nx), float32 in the grid's flat order, and the
+    348 header bytes)."""
+    nx = (ctypes.c_int * 3)()
+    hdr = (ctypes.c_ubyte * 348)()
+    _check(lib().clr_nifti_read(str(path).encode(), nx, None, ctypes.c_int64(0), hdr))
+    out = np.empty(tuple(nx), dtype=np.float32)
+    _check(lib().clr_nifti_read(str(path).encode(), nx, out.ctypes.data_as(ctypes.c_void_p),
+                                ctypes.c_int64(out.size), hdr))
+    return out, bytes(hdr)
+
+
+def nifti_write(path, a, spacing=None, header=None):
+    """Writes a FLOAT32 NIfTI-1 volume from an array of shape (n1, n2, n3);
+    `header` (348 bytes of a read image) and `spacing` of (x1, x2, x3) are
+    optional."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 3:
+        raise ValueError("expected a 3D array")
+    nx = (ctypes.c_int * 3)(*a.shape)
+    sp = (ctypes.c_float * 3)(*spacing) if spacing is not None else None
+    hd = (ctypes.c_ubyte * 348)(*header) if header is not None else None
+    _check(lib().clr_nifti_write(str(path).encode(), nx, a.ctypes.data_as(ctypes.c_void_p), sp, hd))
+
+
 class GNOptions(ctypes.Structure):
     """clr_gn_options (include/claire_amd.h)."""
     _fields_ = [("maxit", ctypes.c_int), ("gttol", ctypes.c_double), ("gatol", ctypes.c_double),

@@ -7,6 +7,7 @@
 
 #include "../../include/claire_amd.h"
 #include "claire.hpp"
+#include "nifti.hpp"
 #include "solver.hpp"
 
 using claire_amd::Error;
@@ -235,6 +236,24 @@ int clr_evaluate_gradient(clr_ctx* ctx, const float* v, float* g) {
         need(g, "g");
         C(ctx).EvaluateGradient(g, v);
         check_hip_launch();
+    });
+}
+
+int clr_nifti_read(const char* path, int nx[3], float* data, int64_t capacity, unsigned char* header) {
+    return guard([&] {
+        need(path, "path");
+        need(nx, "nx");
+        claire_amd::nifti::read(path, nx, data, capacity, header);
+    });
+}
+
+int clr_nifti_write(const char* path, const int nx[3], const float* data, const float* spacing,
+                    const unsigned char* header) {
+    return guard([&] {
+        need(path, "path");
+        need(nx, "nx");
+        need(data, "data");
+        claire_amd::nifti::write(path, nx, data, spacing, header);
     });
 }
 

@@ -76,6 +76,20 @@ int clr_det_def_grad(clr_ctx* ctx, int inverse, float* jac, double* stats);
  * min(J) <= bound or 1/max(J) <= bound (default bound 0.25, src/RegOpt.cpp:1254). */
 int clr_check_bounds(clr_ctx* ctx, const float* v, int inverse, double bound, int* reached, double* stats);
 
+/* NIfTI-1 single-file images (.nii, .nii.gz), replacing ReadWriteReg::ReadNII
+ * and WriteNII (src/ReadWriteReg.cpp:794-1475).  Host memory.  The image's
+ * (nx, ny, nz) is the grid's (n3, n2, n1) (src/ReadWriteReg.cpp:817-819), so
+ * voxels come in the grid's flat order (x3 fastest), converted to float
+ * without intensity scaling (src/ReadWriteReg.cpp:1083-1110).
+ * clr_nifti_read: nx receives (n1, n2, n3); data (capacity floats) may be
+ * NULL to query the size; header (348 bytes) may be NULL.
+ * clr_nifti_write: FLOAT32 volume; header (348 bytes of a read image, as
+ * nifti_copy_nim_info does, src/ReadWriteReg.cpp:1157-1170) and spacing of
+ * (x1, x2, x3) may be NULL. */
+int clr_nifti_read(const char* path, int nx[3], float* data, int64_t capacity, unsigned char* header);
+int clr_nifti_write(const char* path, const int nx[3], const float* data, const float* spacing,
+                    const unsigned char* header);
+
 /* Arithmetic form of the Hessian matvec.  CLR_PRECISION_FP32 (default) is
  * the reference's single-precision path.  CLR_PRECISION_MIXED stores the
  * Newton-invariant gradient fields ∇m^j and (∇m^j)(X_s) that the streaming
