@@ -16,8 +16,9 @@
 //            holds complete x1 columns, so forward FFT, per-mode multiplier
 //            and inverse FFT happen in one read and one write.
 // Inside a block a length-N FFT is a Stockham autosort sequence of radix-16/8/
-// 4/2 stages; each thread keeps 16 values in registers and the block swaps
-// them through LDS between stages.  The inverse uses the forward radices in
+// 5/4/3/2 stages; each thread keeps V values in registers (V a multiple of
+// every radix of the length) and the block swaps them through LDS between
+// stages.  The inverse uses the forward radices in
 // reverse order, so the forward output layout in registers is exactly the
 // inverse input layout and fused forward·multiply·inverse needs no exchange.
 // Twiddles come from a table generated in double precision on the host.
@@ -99,13 +100,49 @@ __device__ __forceinline__ void dft16(c32* v) {
     }
 }
 
+constexpr float kS3 = 0.86602540378443864676f;   // sin(2π/3)
+constexpr float kC51 = 0.30901699437494742410f;  // cos(2π/5)
+constexpr float kC52 = -0.80901699437494742410f; // cos(4π/5)
+constexpr float kS51 = 0.95105651629515357212f;  // sin(2π/5)
+constexpr float kS52 = 0.58778525229247312917f;  // sin(4π/5)
+
+// X_k = Σ_m x_m·w^{mk}, w = exp(DIR·2πi/3)
+template <int DIR>
+__device__ __forceinline__ void dft3(c32* v) {
+    const c32 t1 = cadd(v[1], v[2]);
+    const c32 t2 = {v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y};
+    const c32 d = csub(v[1], v[2]);
+    const c32 t3 = mul_i<DIR>(c32{kS3 * d.x, kS3 * d.y});
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, t3);
+    v[2] = csub(t2, t3);
+}
+
+// X_k = Σ_m x_m·w^{mk}, w = exp(DIR·2πi/5)
+template <int DIR>
+__device__ __forceinline__ void dft5(c32* v) {
+    const c32 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const c32 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const c32 a1 = {v[0].x + kC51 * t1.x + kC52 * t2.x, v[0].y + kC51 * t1.y + kC52 * t2.y};
+    const c32 a2 = {v[0].x + kC52 * t1.x + kC51 * t2.x, v[0].y + kC52 * t1.y + kC51 * t2.y};
+    const c32 b1 = mul_i<DIR>(c32{kS51 * t3.x + kS52 * t4.x, kS51 * t3.y + kS52 * t4.y});
+    const c32 b2 = mul_i<DIR>(c32{kS52 * t3.x - kS51 * t4.x, kS52 * t3.y - kS51 * t4.y});
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+}
+
 template <int R, int DIR>
 __device__ __forceinline__ void dft(c32* v) {
     if constexpr (R == 2) dft2<DIR>(v);
+    else if constexpr (R == 3) dft3<DIR>(v);
     else if constexpr (R == 4) dft4<DIR>(v);
+    else if constexpr (R == 5) dft5<DIR>(v);
     else if constexpr (R == 8) dft8<DIR>(v);
     else if constexpr (R == 16) dft16<DIR>(v);
-    else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "unsupported radix");
+    else static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16, "unsupported radix");
 }
 
 // ---------------------------------------------------------------------------
@@ -127,6 +164,11 @@ template <> struct Plan<64>  { static constexpr int V = 16; using Fwd = Radices<
 template <> struct Plan<128> { static constexpr int V = 16; using Fwd = Radices<16, 8>;     using Inv = Radices<8, 16>; };
 template <> struct Plan<256> { static constexpr int V = 16; using Fwd = Radices<16, 16>;    using Inv = Radices<16, 16>; };
 template <> struct Plan<512> { static constexpr int V = 16; using Fwd = Radices<16, 16, 2>; using Inv = Radices<2, 16, 16>; };
+// lengths with factors 3 and 5 (every radix divides V)
+template <> struct Plan<48>  { static constexpr int V = 12; using Fwd = Radices<4, 4, 3>;     using Inv = Radices<3, 4, 4>; };
+template <> struct Plan<96>  { static constexpr int V = 24; using Fwd = Radices<8, 4, 3>;     using Inv = Radices<3, 4, 8>; };
+template <> struct Plan<150> { static constexpr int V = 30; using Fwd = Radices<2, 3, 5, 5>;  using Inv = Radices<5, 5, 3, 2>; };
+template <> struct Plan<192> { static constexpr int V = 24; using Fwd = Radices<8, 8, 3>;     using Inv = Radices<3, 8, 8>; };
 
 template <class L> struct First;
 template <int R, int... RS> struct First<Radices<R, RS...>> { static constexpr int value = R; };
@@ -312,7 +354,7 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_row_r2c(const float* __rest
     c32* ob = out + rb * NH;
     for (int k = t; k < NH; k += T) {
         const c32 zk = s[lay(k, c)];
-        const c32 zm = s[lay((N - k) & (N - 1), c)];
+        const c32 zm = s[lay(k == 0 ? 0 : N - k, c)];
         if (va) oa[k] = {0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y)};
         if (vb) ob[k] = {0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x)};
     }
@@ -560,6 +602,10 @@ void dispatch_n(int n, A&&... a) {
         case 128: F<128>::go(a...); break;
         case 256: F<256>::go(a...); break;
         case 512: F<512>::go(a...); break;
+        case 48: F<48>::go(a...); break;
+        case 96: F<96>::go(a...); break;
+        case 150: F<150>::go(a...); break;
+        case 192: F<192>::go(a...); break;
         default: throw Error(kErrUnsupportedSize, "FFT length " + std::to_string(n) + " not supported");
     }
 }
@@ -632,13 +678,15 @@ ColGeom geom_x1(const Dims& d) {
 }  // namespace
 
 bool fft_size_supported(int n) {
-    return n == 8 || n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
+    return n == 8 || n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 48 || n == 96 ||
+           n == 150 || n == 192;
 }
 
 FFTPlan3D::FFTPlan3D(const Dims& d) : dims(d) {
     for (int a = 0; a < 3; ++a) {
         CLR_REQUIRE(fft_size_supported(d.n[a]), kErrUnsupportedSize,
-                    "grid size " + std::to_string(d.n[a]) + " not supported (8..512, power of two)");
+                    "grid size " + std::to_string(d.n[a]) +
+                        " not supported (powers of two 8..512, and 48, 96, 150, 192)");
         std::vector<c32> h(d.n[a]);
         for (int k = 0; k < d.n[a]; ++k) {
             const double ang = -2.0 * 3.14159265358979323846 * static_cast<double>(k) / static_cast<double>(d.n[a]);

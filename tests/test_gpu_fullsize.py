@@ -129,3 +129,27 @@ def test_mixed_hessian_accuracy_vs_fp64_128(norm, oracle):
     e_fp32, e_mixed = relerr(H, H64), relerr(Hm, H64)
     print(f"relerr vs FP64: gpu fp32 {e_fp32:.3e}  gpu mixed {e_mixed:.3e}")
     assert e_mixed <= max(1e-4, 2.0 * e_fp32)
+
+
+def test_reference_image_size_128x150x128(oracle):
+    """The size of the reference's example images (doc/data/brain0{1,2}.nii.gz,
+    128×150×128): radix-3/5 FFT stages along x2, ragged LDS tiles, and the
+    Hessian against the reference FP32 path."""
+    shape = (128, 150, 128)
+    print(f"\n[gpu case] reference image size {shape}", flush=True)
+    v = synth.bandlimited_velocity(shape, kmax=6, cfl=2.0, nt=NT, seed=70)
+    mT = synth.gaussian_blobs(shape, count=16, seed=71)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0))
+    try:
+        back = c.fft_c2r(c.fft_r2c(mT)) / np.float32(mT.size)
+        assert relerr(back, mT) < 2e-6
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        H = c.hessian_matvec(v)
+    finally:
+        c.close()
+    H32 = oracle.hessian_matvec(v, oracle.solve_state(v, mT, NT, prec="f32"), v, NT, "h1sn", (1e-2, 1e-4, 0.0),
+                                prec="f32")
+    err = relerr(H, H32)
+    print(f"Hessian relerr vs reference FP32 path {err:.3e}")
+    assert err < 2e-5

@@ -13,7 +13,7 @@ from conftest import relerr
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(16, 16, 16), (32, 16, 64)]
+SHAPES = [(16, 16, 16), (32, 16, 64), (48, 96, 150)]  # the last one uses radix-3/5 FFT stages
 NORMS = ["h1", "h2", "h1sn", "h2sn"]
 BETA = (1e-2, 1e-4, 0.0)
 NT = 4
