@@ -176,31 +176,44 @@ __device__ __forceinline__ bool plan_encode(float u, int idx, int n, uint32_t& w
 }
 
 // ---------------------------------------------------------------------------
-// Epilogues of the trajectory-based gathers: `aux(i)` loads the per-point
-// operand the update needs (issued with the query point), `store` writes
-// the update of f(X).
+// Epilogues of the trajectory-based gathers.  Direct kernels use aux(i) (the
+// per-point operand of the update) and store(i, f(X), aux); the LDS kernel
+// reads auxp() and writes apply(f(X), aux) to outp() itself, and kHasAux
+// says whether there is an operand at all.
 // ---------------------------------------------------------------------------
 struct EpiPlain {  // out = f(X)
+    static constexpr bool kHasAux = false;
     float* out;
     int64_t ozstride;
     __device__ void select(int z) { out += z * ozstride; }
+    __device__ const float* auxp() const { return nullptr; }
+    __device__ float* outp() const { return out; }
+    __device__ float apply(float v, float) const { return v; }
     __device__ float aux(int64_t) const { return 0.0f; }
     __device__ void store(int64_t i, float v, float) const { out[i] = v; }
 };
 
 struct EpiIncStep {  // out = sign·(f(X) − s)   (src/CLAIRE.cpp:3228-3273)
+    static constexpr bool kHasAux = true;
     const float* s;
     float* out;
     float sign;
     __device__ void select(int) {}
+    __device__ const float* auxp() const { return s; }
+    __device__ float* outp() const { return out; }
+    __device__ float apply(float v, float sv) const { return sign * (v - sv); }
     __device__ float aux(int64_t i) const { return s[i]; }
     __device__ void store(int64_t i, float v, float sv) const { out[i] = sign * (v - sv); }
 };
 
 struct EpiAdjStep {  // out = c·f(X)   (src/CLAIRE.cpp:2668-2673, 3955-3962)
+    static constexpr bool kHasAux = true;
     const float* c;
     float* out;
     __device__ void select(int) {}
+    __device__ const float* auxp() const { return c; }
+    __device__ float* outp() const { return out; }
+    __device__ float apply(float v, float cv) const { return cv * v; }
     __device__ float aux(int64_t i) const { return c[i]; }
     __device__ void store(int64_t i, float v, float cv) const { out[i] = cv * v; }
 };

@@ -5,8 +5,10 @@
 // src/interp3.cpp:2508-2533 on the rescaled plan points, modular ghost
 // indexing), but the field is read through LDS:
 //
-//   * a block owns a 16 (x2) × 64 (x3) tile of output points and marches
-//     along x1 over a chunk of 32 planes ("2.5D" streaming);
+//   * a block owns a 16 (x2) × 32 or 64 (x3) tile of output points and
+//     marches along x1 over a chunk of planes ("2.5D" streaming), the chunk
+//     length chosen per launch so that the blocks fill whole rounds of the
+//     GPU;
 //   * LDS holds a ring of input planes, each the tile plus a halo of D+1
 //     (low side) and D+2 (high side) cells in x2 and x3, where D is the exact
 //     per-axis stencil displacement bound of the plan (D <= 2 covers CFL 2),
@@ -16,9 +18,7 @@
 //     in flight and land in the spare slot / registers afterwards, so each
 //     plane costs one barrier and no exposed HBM latency;
 //   * every input element is read from HBM/L2 once per (tile, chunk); the 64
-//     stencil values per point come from LDS (ds_read2_b32 pairs), and each
-//     thread contracts its two points together so twice as many LDS reads
-//     are in flight.
+//     stencil values per point come from LDS (ds_read2_b32 pairs).
 // Blocks are remapped so that each XCD streams one contiguous range of
 // tiles and keeps the shared halo rows in its own L2.
 #include "gather_device.hpp"
@@ -30,21 +30,23 @@ namespace {
 using namespace gather;
 
 constexpr int kLT2 = 16;        // tile extent in x2
-constexpr int kLT3 = 64;        // tile extent in x3
-constexpr int kLC1 = 32;        // planes per block
-constexpr int kLThreads = 512;  // 8 waves; each thread evaluates 2 points per plane
-constexpr int kPts = 2;
-static_assert(kLT2 * kLT3 == kLThreads * kPts, "one point pair per thread and plane");
+constexpr int kLThreads = 512;  // 8 waves; wave w covers tile rows 2w and 2w+1
 
 // LDS plane geometry for displacement bound D.  A plane holds E2 rows of E3
 // valid columns at a row pitch that is a multiple of 32 dwords: a wave reads
 // two tile rows of 32 points each (one row per half-wave), so lanes whose
 // stencils start on different rows or planes still hit different banks
 // unless their columns collide mod 32.
+//   D <= 2: a 16 × 32 tile, one point per thread, 53 KB of LDS -- three
+//           blocks (24 waves) per CU;
+//   D <= 4: a 16 × 64 tile, two points per thread 32 columns apart, 135 KB.
 template <int D>
 struct LdsGeo {
+    static constexpr int LT3 = (D <= 2) ? 32 : 64;  // tile extent in x3
+    static constexpr int PTS = LT3 / 32;             // points per thread and plane
+    static constexpr int WAVES_PER_EU = (D <= 2) ? 6 : 2;
     static constexpr int HLO = D + 1, HHI = D + 2;
-    static constexpr int E2 = kLT2 + HLO + HHI, E3 = kLT3 + HLO + HHI;
+    static constexpr int E2 = kLT2 + HLO + HHI, E3 = LT3 + HLO + HHI;
     static constexpr int PITCH = (E3 + 31) / 32 * 32;
     static constexpr int P = E2 * PITCH;       // dwords per LDS plane
     static constexpr int NV = E2 * E3;         // valid elements per plane
@@ -58,6 +60,32 @@ __device__ __forceinline__ int pmod(int a, int n) {
     return r < 0 ? r + n : r;
 }
 
+// Raw buffer accesses: a 32-bit per-lane byte offset plus a uniform byte
+// offset (the plane), against a resource that bounds the array (reads past
+// it return 0, writes past it are dropped).  Word 3 of a gfx9 raw buffer
+// resource: 32-bit data, no swizzle.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kBufferWord3 = 0x00020000;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, static_cast<int>(bytes), kBufferWord3);
+}
+__device__ __forceinline__ float bload(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+__device__ __forceinline__ PlanPt bload_plan(rsrc_t r, uint32_t voff, uint32_t soff) {
+    typedef unsigned int u3 __attribute__((ext_vector_type(3)));
+    const u3 w = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+    PlanPt p;
+    p.w[0] = w.x;
+    p.w[1] = w.y;
+    p.w[2] = w.z;
+    return p;
+}
+
 // Keeps the compiler from splitting an LDS base address into a register part
 // plus a constant too large for the ds_read2 offset fields; the stencil's row
 // and column offsets then all fit the instruction's immediates.
@@ -67,13 +95,13 @@ __device__ __forceinline__ int opaque(int x) {
 }
 
 template <int D, class Epi>
-__global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(D <= 2 ? 4 : 2))) void k_gather_lds(
+__global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGeo<D>::WAVES_PER_EU))) void k_gather_lds(
     GatherDims g, const PlanPt* __restrict__ plan, const float* __restrict__ fbase, int64_t fzstride, Epi epi,
-    int nblocks) {
+    int nblocks, int chunk) {
     using G = LdsGeo<D>;
     __shared__ float lds[G::W * G::P];
     const int tid = threadIdx.x;
-    const int nt3 = (g.n3 + kLT3 - 1) / kLT3;
+    const int nt3 = (g.n3 + G::LT3 - 1) / G::LT3;
     const int nt2 = (g.n2 + kLT2 - 1) / kLT2;
     const int L = xcd_remap(blockIdx.x, nblocks);
     const int t3 = L % nt3;
@@ -81,128 +109,148 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(D <= 
     const int ch = L / (nt3 * nt2);
     const float* f = fbase + static_cast<int64_t>(blockIdx.y) * fzstride;
     epi.select(blockIdx.y);
-    const int x1s = ch * kLC1;
-    const int x1e = min(x1s + kLC1, g.n1);
+    // Byte sizes fit 32 bits: the host requires nl < 2^28 (12 B per point).
+    const uint32_t fbytes = static_cast<uint32_t>(g.nl) * 4u;
+    const rsrc_t rf = make_rsrc(f, fbytes);
+    const rsrc_t rp = make_rsrc(plan, 3u * fbytes);
+    const rsrc_t ra = make_rsrc(Epi::kHasAux ? epi.auxp() : f, fbytes);
+    const rsrc_t ro = make_rsrc(epi.outp(), fbytes);
+    const uint32_t plane4 = static_cast<uint32_t>(g.n2) * static_cast<uint32_t>(g.n3) * 4u;  // bytes per plane
+    const int x1s = ch * chunk;
+    const int x1e = min(x1s + chunk, g.n1);
     const int r2 = t2 * kLT2 - G::HLO;
-    const int r3 = t3 * kLT3 - G::HLO;
-    const int64_t plane = static_cast<int64_t>(g.n2) * g.n3;
+    const int r3 = t3 * G::LT3 - G::HLO;
 
     // Offsets (within a global plane and within an LDS plane) of the tile
     // elements this thread loads; elements past the tile reload element
     // NV−1 (never stored).
-    int foff[G::FILL], loff[G::FILL];
+    uint32_t foff[G::FILL];
+    int loff[G::FILL];
 #pragma unroll
     for (int k = 0; k < G::FILL; ++k) {
         const int e = min(tid + kLThreads * k, G::NV - 1);
         const int e2 = e / G::E3;
         const int e3 = e - e2 * G::E3;
-        foff[k] = pmod(r2 + e2, g.n2) * g.n3 + pmod(r3 + e3, g.n3);
+        foff[k] = 4u * static_cast<uint32_t>(pmod(r2 + e2, g.n2) * g.n3 + pmod(r3 + e3, g.n3));  // bytes
         loff[k] = e2 * G::PITCH + e3;
     }
 
     // This thread's two output points within a plane: wave w covers tile rows
     // 2w and 2w+1, 32 columns each, and the two points are 32 columns apart.
-    // Invalid points load point 0 of the plane and skip the store.
+    // Invalid points load point 0 of the plane and skip the store.  rc4 is
+    // the byte offset of the point's own cell in an LDS plane.
     const int lane = tid & 63;
     const int lx2 = 2 * (tid >> 6) + (lane >> 5);
     const int x2 = t2 * kLT2 + lx2;
-    int off[kPts], rc[kPts];
-    bool valid[kPts];
+    uint32_t off[G::PTS];
+    int rc4[G::PTS];
+    bool valid[G::PTS];
 #pragma unroll
-    for (int k = 0; k < kPts; ++k) {
+    for (int k = 0; k < G::PTS; ++k) {
         const int lx3 = (lane & 31) + 32 * k;
-        const int x3 = t3 * kLT3 + lx3;
+        const int x3 = t3 * G::LT3 + lx3;
         valid[k] = x2 < g.n2 && x3 < g.n3;
-        off[k] = valid[k] ? x2 * g.n3 + x3 : 0;
-        rc[k] = (lx2 + G::HLO) * G::PITCH + lx3 + G::HLO;
+        off[k] = valid[k] ? 4u * static_cast<uint32_t>(x2 * g.n3 + x3) : 0u;  // bytes of a float
+        rc4[k] = 4 * ((lx2 + G::HLO) * G::PITCH + lx3 + G::HLO);
     }
 
     // prologue: planes x1s−HLO … x1s+HHI into ring slots 0 … WIN−1, and the
     // query points of plane x1s
     for (int k = 0; k < G::WIN; ++k) {
-        const float* src = f + pmod(x1s - G::HLO + k, g.n1) * plane;
+        const uint32_t src = static_cast<uint32_t>(pmod(x1s - G::HLO + k, g.n1)) * plane4;
         float* dst = lds + k * G::P;
 #pragma unroll
         for (int q = 0; q < G::FILL; ++q)
-            if (tid + kLThreads * q < G::NV) dst[loff[q]] = src[foff[q]];
+            if (tid + kLThreads * q < G::NV) dst[loff[q]] = bload(rf, foff[q], src);
     }
-    PlanPt cur[kPts];
-    float caux[kPts];
+    // Query points of plane x1 live in (qa, aa) or (qb, ab); the loop below
+    // alternates the two so that the prefetch of the next plane's points
+    // needs no register copies.
+    PlanPt qa[G::PTS], qb[G::PTS];
+    float aa[G::PTS], ab[G::PTS];
+    {
+        const uint32_t pb = static_cast<uint32_t>(x1s) * plane4;
 #pragma unroll
-    for (int k = 0; k < kPts; ++k) {
-        const int64_t i = static_cast<int64_t>(x1s) * plane + off[k];
-        cur[k] = plan[i];
-        caux[k] = epi.aux(i);
+        for (int k = 0; k < G::PTS; ++k) {
+            qa[k] = bload_plan(rp, 3u * off[k], 3u * pb);
+            aa[k] = Epi::kHasAux ? bload(ra, off[k], pb) : 0.0f;
+        }
     }
     __syncthreads();
 
+    constexpr int P4 = 4 * G::P;  // bytes per ring slot
+    const char* ldsb = reinterpret_cast<const char*>(lds);
     int cs = 0;  // ring slot of plane x1 − HLO
-    for (int x1 = x1s; x1 < x1e; ++x1) {
+    auto step = [&](int x1, const PlanPt (&cur)[G::PTS], const float (&caux)[G::PTS], PlanPt (&nxt)[G::PTS],
+                    float (&naux)[G::PTS]) {
         // Prefetch for the next step, unconditionally (past the chunk end it
         // reloads a plane of this chunk; the values are never used).
         const int xn = (x1 + 1 < x1e) ? x1 + 1 : x1;
         float pre[G::FILL];
         {
-            const float* src = f + pmod(x1 + G::HHI + 1, g.n1) * plane;
+            const uint32_t src = static_cast<uint32_t>(pmod(x1 + G::HHI + 1, g.n1)) * plane4;
 #pragma unroll
-            for (int q = 0; q < G::FILL; ++q) pre[q] = src[foff[q]];
+            for (int q = 0; q < G::FILL; ++q) pre[q] = bload(rf, foff[q], src);
         }
-        PlanPt nxt[kPts];
-        float naux[kPts];
+        {
+            const uint32_t pb = static_cast<uint32_t>(xn) * plane4;
 #pragma unroll
-        for (int k = 0; k < kPts; ++k) {
-            const int64_t i = static_cast<int64_t>(xn) * plane + off[k];
-            nxt[k] = plan[i];
-            naux[k] = epi.aux(i);
+            for (int k = 0; k < G::PTS; ++k) {
+                nxt[k] = bload_plan(rp, 3u * off[k], 3u * pb);
+                naux[k] = Epi::kHasAux ? bload(ra, off[k], pb) : 0.0f;
+            }
         }
 
-        // Stencils: weights and the LDS address of row (a, b = 0) of each point.
-        // The plan's bound keeps every offset within the ring; the clamps only
-        // pin LDS addresses inside the allocation.
-        float w1[kPts][4], w2[kPts][4], w3[kPts][4];
-        int base[kPts][4], base2[kPts][4];
+        // Stencils: weights and the LDS byte addresses of rows 0 and 3 of the
+        // four planes of each point.  The plan's bound keeps every offset
+        // within the ring; the clamps only pin LDS addresses inside the
+        // allocation.
+        float w1[G::PTS][4], w2[G::PTS][4], w3[G::PTS][4];
+        int base[G::PTS][4], base3[G::PTS][4];
 #pragma unroll
-        for (int k = 0; k < kPts; ++k) {
+        for (int k = 0; k < G::PTS; ++k) {
             lagrange_weights(plan_t(cur[k].w[0]), w1[k]);
             lagrange_weights(plan_t(cur[k].w[1]), w2[k]);
             lagrange_weights(plan_t(cur[k].w[2]), w3[k]);
-            const int lb = min(max(rc[k] + plan_off(cur[k].w[1]) * G::PITCH + plan_off(cur[k].w[2]), 0),
-                               G::P - 3 * G::PITCH - 4);
-            const int s0 = min(max(cs + G::HLO + plan_off(cur[k].w[0]), 0), 2 * G::W - 4);
+            const int lb4 = min(max(rc4[k] + plan_off(cur[k].w[1]) * (4 * G::PITCH) + 4 * plan_off(cur[k].w[2]), 0),
+                                4 * (G::P - 3 * G::PITCH - 4));
+            int sw = min(max(cs + G::HLO + plan_off(cur[k].w[0]), 0), 2 * G::W - 4);
+            sw -= (sw >= G::W) ? G::W : 0;  // ring slot of the first stencil plane
+            base[k][0] = opaque(sw * P4 + lb4);
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                int sa = s0 + a;
-                sa -= (sa >= G::W) ? G::W : 0;
-                // rows 0-1 and 2-3 from two bases: 3·PITCH + 3 exceeds the
-                // 8-bit dword offsets of ds_read2_b32
-                base[k][a] = opaque(sa * G::P + lb);
-                base2[k][a] = opaque(base[k][a] + 2 * G::PITCH);
-            }
+            for (int a = 1; a < 4; ++a) base[k][a] = opaque(base[k][a - 1] + ((sw + a == G::W) ? P4 - G::W * P4 : P4));
+            // rows 0-2 fit the 8-bit dword offsets of ds_read2_b32 from base;
+            // row 3 (3·PITCH + 3 dwords away) needs its own base
+#pragma unroll
+            for (int a = 0; a < 4; ++a) base3[k][a] = opaque(base[k][a] + 3 * 4 * G::PITCH);
         }
-        float v[kPts];
+        float v[G::PTS];
 #pragma unroll
-        for (int k = 0; k < kPts; ++k) v[k] = 0.f;
+        for (int k = 0; k < G::PTS; ++k) v[k] = 0.f;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            float sa[kPts];
+            float sa[G::PTS];
 #pragma unroll
-            for (int k = 0; k < kPts; ++k) sa[k] = 0.f;
+            for (int k = 0; k < G::PTS; ++k) sa[k] = 0.f;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
 #pragma unroll
-                for (int k = 0; k < kPts; ++k) {
-                    const float* p = lds + (b < 2 ? base[k][a] + b * G::PITCH : base2[k][a] + (b - 2) * G::PITCH);
+                for (int k = 0; k < G::PTS; ++k) {
+                    const float* p = (b < 3) ? reinterpret_cast<const float*>(ldsb + base[k][a]) + b * G::PITCH
+                                             : reinterpret_cast<const float*>(ldsb + base3[k][a]);
                     const float r = w3[k][0] * p[0] + w3[k][1] * p[1] + w3[k][2] * p[2] + w3[k][3] * p[3];
                     sa[k] += w2[k][b] * r;
                 }
             }
 #pragma unroll
-            for (int k = 0; k < kPts; ++k) v[k] += w1[k][a] * sa[k];
+            for (int k = 0; k < G::PTS; ++k) v[k] += w1[k][a] * sa[k];
         }
-        const int64_t pb = static_cast<int64_t>(x1) * plane;
+        {
+            const uint32_t pb = static_cast<uint32_t>(x1) * plane4;
 #pragma unroll
-        for (int k = 0; k < kPts; ++k)
-            if (valid[k]) epi.store(pb + off[k], v[k], caux[k]);
+            for (int k = 0; k < G::PTS; ++k)
+                if (valid[k]) bstore(ro, epi.apply(v[k], caux[k]), off[k], pb);
+        }
 
         // land the prefetched plane (x1 + HHI + 1) in the spare ring slot
         const int spare = (cs == 0) ? G::W - 1 : cs - 1;
@@ -210,14 +258,16 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(D <= 
 #pragma unroll
         for (int q = 0; q < G::FILL; ++q)
             if (tid + kLThreads * q < G::NV) dst[loff[q]] = pre[q];
-#pragma unroll
-        for (int k = 0; k < kPts; ++k) {
-            cur[k] = nxt[k];
-            caux[k] = naux[k];
-        }
         cs = (cs + 1 == G::W) ? 0 : cs + 1;
         __syncthreads();
+    };
+
+    int x1 = x1s;
+    for (; x1 + 1 < x1e; x1 += 2) {
+        step(x1, qa, aa, qb, ab);
+        step(x1 + 1, qb, ab, qa, aa);
     }
+    if (x1 < x1e) step(x1, qa, aa, qb, ab);
 }
 
 // Incremental state right-hand sides, streamed with 16-byte accesses:
@@ -320,22 +370,58 @@ __global__ __launch_bounds__(256) void k_to_half(const float* __restrict__ in, _
     reinterpret_cast<h4*>(out)[i] = h;
 }
 
+// Planes per block: the chunk count that fills whole rounds of resident
+// blocks (LDS allows 3 blocks per CU for D <= 2, 1 for D <= 4), weighed
+// against the per-block prologue of WIN planes.
+int num_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        CLR_HIP_CHECK(hipGetDevice(&dev));
+        CLR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        if (cus <= 0) cus = 1;
+    }
+    return cus;
+}
+
+template <int D>
+int choose_chunk(int n1, int64_t tiles) {
+    using G = LdsGeo<D>;
+    const int64_t slots = static_cast<int64_t>(num_cus()) * (D <= 2 ? 3 : 1);
+    int best = n1;
+    double best_score = -1.0;
+    for (int chunk = n1; chunk >= 4; --chunk) {
+        const int64_t nch = (n1 + chunk - 1) / chunk;
+        if ((n1 + (chunk - 1) - 1) / (chunk - 1) == nch && chunk > 4) continue;  // same count, larger chunk kept
+        const int64_t blocks = tiles * nch;
+        const int64_t rounds = (blocks + slots - 1) / slots;
+        const double fill = static_cast<double>(blocks) / static_cast<double>(rounds * slots);
+        const double score = fill * chunk / static_cast<double>(chunk + G::WIN / 4);
+        if (score > best_score + 1e-9) {
+            best_score = score;
+            best = chunk;
+        }
+    }
+    return best;
+}
+
 template <int D, class Epi>
 void launch_lds(const GatherDims& g, const uint32_t* P, const float* f, int64_t fzstride, int nz, const Epi& epi,
                 hipStream_t st) {
-    const int nt3 = (g.n3 + kLT3 - 1) / kLT3;
+    const int nt3 = (g.n3 + LdsGeo<D>::LT3 - 1) / LdsGeo<D>::LT3;
     const int nt2 = (g.n2 + kLT2 - 1) / kLT2;
-    const int nch = (g.n1 + kLC1 - 1) / kLC1;
+    const int chunk = choose_chunk<D>(g.n1, static_cast<int64_t>(nt3) * nt2 * nz);
+    const int nch = (g.n1 + chunk - 1) / chunk;
     const int nb = nt3 * nt2 * nch;
     hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
-                       reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb);
+                       reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
 }
 
 template <class Epi>
 void dispatch(int dbound, const GatherDims& g, const uint32_t* P, const float* f, int64_t fzstride, int nz,
               const Epi& epi, hipStream_t st) {
     CLR_REQUIRE(g.n1 >= 4 && g.n2 >= 4 && g.n3 >= 4, kErrInvalidArgument, "gather needs n >= 4 per axis");
-    CLR_REQUIRE(g.nl < (int64_t(1) << 31), kErrInvalidArgument, "grid too large for 32-bit stencil offsets");
+    CLR_REQUIRE(g.nl < (int64_t(1) << 28), kErrInvalidArgument, "grid too large for 32-bit buffer byte offsets");
     CLR_REQUIRE(dbound >= 0 && dbound <= 4, kErrInvalidArgument, "displacement bound too large for the LDS gather");
     if (dbound <= 2)
         launch_lds<2>(g, P, f, fzstride, nz, epi, st);

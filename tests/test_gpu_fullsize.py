@@ -134,7 +134,7 @@ def test_mixed_hessian_accuracy_vs_fp64_128(norm, oracle):
 def test_reference_image_size_128x150x128(oracle):
     """The size of the reference's example images (doc/data/brain0{1,2}.nii.gz,
     128×150×128): radix-3/5 FFT stages along x2, ragged LDS tiles, and the
-    Hessian against the reference FP32 path."""
+    Hessian against FP64 next to the reference FP32 path."""
     shape = (128, 150, 128)
     print(f"\n[gpu case] reference image size {shape}", flush=True)
     v = synth.bandlimited_velocity(shape, kmax=6, cfl=2.0, nt=NT, seed=70)
@@ -148,8 +148,10 @@ def test_reference_image_size_128x150x128(oracle):
         H = c.hessian_matvec(v)
     finally:
         c.close()
-    H32 = oracle.hessian_matvec(v, oracle.solve_state(v, mT, NT, prec="f32"), v, NT, "h1sn", (1e-2, 1e-4, 0.0),
-                                prec="f32")
-    err = relerr(H, H32)
-    print(f"Hessian relerr vs reference FP32 path {err:.3e}")
-    assert err < 2e-5
+    beta = (1e-2, 1e-4, 0.0)
+    H32 = oracle.hessian_matvec(v, oracle.solve_state(v, mT, NT, prec="f32"), v, NT, "h1sn", beta, prec="f32")
+    H64 = oracle.hessian_matvec(v, oracle.solve_state(v, mT, NT), v, NT, "h1sn", beta)
+    e_gpu, e_ref = relerr(H, H64), relerr(H32, H64)
+    print(f"Hessian relerr vs FP64: gpu {e_gpu:.3e}  reference FP32 path {e_ref:.3e}  gpu vs ref {relerr(H, H32):.3e}")
+    # FP32 conditioning at 2.5M points: both FP32 paths sit ~2e-5 from FP64
+    assert e_gpu <= max(1e-5, 2.0 * e_ref)
