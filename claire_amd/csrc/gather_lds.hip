@@ -21,6 +21,9 @@
 //     stencil values per point come from LDS (ds_read2_b32 pairs).
 // Blocks are remapped so that each XCD streams one contiguous range of
 // tiles and keeps the shared halo rows in its own L2.
+#include <algorithm>
+#include <cstdlib>
+
 #include "gather_device.hpp"
 
 namespace claire_amd {
@@ -370,9 +373,9 @@ __global__ __launch_bounds__(256) void k_to_half(const float* __restrict__ in, _
     reinterpret_cast<h4*>(out)[i] = h;
 }
 
-// Planes per block: the chunk count that fills whole rounds of resident
-// blocks (LDS allows 3 blocks per CU for D <= 2, 1 for D <= 4), weighed
-// against the per-block prologue of WIN planes.
+// Planes per block: at most 48, and the chunk count that fills whole rounds
+// of resident blocks (LDS allows 3 blocks per CU for D <= 2, 1 for D <= 4),
+// weighed against the per-block prologue of WIN planes.
 int num_cus() {
     static int cus = 0;
     if (cus == 0) {
@@ -384,13 +387,26 @@ int num_cus() {
     return cus;
 }
 
+// CLAIRE_AMD_GATHER_CHUNK (planes, >= 4) overrides the choice, for tuning.
+int chunk_override() {
+    static const int v = [] {
+        const char* e = std::getenv("CLAIRE_AMD_GATHER_CHUNK");
+        return e ? std::atoi(e) : 0;
+    }();
+    return v;
+}
+
 template <int D>
 int choose_chunk(int n1, int64_t tiles) {
     using G = LdsGeo<D>;
+    if (chunk_override() >= 4) return std::min(chunk_override(), n1);
     const int64_t slots = static_cast<int64_t>(num_cus()) * (D <= 2 ? 3 : 1);
-    int best = n1;
+    // Chunks longer than 48 planes measured slower at 256³ (fewer, longer
+    // blocks start and stream in lockstep); 32-43 planes were best.
+    const int cmax = std::min(n1, 48);
+    int best = cmax;
     double best_score = -1.0;
-    for (int chunk = n1; chunk >= 4; --chunk) {
+    for (int chunk = cmax; chunk >= 4; --chunk) {
         const int64_t nch = (n1 + chunk - 1) / chunk;
         if ((n1 + (chunk - 1) - 1) / (chunk - 1) == nch && chunk > 4) continue;  // same count, larger chunk kept
         const int64_t blocks = tiles * nch;
