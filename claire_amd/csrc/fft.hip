@@ -403,20 +403,54 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_row_c2r(const c32* __restri
             v[q * R0 + r] = {A.x - Bv.y, A.y + Bv.x};
         }
     }
-    fft_regs<N, +1>(v, s, RowLayout<N>{}, c, t, tw);
+    const RowLayout<N> lay;
+    fft_regs<N, +1>(v, s, lay, c, t, tw);
     float* oa = out + ra * N;
     float* ob = out + rb * N;
     const float* aa = add ? add + ra * N : nullptr;
     const float* ab = add ? add + rb * N : nullptr;
+    if constexpr (N % 4 == 0) {
+        // Stage the result through LDS so that each lane writes 4 consecutive
+        // reals (16-byte stores) of both rows.
+        __syncthreads();
 #pragma unroll
-    for (int q = 0; q < V / RL; ++q) {
-        const int j = t + T * q;
+        for (int q = 0; q < V / RL; ++q) {
+            const int j = t + T * q;
 #pragma unroll
-        for (int r = 0; r < RL; ++r) {
-            const int n = j + r * (N / RL);
-            const c32 z = v[q * RL + r];
-            if (va) oa[n] = aa ? alpha * z.x + beta * aa[n] : alpha * z.x;
-            if (vb) ob[n] = ab ? alpha * z.y + beta * ab[n] : alpha * z.y;
+            for (int r = 0; r < RL; ++r) s[lay(j + r * (N / RL), c)] = v[q * RL + r];
+        }
+        __syncthreads();
+        for (int m = t; m < N / 4; m += T) {
+            const c32 z0 = s[lay(4 * m, c)], z1 = s[lay(4 * m + 1, c)], z2 = s[lay(4 * m + 2, c)],
+                      z3 = s[lay(4 * m + 3, c)];
+            if (va) {
+                float4 o = make_float4(alpha * z0.x, alpha * z1.x, alpha * z2.x, alpha * z3.x);
+                if (aa) {
+                    const float4 x = reinterpret_cast<const float4*>(aa)[m];
+                    o = make_float4(o.x + beta * x.x, o.y + beta * x.y, o.z + beta * x.z, o.w + beta * x.w);
+                }
+                reinterpret_cast<float4*>(oa)[m] = o;
+            }
+            if (vb) {
+                float4 o = make_float4(alpha * z0.y, alpha * z1.y, alpha * z2.y, alpha * z3.y);
+                if (ab) {
+                    const float4 x = reinterpret_cast<const float4*>(ab)[m];
+                    o = make_float4(o.x + beta * x.x, o.y + beta * x.y, o.z + beta * x.z, o.w + beta * x.w);
+                }
+                reinterpret_cast<float4*>(ob)[m] = o;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < V / RL; ++q) {
+            const int j = t + T * q;
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                const int n = j + r * (N / RL);
+                const c32 z = v[q * RL + r];
+                if (va) oa[n] = aa ? alpha * z.x + beta * aa[n] : alpha * z.x;
+                if (vb) ob[n] = ab ? alpha * z.y + beta * ab[n] : alpha * z.y;
+            }
         }
     }
 }
