@@ -150,7 +150,8 @@ class DeviceArray:
 class Claire:
     """One registration problem (GN, semi-Lagrangian, nc = 1) on the GPU."""
 
-    def __init__(self, nx, nt=4, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0), stream=None, precision="fp32"):
+    def __init__(self, nx, nt=4, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0), stream=None, precision="fp32",
+                 rk_order=2):
         self.nx = tuple(int(n) for n in nx)
         self.nt = int(nt)
         self.nl = int(np.prod(self.nx))
@@ -163,6 +164,8 @@ class Claire:
         self._bufs = {}
         if precision != "fp32":
             self.set_precision(precision)
+        if rk_order != 2:
+            self.set_rk_order(rk_order)
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -285,6 +288,11 @@ class Claire:
         """"fp32" (the reference's path) or "mixed" (FP16 storage of the
         Newton-invariant gradient fields in the Hessian matvec)."""
         _check(lib().clr_set_precision(self.ctx, ctypes.c_int({"fp32": 0, "mixed": 1}[precision])))
+
+    def set_rk_order(self, order):
+        """Characteristic scheme of every later trajectory: 2 or 4 (the
+        reference's -rkorder option)."""
+        _check(lib().clr_set_rk_order(self.ctx, ctypes.c_int(int(order))))
 
     def solve_gn(self, v0=None, hist_len=128, **options):
         """Gauss–Newton–Krylov registration (clr_solve_gn) from v0 (zero by

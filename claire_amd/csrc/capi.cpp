@@ -282,6 +282,10 @@ int clr_set_precision(clr_ctx* ctx, int precision) {
     return guard([&] { C(ctx).SetPrecision(precision); });
 }
 
+int clr_set_rk_order(clr_ctx* ctx, int order) {
+    return guard([&] { C(ctx).SetRKOrder(order); });
+}
+
 int clr_gn_options_default(clr_gn_options* opt) {
     return guard([&] {
         need(opt, "opt");

@@ -401,11 +401,19 @@ int SemiLagrangian::ComputeTrajectory(const VecField* v, const std::string& flag
     const int w = Which(flag);
     m_X[w].resize(3 * static_cast<size_t>(m_Opt->domain.nl));
     const float* vc[3] = {v->m_X1, v->m_X2, v->m_X3};
-    {
+    const float sigma = w == 0 ? 1.0f : -1.0f;
+    if (m_Opt->rkorder == 4) {
+        m_F.resize(3 * static_cast<size_t>(m_Opt->domain.nl));
+        // bytes: v read (12), v gathered 3× (36), f and X written, read and
+        // rewritten (12 + 12 + 24 + 24 + 12 + 12) per point
+        KernelScope k(m_Opt->prof, "gather_traj_rk4", 144.0 * m_Opt->domain.nl);
+        launch_trajectory_rk4(m_Opt->domain, vc, sigma, m_Opt->GetTimeStepSize(), m_X[w].get(), m_F.get(),
+                              m_Opt->stream);
+    } else {
+        CLR_REQUIRE(m_Opt->rkorder == 2, kErrInvalidArgument, "rk order must be 2 or 4");
         // bytes: v read (12) + v gathered (12) + X written (12) per point
         KernelScope k(m_Opt->prof, "gather_traj_rk2", 36.0 * m_Opt->domain.nl);
-        launch_trajectory_rk2(m_Opt->domain, vc, w == 0 ? 1.0f : -1.0f, m_Opt->GetTimeStepSize(), m_X[w].get(),
-                              m_Opt->stream);
+        launch_trajectory_rk2(m_Opt->domain, vc, sigma, m_Opt->GetTimeStepSize(), m_X[w].get(), m_Opt->stream);
     }
     m_P[w].resize(3 * static_cast<size_t>(m_Opt->domain.nl));
     {
@@ -529,7 +537,8 @@ size_t CLAIRE::DeviceBytes() const {
     const size_t floats = 5 * 3 * nl + 2 * nl + (nt + 1) * nl + nl + 3 * nl + 4 * nl + (nt + 1) * 3 * nl +
                           nt * 3 * nl + nt * nl + nt * nl + nl + 2 * 3 * nl;
     const size_t halves = m_Opt.precision == 1 ? (2 * nt + 1) * 3 * nl : 0;
-    return floats * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
+    const size_t rk4 = m_Opt.rkorder == 4 ? 3 * nl : 0;  // RK4 stage sum
+    return (floats + rk4) * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
 }
 
 void CLAIRE::RequireVelocity() const { CLR_REQUIRE(m_HaveVelocity, kErrState, "velocity field not set"); }
@@ -680,6 +689,16 @@ int CLAIRE::ComputeDetDefGrad(bool inverse, float* jac, double stats[3]) {
 int CLAIRE::SetPrecision(int precision) {
     CLR_REQUIRE(precision == 0 || precision == 1, kErrInvalidArgument, "precision must be 0 (FP32) or 1 (mixed)");
     m_Opt.precision = precision;
+    return 0;
+}
+
+int CLAIRE::SetRKOrder(int order) {
+    // src/RegOpt.cpp:1690-1693
+    CLR_REQUIRE(order == 2 || order == 4, kErrInvalidArgument, "options for rk order are 2 and 4");
+    if (order != m_Opt.rkorder) {
+        m_Opt.rkorder = order;
+        m_TrajValid = m_StateValid = m_GradMValid = m_GradMXValid = m_HalfValid = false;
+    }
     return 0;
 }
 

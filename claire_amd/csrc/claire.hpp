@@ -124,6 +124,9 @@ struct RegOpt {
     // streaming kernels are stored as IEEE half (FP32 arithmetic); objective,
     // gradient and reductions are unchanged.
     int precision = 0;
+    // Order of the characteristic scheme, 2 or 4 (m_PDESolver.rkorder,
+    // src/RegOpt.cpp:674-676, default 2 at src/RegOpt.cpp:1087).
+    int rkorder = 2;
     // include/RegOpt.hpp:485-500, src/RegOpt.cpp:2505-2512
     float GetLebesgueMeasure() const { return domain.hd; }
     float GetTimeStepSize() const { return 1.0f / static_cast<float>(nt); }
@@ -257,6 +260,7 @@ class SemiLagrangian {
     const RegOpt* m_Opt;
     DeviceArray<float> m_X[2];
     DeviceArray<uint32_t> m_P[2];
+    DeviceArray<float> m_F;  // RK4 stage sum f (m_WorkVecField2 of the reference)
     DeviceArray<int> m_Scratch;
     bool m_Valid[2] = {false, false};
     int m_D[2] = {0, 0};
@@ -300,6 +304,9 @@ class CLAIRE {
     int PrepareHessian();
     // 0 plain FP32, 1 mixed (RegOpt::precision).
     int SetPrecision(int precision);
+    // Characteristic scheme of every later trajectory: 2 or 4
+    // (SemiLagrangian::ComputeTrajectory, src/SemiLagrangian.cpp:172-178).
+    int SetRKOrder(int order);
     // det(∇y) (inverse = false) or det(∇y⁻¹) by SL transport
     // (DeformationFields::ComputeDetDefGrad); jac may be null; stats receive
     // min, max and mean.

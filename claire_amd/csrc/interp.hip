@@ -102,6 +102,72 @@ __global__ __launch_bounds__(kThreads) void k_traj_rk2(GatherDims g, HH hh, Traj
     }
 }
 
+// SemiLagrangian::ComputeTrajectoryRK4 (src/SemiLagrangian.cpp:297-459), in
+// three launches, each one gather of v at the previous stage point:
+//   STAGE 1: X¹ = (x − σ·ht/2·v)/2π in registers, f = v + 2·v(X¹),
+//            X² = (x − σ·ht/2·v(X¹))/2π;
+//   STAGE 2: f += 2·v(X²), X³ = (x − σ·ht·v(X²))/2π;
+//   STAGE 3: f += v(X³), X = (x − σ·(ht/6)·f)/2π.
+// f (3 floats per point, interleaved like X) and the stage points live in F
+// and X between launches.  Precision follows the reference: f += 2.0·vX in
+// double, the last update and every division by 2π in double.
+struct Rk4Params {
+    float hx[3];
+    float shth;   // σ·ht/2
+    float sht;    // σ·ht
+    double s6;    // σ·(ht/6.0)
+};
+
+__device__ __forceinline__ void grid_point(const GatherDims& g, const float hx[3], int64_t i, float x[3]) {
+    const int64_t plane = static_cast<int64_t>(g.n2) * g.n3;
+    const int i1 = static_cast<int>(i / plane);
+    const int i2 = static_cast<int>((i / g.n3) % g.n2);
+    const int i3 = static_cast<int>(i % g.n3);
+    x[0] = hx[0] * static_cast<float>(i1);
+    x[1] = hx[1] * static_cast<float>(i2);
+    x[2] = hx[2] * static_cast<float>(i3);
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(kThreads) void k_traj_rk4(GatherDims g, HH hh, Rk4Params tp, P3 v,
+                                                       float* __restrict__ X, float* __restrict__ F, int nblocks) {
+    const int64_t i = point_index(nblocks);
+    if (i >= g.nl) return;
+    float x[3];
+    grid_point(g, tp.hx, i, x);
+    const double twopi = 2.0 * 3.14159265358979323846;
+    float vl[3], q[3];
+    if constexpr (STAGE == 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            vl[d] = v.in[d][i];
+            const float t = x[d] - tp.shth * vl[d];
+            q[d] = static_cast<float>(static_cast<double>(t) / twopi);
+        }
+    } else {
+        load_q3(X, i, q);
+    }
+    Stencil S;
+    setup_stencil(q, g, hh.h, S);
+    float vx[3];
+    eval_stencil<3>(v.in, S, g, vx);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if constexpr (STAGE == 1) {
+            F[3 * i + d] = static_cast<float>(static_cast<double>(vl[d]) + 2.0 * static_cast<double>(vx[d]));
+            const float t = x[d] - tp.shth * vx[d];
+            X[3 * i + d] = static_cast<float>(static_cast<double>(t) / twopi);
+        } else if constexpr (STAGE == 2) {
+            F[3 * i + d] = static_cast<float>(static_cast<double>(F[3 * i + d]) + 2.0 * static_cast<double>(vx[d]));
+            const float t = x[d] - tp.sht * vx[d];
+            X[3 * i + d] = static_cast<float>(static_cast<double>(t) / twopi);
+        } else {
+            const float f = F[3 * i + d] + vx[d];
+            X[3 * i + d] = static_cast<float>((static_cast<double>(x[d]) - tp.s6 * static_cast<double>(f)) / twopi);
+        }
+    }
+}
+
 // Packed plan of a trajectory (gather_device.hpp) and the smallest D such
 // that every stencil lies within [i − D − 1, i + D + 2] per axis (the window
 // of the LDS-staged gathers).  Points that cannot be encoded (non-finite
@@ -240,6 +306,25 @@ void launch_trajectory_rk2(const Dims& d, const float* const v[3], float sigma, 
     tp.shth = sigma * hthalf;
     P3 p{{v[0], v[1], v[2]}, {nullptr, nullptr, nullptr}};
     hipLaunchKernelGGL(k_traj_rk2, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), tp, p, X, nb);
+}
+
+void launch_trajectory_rk4(const Dims& d, const float* const v[3], float sigma, float ht, float* X, float* F,
+                           hipStream_t st) {
+    const GatherDims g = gather_dims(d);
+    check_gather_dims(g);
+    CLR_REQUIRE(F != nullptr && F != X, kErrInvalidArgument, "RK4 trajectory needs a separate 3-field work buffer");
+    const int nb = nblocks_for(g.nl);
+    Rk4Params tp;
+    for (int a = 0; a < 3; ++a) tp.hx[a] = d.hx[a];
+    const float hthalf = 0.5f * ht;
+    tp.shth = sigma * hthalf;
+    tp.sht = sigma * ht;
+    tp.s6 = static_cast<double>(sigma) * (static_cast<double>(ht) / 6.0);
+    P3 p{{v[0], v[1], v[2]}, {nullptr, nullptr, nullptr}};
+    const HH hh = make_hh(g);
+    hipLaunchKernelGGL(k_traj_rk4<1>, dim3(nb), dim3(kThreads), 0, st, g, hh, tp, p, X, F, nb);
+    hipLaunchKernelGGL(k_traj_rk4<2>, dim3(nb), dim3(kThreads), 0, st, g, hh, tp, p, X, F, nb);
+    hipLaunchKernelGGL(k_traj_rk4<3>, dim3(nb), dim3(kThreads), 0, st, g, hh, tp, p, X, F, nb);
 }
 
 int launch_make_plan(const GatherDims& g, const float* X, uint32_t* P, int* scratch, hipStream_t st) {

@@ -64,6 +64,9 @@ void launch_gather3(const GatherDims& g, const float* const f[3], const float* X
 // RK2 characteristic, Euler predictor and velocity gather fused:
 //   X̃ = (x − σ·ht·v)/2π,  X = (x − σ·ht/2·(v(X̃) + v))/2π
 void launch_trajectory_rk2(const Dims& d, const float* const v[3], float sigma, float ht, float* X, hipStream_t st);
+// RK4 characteristic (three launches); F: work buffer of 3·nl floats.
+void launch_trajectory_rk4(const Dims& d, const float* const v[3], float sigma, float ht, float* X, float* F,
+                           hipStream_t st);
 
 // Packed plan of a trajectory: 3 words per point (12 B), one per axis, the
 // fraction t ∈ [1,2) of the rescaled coordinate u = q·N + 3 and the offset of

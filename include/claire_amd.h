@@ -99,6 +99,12 @@ int clr_nifti_write(const char* path, const int nx[3], const float* data, const 
 #define CLR_PRECISION_FP32 0
 #define CLR_PRECISION_MIXED 1
 int clr_set_precision(clr_ctx* ctx, int precision);
+/* Order of the characteristic scheme of every later trajectory: 2 (default)
+ * or 4.  Replaces RegOpt m_PDESolver.rkorder, set by "-rkorder <int>"
+ * (src/RegOpt.cpp:674-676, checked at src/RegOpt.cpp:1690-1693) and read by
+ * SemiLagrangian::ComputeTrajectory (src/SemiLagrangian.cpp:172-178).
+ * Changing it drops the cached trajectories and everything computed on them. */
+int clr_set_rk_order(clr_ctx* ctx, int order);
 /* Device memory held by the context (bytes). */
 int clr_device_bytes(clr_ctx* ctx, size_t* bytes);
 

@@ -532,6 +532,86 @@ void trajectory_rk2(const Grid<T>& g, const T* const v[3], T ht, int sigma, T* X
     }
 }
 
+// SemiLagrangian::ComputeTrajectoryRK4 (src/SemiLagrangian.cpp:297-459),
+// selected by the -rkorder option.  Four stages of the stationary backward
+// characteristic, σ = +1 state / −1 adjoint, with the reference's mixed
+// precision: f += 2.0·vX in double, hthalf = 0.5·ht and ht in float, ht/6.0
+// in double, and the division by 2π in double.
+template <class T>
+void trajectory_rk4(const Grid<T>& g, const T* const v[3], T ht, int sigma, T* X) {
+    const T s = static_cast<T>(sigma);
+    const T hthalf = static_cast<T>(0.5) * ht;
+    const int n1 = g.n[1], n2 = g.n[2];
+    const int64_t nl = g.nl;
+    auto coords = [&](int64_t l, T x[3]) {
+        const int i1 = static_cast<int>(l / (static_cast<int64_t>(n1) * n2));
+        const int i2 = static_cast<int>((l / n2) % n1);
+        const int i3 = static_cast<int>(l % n2);
+        x[0] = g.hx[0] * static_cast<T>(i1);
+        x[1] = g.hx[1] * static_cast<T>(i2);
+        x[2] = g.hx[2] * static_cast<T>(i3);
+    };
+    std::vector<T> f(3 * nl), vx(3 * nl), tmp(nl);
+    auto interp_v = [&]() {
+        for (int d = 0; d < 3; ++d) {
+            interp(g, v[d], X, tmp.data());
+#pragma omp parallel for schedule(static)
+            for (int64_t l = 0; l < nl; ++l) vx[3 * l + d] = tmp[l];
+        }
+    };
+    // stage 1: f = v, X = x − σ·ht/2·v
+#pragma omp parallel for schedule(static)
+    for (int64_t l = 0; l < nl; ++l) {
+        T x[3];
+        coords(l, x);
+        for (int d = 0; d < 3; ++d) {
+            f[3 * l + d] = v[d][l];
+            X[3 * l + d] = static_cast<T>(static_cast<double>(x[d] - s * hthalf * v[d][l]) / (2.0 * kPi));
+        }
+    }
+    // stages 2 and 3: f += 2·v(X), X = x − σ·(ht/2 | ht)·v(X)
+    for (int stage = 2; stage <= 3; ++stage) {
+        interp_v();
+        const T step = (stage == 2) ? hthalf : ht;
+#pragma omp parallel for schedule(static)
+        for (int64_t l = 0; l < nl; ++l) {
+            T x[3];
+            coords(l, x);
+            for (int d = 0; d < 3; ++d) {
+                const T w = vx[3 * l + d];
+                f[3 * l + d] = static_cast<T>(f[3 * l + d] + 2.0 * w);
+                X[3 * l + d] = static_cast<T>(static_cast<double>(x[d] - s * step * w) / (2.0 * kPi));
+            }
+        }
+    }
+    // stage 4: f += v(X), X = x − σ·(ht/6)·f
+    interp_v();
+#pragma omp parallel for schedule(static)
+    for (int64_t l = 0; l < nl; ++l) {
+        T x[3];
+        coords(l, x);
+        for (int d = 0; d < 3; ++d) {
+            f[3 * l + d] += vx[3 * l + d];
+            X[3 * l + d] = static_cast<T>((x[d] - s * (ht / 6.0) * f[3 * l + d]) / (2.0 * kPi));
+        }
+    }
+}
+
+// The characteristic scheme of the semi-Lagrangian solves (RegOpt
+// m_PDESolver.rkorder, default 2).  Test infrastructure sets it per call.
+inline int& rk_order() {
+    static int order = 2;
+    return order;
+}
+
+template <class T>
+void trajectory(const Grid<T>& g, const T* const v[3], T ht, int sigma, T* X) {
+    if (rk_order() == 4)
+        trajectory_rk4(g, v, ht, sigma, X);
+    else
+        trajectory_rk2(g, v, ht, sigma, X);
+}
+
 // ---------------------------------------------------------------------------
 // Transport solves (GN, SL, nc = 1).
 // ---------------------------------------------------------------------------
@@ -556,7 +636,7 @@ void solve_state(const Grid<T>& g, int nt, const T* const v[3], const T* mT, T* 
     }
     const T ht = T(1) / static_cast<T>(nt);
     std::vector<T> X(3 * g.nl);
-    trajectory_rk2(g, v, ht, +1, X.data());
+    trajectory(g, v, ht, +1, X.data());
     for (int j = 0; j < nt; ++j) interp(g, m + j * g.nl, X.data(), m + (j + 1) * g.nl);
 }
 
@@ -597,7 +677,7 @@ void solve_adjoint(const Grid<T>& g, int nt, const T* const v[3], const T* m, co
     } else {
         const T ht = T(1) / static_cast<T>(nt);
         std::vector<T> X(3 * nl), divv(nl), divvx(nl), lx(nl);
-        trajectory_rk2(g, v, ht, -1, X.data());
+        trajectory(g, v, ht, -1, X.data());
         div(g, v[0], v[1], v[2], divv.data());
         interp(g, divv.data(), X.data(), divvx.data());
         T scale = ht;
@@ -666,7 +746,7 @@ void solve_inc_state_x(const Grid<T>& g, int nt, const T* X, const T* m, const T
 template <class T>
 void solve_inc_state(const Grid<T>& g, int nt, const T* const v[3], const T* m, const T* const vt[3], T* mtilde) {
     std::vector<T> X(3 * g.nl);
-    trajectory_rk2(g, v, T(1) / static_cast<T>(nt), +1, X.data());
+    trajectory(g, v, T(1) / static_cast<T>(nt), +1, X.data());
     solve_inc_state_x(g, nt, X.data(), m, vt, mtilde);
 }
 
@@ -735,7 +815,7 @@ void solve_inc_adjoint(const Grid<T>& g, int nt, const T* const v[3], const T* m
     std::vector<T> X;
     if (!velocity_is_zero(g, v)) {
         X.resize(3 * g.nl);
-        trajectory_rk2(g, v, T(1) / static_cast<T>(nt), -1, X.data());
+        trajectory(g, v, T(1) / static_cast<T>(nt), -1, X.data());
     }
     solve_inc_adjoint_x(g, nt, v, X.data(), m, mtilde, ltilde, bt);
 }
@@ -766,8 +846,8 @@ void hessian_matvec(const Grid<T>& g, int nt, int norm, const T* beta, const T* 
                     const T* const vt[3], T* const hv[3], bool scale) {
     const T ht = T(1) / static_cast<T>(nt);
     std::vector<T> Xs(3 * g.nl), Xa(3 * g.nl);
-    trajectory_rk2(g, v, ht, +1, Xs.data());
-    trajectory_rk2(g, v, ht, -1, Xa.data());
+    trajectory(g, v, ht, +1, Xs.data());
+    trajectory(g, v, ht, -1, Xa.data());
     hessian_matvec_x(g, nt, norm, beta, v, Xs.data(), Xa.data(), m, vt, hv, scale);
 }
 
@@ -819,7 +899,7 @@ void det_def_grad(const Grid<T>& g, int nt, const T* const v[3], bool inverse, T
     const T alpha = inverse ? T(-1) : T(1);
     std::vector<T> X(3 * nl), divv(nl), divvx(nl), jx(nl);
     std::fill(jac, jac + nl, T(1));
-    trajectory_rk2(g, v, ht, +1, X.data());
+    trajectory(g, v, ht, +1, X.data());
     div(g, v[0], v[1], v[2], divv.data());
     interp(g, divv.data(), X.data(), divvx.data());
     for (int j = 0; j < nt; ++j) {
@@ -938,8 +1018,8 @@ void gn_solve(const Grid<T>& g, int nt, int norm, const T* beta, T* const v[3], 
 
         // Newton system H d = −g at the current iterate
         solve_state(g, nt, V.data(), mT, m.data());
-        trajectory_rk2(g, V.data(), ht, +1, Xs.data());
-        trajectory_rk2(g, V.data(), ht, -1, Xa.data());
+        trajectory(g, V.data(), ht, +1, Xs.data());
+        trajectory(g, V.data(), ht, -1, Xa.data());
         if (kg0 < 0.0) kg0 = gnorm;
         double rtol = o.krylov_rtol;
         if (o.forcing != 0 && kg0 > 0.0) {

@@ -38,6 +38,17 @@ int oracle_num_threads(void) {
 #endif
 }
 
+// Order of the characteristic scheme (2 or 4) used by every solve that
+// follows (RegOpt m_PDESolver.rkorder).
+int oracle_set_rk_order(int order) {
+    if (order != 2 && order != 4) {
+        g_err = "rk order must be 2 or 4";
+        return 1;
+    }
+    oracle::rk_order() = order;
+    return 0;
+}
+
 void oracle_set_num_threads(int n) {
 #ifdef _OPENMP
     if (n > 0) omp_set_num_threads(n);
@@ -59,8 +70,8 @@ void oracle_set_num_threads(int n) {
             T* h[3] = {h1, h2, h3};                                                                                 \
             const T ht = T(1) / static_cast<T>(nt);                                                                 \
             std::vector<T> Xs(3 * g.nl), Xa(3 * g.nl);                                                              \
-            oracle::trajectory_rk2(g, v, ht, +1, Xs.data());                                                        \
-            oracle::trajectory_rk2(g, v, ht, -1, Xa.data());                                                        \
+            oracle::trajectory(g, v, ht, +1, Xs.data());                                                            \
+            oracle::trajectory(g, v, ht, -1, Xa.data());                                                            \
             const auto t0 = std::chrono::steady_clock::now();                                                       \
             for (int r = 0; r < reps; ++r) oracle::hessian_matvec_x(g, nt, norm, b, v, Xs.data(), Xa.data(), m, w, h, true); \
             *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();               \
@@ -152,7 +163,7 @@ void oracle_set_num_threads(int n) {
         return guard([&] {                                                                                         \
             oracle::Grid<T> g(nx);                                                                                  \
             const T* v[3] = {v1, v2, v3};                                                                           \
-            oracle::trajectory_rk2(g, v, T(1) / static_cast<T>(nt), sigma, X);                                     \
+            oracle::trajectory(g, v, T(1) / static_cast<T>(nt), sigma, X);                                         \
         });                                                                                                        \
     }                                                                                                              \
     int oracle_solve_state_##SFX(const int* nx, int nt, const T* v1, const T* v2, const T* v3, const T* mT,       \

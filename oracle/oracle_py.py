@@ -50,6 +50,27 @@ def num_threads() -> int:
     return int(lib().oracle_num_threads())
 
 
+def set_rk_order(order: int) -> None:
+    """Characteristic scheme of every later solve: 2 (default) or 4."""
+    if lib().oracle_set_rk_order(int(order)) != 0:
+        raise ValueError(lib().oracle_last_error().decode())
+
+
+class rk_order:
+    """Context manager that runs a block with the given characteristic order."""
+
+    def __init__(self, order: int):
+        self.order = int(order)
+
+    def __enter__(self):
+        set_rk_order(self.order)
+        return self
+
+    def __exit__(self, *exc):
+        set_rk_order(2)
+        return False
+
+
 def _dt(prec):
     return np.float64 if prec == "f64" else np.float32
 

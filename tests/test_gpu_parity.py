@@ -270,3 +270,35 @@ def test_det_def_grad(shape, inverse, oracle):
     assert np.allclose(st2, st, rtol=1e-6, atol=0)
     assert (not reached) == (min(st[0], 1.0 / st[1]) > 0.25)
     assert reached_tight
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_rk4_characteristic(shape, oracle):
+    """-rkorder 4 (src/SemiLagrangian.cpp:297-459): both RK4 trajectories, the
+    state solve and the Hessian matvec on them against the oracle run with
+    the same order; switching back to order 2 recomputes the trajectory."""
+    print(f"\n[gpu case] rk4 {shape}", flush=True)
+    v, mT, mR, vt = _data(shape, seed=70)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA, rk_order=4)
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        c.set_reference_image(mR)
+        X = {flag: c.trajectory(flag) for flag in (STATE, ADJOINT)}
+        m = c.solve_state()
+        H = c.hessian_matvec(vt)
+        c.set_rk_order(2)
+        X2 = c.trajectory(STATE)
+    finally:
+        c.close()
+    with oracle.rk_order(4):
+        for flag, sigma in ((STATE, 1), (ADJOINT, -1)):
+            assert np.abs(X[flag] - oracle.trajectory(v, NT, sigma, prec="f32")).max() < 1e-6
+            assert np.abs(X[flag] - oracle.trajectory(v, NT, sigma)).max() < 2e-6
+        m32 = oracle.solve_state(v, mT, NT, prec="f32")
+        m64 = oracle.solve_state(v, mT, NT)
+        H32 = oracle.hessian_matvec(v, m32, vt, NT, "h1sn", BETA, prec="f32")
+        H64 = oracle.hessian_matvec(v, m64, vt, NT, "h1sn", BETA)
+    assert relerr(m, m32) < 1e-5 and relerr(m, m64) < 1e-5
+    assert_fp32_parity(H, H32, H64)
+    assert np.abs(X2 - oracle.trajectory(v, NT, +1, prec="f32")).max() < 1e-6

@@ -6,6 +6,8 @@ convergence, constant-velocity closed forms, the v = 0 branches of the
 reference (src/CLAIRE.cpp:1909-1920, 2249-2312, 3367-3421), linearity and
 the gradient Taylor test (src/OptimizationProblem.cpp:157-239).
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -116,6 +118,75 @@ def test_trajectory_constant_velocity(oracle, sigma):
     x = np.stack(_grid(shape), -1)
     expect = (x - sigma * c / nt) / (2 * np.pi)
     assert np.abs(X - expect).max() < 1e-14
+
+
+def _coupled_velocity(y1, y2):
+    """A smooth, periodic, nonlinear velocity independent of x3."""
+    return np.stack([1.5 * np.sin(y2) + 0.3 * np.cos(2 * y1), -1.2 * np.sin(y1) * np.cos(y2),
+                     0.5 * np.cos(y1 + y2)])
+
+
+@functools.lru_cache(maxsize=None)
+def _exact_foot_points(shape, ht, sigma, substeps=100):
+    """Foot point of dy/dt = −σ·v(y), y(0) = x, after time ht, by a fine RK4
+    on the analytic velocity; returned normalized by 2π like the trajectory."""
+    y = np.stack(_grid(shape)).astype(np.float64)
+    h = ht / substeps
+    f = lambda z: -sigma * _coupled_velocity(z[0], z[1])
+    for _ in range(substeps):
+        k1 = f(y)
+        k2 = f(y + 0.5 * h * k1)
+        k3 = f(y + 0.5 * h * k2)
+        k4 = f(y + h * k3)
+        y = y + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    return np.moveaxis(y, 0, -1) / (2 * np.pi)
+
+
+@pytest.mark.parametrize("order,rate", [(2, 8.0), (4, 32.0)])
+def test_trajectory_local_order(oracle, order, rate):
+    """One characteristic step has local error O(ht^3) for RK2 and O(ht^5)
+    for RK4 (src/SemiLagrangian.cpp:220-295 and :297-459): halving ht divides
+    the error by 8 and by 32.  A wrong stage weight or stage point in the RK4
+    restatement drops it to a lower order."""
+    shape = (128, 128, 8)
+    x1, x2, _ = _grid(shape)
+    v = _coupled_velocity(x1, x2)
+    for sigma in (1, -1):
+        errs = []
+        for nt in (1, 2, 4):
+            with oracle.rk_order(order):
+                X = oracle.trajectory(v, nt, sigma)
+            errs.append(np.abs(X - _exact_foot_points(shape, 1.0 / nt, sigma)).max())
+        ratios = [errs[i] / errs[i + 1] for i in range(len(errs) - 1)]
+        assert all(0.8 * rate < r < 1.3 * rate for r in ratios), (order, sigma, errs, ratios)
+
+
+@pytest.mark.parametrize("sigma", [1, -1])
+def test_rk4_trajectory_constant_velocity_and_state(oracle, sigma):
+    """RK4 is exact for a constant velocity, and whole-cell shifts transport
+    the template exactly, as with RK2."""
+    shape = (8, 16, 8)
+    nt = 4
+    c = np.array([0.3, -0.7, 1.1])
+    v = np.stack([np.full(shape, c[a]) for a in range(3)])
+    with oracle.rk_order(4):
+        X = oracle.trajectory(v, nt, sigma)
+    x = np.stack(_grid(shape), -1)
+    assert np.abs(X - (x - sigma * c / nt) / (2 * np.pi)).max() < 1e-14
+    shape = (16, 16, 16)
+    h = 2 * np.pi / 16
+    shift = np.array([1, 2, -1])
+    v = np.stack([np.full(shape, shift[a] * h * nt) for a in range(3)])
+    mT = synth.gaussian_blobs(shape, count=5, seed=3).astype(np.float64)
+    with oracle.rk_order(4):
+        m = oracle.solve_state(v, mT, nt)
+    for j in range(nt + 1):
+        assert np.abs(m[j] - np.roll(mT, shift=tuple(j * shift), axis=(0, 1, 2))).max() < 1e-12
+
+
+def test_rk_order_rejects_other_orders(oracle):
+    with pytest.raises(ValueError):
+        oracle.set_rk_order(3)
 
 
 def test_state_translation_by_whole_cells(oracle):
