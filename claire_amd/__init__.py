@@ -90,6 +90,24 @@ def nifti_write(path, a, spacing=None, header=None):
     _check(lib().clr_nifti_write(str(path).encode(), nx, a.ctypes.data_as(ctypes.c_void_p), sp, hd))
 
 
+def bin_read(path, shape=None):
+    """Reads a PETSc binary vector (.bin): float32 values in file order,
+    reshaped to `shape` (e.g. the grid's (n1, n2, n3)) when given."""
+    n = ctypes.c_int64()
+    _check(lib().clr_bin_read(str(path).encode(), ctypes.byref(n), None, ctypes.c_int64(0)))
+    out = np.empty(n.value, dtype=np.float32)
+    _check(lib().clr_bin_read(str(path).encode(), ctypes.byref(n), out.ctypes.data_as(ctypes.c_void_p),
+                              ctypes.c_int64(out.size)))
+    return out.reshape(shape) if shape is not None else out
+
+
+def bin_write(path, a):
+    """Writes an array as a PETSc binary vector (.bin) of float32 values in
+    the array's flat order."""
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+    _check(lib().clr_bin_write(str(path).encode(), ctypes.c_int64(a.size), a.ctypes.data_as(ctypes.c_void_p)))
+
+
 class GNOptions(ctypes.Structure):
     """clr_gn_options (include/claire_amd.h)."""
     _fields_ = [("maxit", ctypes.c_int), ("gttol", ctypes.c_double), ("gatol", ctypes.c_double),

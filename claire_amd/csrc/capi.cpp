@@ -8,6 +8,7 @@
 #include "../../include/claire_amd.h"
 #include "claire.hpp"
 #include "nifti.hpp"
+#include "petscbin.hpp"
 #include "solver.hpp"
 
 using claire_amd::Error;
@@ -254,6 +255,22 @@ int clr_nifti_write(const char* path, const int nx[3], const float* data, const 
         need(nx, "nx");
         need(data, "data");
         claire_amd::nifti::write(path, nx, data, spacing, header);
+    });
+}
+
+int clr_bin_read(const char* path, int64_t* n, float* data, int64_t capacity) {
+    return guard([&] {
+        need(path, "path");
+        need(n, "n");
+        claire_amd::petscbin::read(path, n, data, capacity);
+    });
+}
+
+int clr_bin_write(const char* path, int64_t n, const float* data) {
+    return guard([&] {
+        need(path, "path");
+        if (n > 0) need(data, "data");
+        claire_amd::petscbin::write(path, n, data);
     });
 }
 

@@ -89,6 +89,15 @@ int clr_check_bounds(clr_ctx* ctx, const float* v, int inverse, double bound, in
 int clr_nifti_read(const char* path, int nx[3], float* data, int64_t capacity, unsigned char* header);
 int clr_nifti_write(const char* path, const int nx[3], const float* data, const float* spacing,
                     const unsigned char* header);
+/* PETSc binary vectors (.bin), replacing ReadWriteReg::ReadBIN and WriteBIN
+ * (src/ReadWriteReg.cpp:1476-1537, VecLoad/VecView on a binary viewer).
+ * Host memory.  Big-endian: class id 1211214 and length as PetscInt, then the
+ * float values in the grid's flat order.  Written with 8-byte header words
+ * (the reference's --with-64-bit-indices PETSc, deps/build_libs.sh:176);
+ * read with 8- or 4-byte header words.  clr_bin_read: data (capacity floats)
+ * may be NULL to query the length n. */
+int clr_bin_read(const char* path, int64_t* n, float* data, int64_t capacity);
+int clr_bin_write(const char* path, int64_t n, const float* data);
 
 /* Arithmetic form of the Hessian matvec.  CLR_PRECISION_FP32 (default) is
  * the reference's single-precision path.  CLR_PRECISION_MIXED stores the
