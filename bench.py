@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=1)
     p.add_argument("--no-parity", action="store_true")
+    p.add_argument("--no-registration", action="store_true",
+                   help="skip the 64^3 Gauss-Newton-Krylov registration leg")
     p.add_argument("--fp64", action="store_true", help="also score GPU and CPU FP32 against the FP64 oracle")
     p.add_argument("--calib-copy", action="store_true",
                    help="end with three 256 MiB device copies (reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -61,6 +63,63 @@ def measured_traffic(kernel, n, nt):
             if table.get("nt") == nt and kernel in table.get("kernels", {}):
                 return float(table["kernels"][kernel]), os.path.relpath(path, ROOT)
     return None, None
+
+
+def registration_leg(args):
+    """The reference's CPU-runnable configuration: synthetic problem 4 at 64³
+    (template (sin²x1+sin²x2+sin²x3)/3, reference = template transported by
+    the benchmark velocity, rescaled to [0,1]), H1SN β = 1e-2, nt = 4,
+    Gauss–Newton–Krylov with PCG, INVREG, Eisenstat–Walker forcing and
+    Armijo, at most 10 Newton iterations.  GPU wall time of the whole solve
+    (Hessian matvecs replayed as HIP graphs, and eagerly), beside the FP32
+    CPU restatement of the same driver on the host cores."""
+    from claire_amd import Claire, synth
+
+    shape, nt, beta = (64, 64, 64), 4, (1e-2, 0.0, 0.0)
+    mT, vstar = synth.smooth_template_problem(shape)
+    c = Claire(shape, nt=nt, regnorm="h1sn", beta=beta)
+    res = {"workload": "GN-Krylov registration, 64^3 FP32, synthetic problem 4, H1SN beta=1e-2, nt=4, maxit=10"}
+    try:
+        c.set_velocity(vstar)
+        c.set_template_image(mT)
+        mR = synth.rescale(c.solve_state()[nt])
+        c.set_reference_image(mR)
+        c.solve_gn(maxit=1)  # first-use allocations and code-object loads
+        for graphs in (True, False):
+            c.set_matvec_graphs(graphs)
+            c.sync()
+            t0 = time.perf_counter()
+            v, rep, hist = c.solve_gn(maxit=10)
+            c.sync()
+            dt = time.perf_counter() - t0
+            key = "gpu_graphs" if graphs else "gpu_eager"
+            res[key] = {"seconds": round(dt, 4), "iterations": rep["iterations"], "matvecs": rep["matvecs"],
+                        "objective_evals": rep["objective_evals"], "reason": rep["reason"],
+                        "grad_reduction": rep["gnorm"] / rep["gnorm0"], "J": rep["J"]}
+            log(f"registration 64^3 ({key}): {dt:.3f}s, {rep['iterations']} its, {rep['matvecs']} matvecs")
+        c.set_velocity(v)
+        m1 = c.solve_state()[nt]
+        res["mismatch_gpu"] = float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT))
+    finally:
+        c.close()
+    if not args.no_cpu_baseline:
+        from oracle import oracle_py
+
+        threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
+        oracle_py.set_num_threads(threads)
+        t0 = time.perf_counter()
+        v32, rep32, _ = oracle_py.gn_solve(np.zeros((3,) + shape, np.float32), mT, mR, nt, "h1sn", beta, maxit=10,
+                                           prec="f32")
+        dt = time.perf_counter() - t0
+        m1 = oracle_py.solve_state(v32, mT, nt, prec="f32")[nt]
+        res["cpu_fp32"] = {"seconds": round(dt, 3), "cores": threads, "kind": "port",
+                           "iterations": rep32["iterations"], "matvecs": rep32["matvecs"],
+                           "reason": rep32["reason"], "grad_reduction": rep32["gnorm"] / rep32["gnorm0"],
+                           "J": rep32["J"],
+                           "mismatch": float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT))}
+        res["speedup_vs_cpu"] = round(dt / res["gpu_graphs"]["seconds"], 1)
+        log(f"registration 64^3 CPU FP32 port ({threads} threads): {dt:.2f}s")
+    return res
 
 
 def log(msg):
@@ -202,6 +261,8 @@ def main():
                 H64 = oracle_py.hessian_matvec(v, m64, v, nt, regnorm, beta, prec="f64")
                 out["relerr_vs_fp64"] = {"gpu_fp32": rel(Hgpu, H64), "cpu_fp32": rel(Hcpu, H64),
                                          "gpu_mixed": rel(Hmix, H64)}
+    if not args.no_registration:
+        out["registration"] = registration_leg(args)
     if args.calib_copy:
         big = DeviceArray((64 * 1024 * 1024,))
         big2 = DeviceArray((64 * 1024 * 1024,))

@@ -169,7 +169,7 @@ class Claire:
     """One registration problem (GN, semi-Lagrangian, nc = 1) on the GPU."""
 
     def __init__(self, nx, nt=4, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0), stream=None, precision="fp32",
-                 rk_order=2):
+                 rk_order=2, graphs=False):
         self.nx = tuple(int(n) for n in nx)
         self.nt = int(nt)
         self.nl = int(np.prod(self.nx))
@@ -184,6 +184,8 @@ class Claire:
             self.set_precision(precision)
         if rk_order != 2:
             self.set_rk_order(rk_order)
+        if graphs:
+            self.set_matvec_graphs(True)
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -306,6 +308,15 @@ class Claire:
         """"fp32" (the reference's path) or "mixed" (FP16 storage of the
         Newton-invariant gradient fields in the Hessian matvec)."""
         _check(lib().clr_set_precision(self.ctx, ctypes.c_int({"fp32": 0, "mixed": 1}[precision])))
+
+    def set_matvec_graphs(self, on):
+        """Replay Hessian matvecs as captured HIP graphs (same results)."""
+        _check(lib().clr_set_matvec_graphs(self.ctx, ctypes.c_int(int(bool(on)))))
+
+    def matvec_graph_count(self):
+        n = ctypes.c_int()
+        _check(lib().clr_matvec_graph_count(self.ctx, ctypes.byref(n)))
+        return n.value
 
     def set_rk_order(self, order):
         """Characteristic scheme of every later trajectory: 2 or 4 (the

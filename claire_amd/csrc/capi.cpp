@@ -299,6 +299,17 @@ int clr_set_precision(clr_ctx* ctx, int precision) {
     return guard([&] { C(ctx).SetPrecision(precision); });
 }
 
+int clr_set_matvec_graphs(clr_ctx* ctx, int on) {
+    return guard([&] { C(ctx).SetMatVecGraphs(on != 0); });
+}
+
+int clr_matvec_graph_count(clr_ctx* ctx, int* count) {
+    return guard([&] {
+        need(count, "count");
+        *count = C(ctx).MatVecGraphCount();
+    });
+}
+
 int clr_set_rk_order(clr_ctx* ctx, int order) {
     return guard([&] { C(ctx).SetRKOrder(order); });
 }

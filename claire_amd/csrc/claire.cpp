@@ -1,6 +1,8 @@
 // claire.cpp -- host side of the MI355X registration hot path.
 #include "claire.hpp"
 
+#include <algorithm>
+
 #include <cstring>
 
 namespace claire_amd {
@@ -530,7 +532,22 @@ CLAIRE::CLAIRE(const RegOpt& opt) : m_Opt(opt) {
     m_VtX.resize(3 * nl);
 }
 
-CLAIRE::~CLAIRE() = default;
+CLAIRE::~CLAIRE() {
+    DropGraphs();
+    if (m_CaptureStream) (void)hipStreamDestroy(m_CaptureStream);
+}
+
+void CLAIRE::DropGraphs() {
+    for (auto& g : m_Graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    m_Graphs.clear();
+}
+
+int CLAIRE::SetMatVecGraphs(bool on) {
+    m_Opt.graphs = on ? 1 : 0;
+    if (!on) DropGraphs();
+    return 0;
+}
 
 size_t CLAIRE::DeviceBytes() const {
     const size_t nl = static_cast<size_t>(m_Opt.domain.nl), nt = static_cast<size_t>(m_Opt.nt);
@@ -547,6 +564,7 @@ int CLAIRE::SetControlVariable(const float* v) {
     m_VelocityField->SetComponents(v);
     m_HaveVelocity = true;
     m_TrajValid = m_StateValid = m_GradMValid = m_GradMXValid = m_HalfValid = false;
+    ++m_GraphEpoch;
     // CLAIREBase::IsVelocityZero (src/CLAIREBase.cpp:578-605): ∞-norm == 0
     m_VelocityIsZero = m_Red->AbsMax(m_VelocityField->m_X1, 3 * m_Opt.domain.nl) == 0.0;
     return 0;
@@ -591,6 +609,7 @@ void CLAIRE::EnsureTrajectories() {
                              m_Opt.GetTimeStepSize(), 1.0f, m_AdjCoeff.get(), m_Opt.stream);
     }
     m_TrajValid = true;
+    ++m_GraphEpoch;
 }
 
 int CLAIRE::SolveStateEquation() {
@@ -688,6 +707,7 @@ int CLAIRE::ComputeDetDefGrad(bool inverse, float* jac, double stats[3]) {
 
 int CLAIRE::SetPrecision(int precision) {
     CLR_REQUIRE(precision == 0 || precision == 1, kErrInvalidArgument, "precision must be 0 (FP32) or 1 (mixed)");
+    if (precision != m_Opt.precision) ++m_GraphEpoch;
     m_Opt.precision = precision;
     return 0;
 }
@@ -818,17 +838,76 @@ int CLAIRE::HessianMatVec(float* Hvtilde, const float* vtilde, bool scale) {
     }
     CLR_REQUIRE(Hvtilde != nullptr, kErrInvalidArgument, "null output vector");
     CLR_REQUIRE(Hvtilde != vt->m_X1, kErrInvalidArgument, "HessianMatVec cannot run in place");
+    // Newton-invariant caches, computed eagerly (never inside a capture)
+    EnsureGradientsAtStateTrajectory();
+    if (m_Opt.precision == 1) EnsureHalfGradients();
+    if (!m_Opt.graphs || (m_Opt.prof && m_Opt.prof->enabled())) {
+        HessianMatVecBody(Hvtilde, vt, scale);
+        return 0;
+    }
+    MatVecGraph* entry = nullptr;
+    m_Graphs.erase(std::remove_if(m_Graphs.begin(), m_Graphs.end(),
+                                  [&](MatVecGraph& g) {
+                                      if (g.epoch == m_GraphEpoch) return false;
+                                      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+                                      return true;
+                                  }),
+                   m_Graphs.end());
+    for (auto& g : m_Graphs)
+        if (g.in == vt->m_X1 && g.out == Hvtilde && g.scale == scale) entry = &g;
+    if (!entry) {
+        if (m_Graphs.size() >= 4) {
+            if (m_Graphs.front().exec) (void)hipGraphExecDestroy(m_Graphs.front().exec);
+            m_Graphs.erase(m_Graphs.begin());
+        }
+        m_Graphs.push_back(MatVecGraph{vt->m_X1, Hvtilde, scale, m_GraphEpoch, 0, nullptr});
+        entry = &m_Graphs.back();
+    }
+    if (entry->uses++ < 2) {
+        HessianMatVecBody(Hvtilde, vt, scale);
+        return 0;
+    }
+    if (!entry->exec) CaptureMatVec(Hvtilde, vt, scale, &entry->exec);
+    CLR_HIP_CHECK(hipGraphLaunch(entry->exec, m_Opt.stream));
+    return 0;
+}
+
+void CLAIRE::CaptureMatVec(float* Hvtilde, const VecField* vt, bool scale, hipGraphExec_t* exec) {
+    if (!m_CaptureStream) CLR_HIP_CHECK(hipStreamCreateWithFlags(&m_CaptureStream, hipStreamNonBlocking));
+    const hipStream_t saved = m_Opt.stream;
+    Profiler* const saved_prof = m_Opt.prof;
+    m_Opt.stream = m_CaptureStream;
+    m_Opt.prof = nullptr;
+    hipGraph_t graph = nullptr;
+    CLR_HIP_CHECK(hipStreamBeginCapture(m_CaptureStream, hipStreamCaptureModeThreadLocal));
+    try {
+        HessianMatVecBody(Hvtilde, vt, scale);
+    } catch (...) {
+        m_Opt.stream = saved;
+        m_Opt.prof = saved_prof;
+        (void)hipStreamEndCapture(m_CaptureStream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    m_Opt.stream = saved;
+    m_Opt.prof = saved_prof;
+    CLR_HIP_CHECK(hipStreamEndCapture(m_CaptureStream, &graph));
+    const hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) *exec = nullptr;
+    CLR_HIP_CHECK(e);
+}
+
+void CLAIRE::HessianMatVecBody(float* Hvtilde, const VecField* vt, bool scale) {
     VecField hv(&m_Opt, Hvtilde);
     // \tilde{m} (src/CLAIRE.cpp:1345) with λ̃⁰ = −m̃^nt fused into the last step
     const bool half = m_Opt.precision == 1;
-    if (half) EnsureHalfGradients();
     IncStateCore(vt, m_Lam.get(), half);
     // \tilde{λ} and \tilde{b}·hd (src/CLAIRE.cpp:1348, 3307-3491)
     IncAdjointCore(m_Lam.get(), m_IncBodyForce.get(), half);
     // Hṽ = β·A[ṽ] + b̃ (src/CLAIRE.cpp:1356-1361); 1/hd when !scale (src/CLAIRE.cpp:1283-1286)
     const float a = scale ? 1.0f : 1.0f / m_Opt.GetLebesgueMeasure();
     m_Regularization->ApplyFused(&hv, vt, a, m_IncBodyForce.get(), a);
-    return 0;
 }
 
 int CLAIRE::SolveAdjointEquation() {

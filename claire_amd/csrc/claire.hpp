@@ -124,6 +124,8 @@ struct RegOpt {
     // streaming kernels are stored as IEEE half (FP32 arithmetic); objective,
     // gradient and reductions are unchanged.
     int precision = 0;
+    // Replay the Hessian matvec as a captured HIP graph (CLAIRE::HessianMatVec).
+    int graphs = 0;
     // Order of the characteristic scheme, 2 or 4 (m_PDESolver.rkorder,
     // src/RegOpt.cpp:674-676, default 2 at src/RegOpt.cpp:1087).
     int rkorder = 2;
@@ -307,6 +309,9 @@ class CLAIRE {
     // Characteristic scheme of every later trajectory: 2 or 4
     // (SemiLagrangian::ComputeTrajectory, src/SemiLagrangian.cpp:172-178).
     int SetRKOrder(int order);
+    // Capture the Hessian matvec into HIP graphs and replay them (on/off).
+    int SetMatVecGraphs(bool on);
+    int MatVecGraphCount() const { return static_cast<int>(m_Graphs.size()); }
     // det(∇y) (inverse = false) or det(∇y⁻¹) by SL transport
     // (DeformationFields::ComputeDetDefGrad); jac may be null; stats receive
     // min, max and mean.
@@ -321,6 +326,29 @@ class CLAIRE {
     void IncStateCore(const VecField* vt, float* lt0_out, bool half);
     void IncAdjointCore(const float* lt0, VecField* bt, bool half);
     void EnsureHalfGradients();
+    void HessianMatVecBody(float* Hvtilde, const VecField* vt, bool scale);
+    void CaptureMatVec(float* Hvtilde, const VecField* vt, bool scale, hipGraphExec_t* exec);
+    void DropGraphs();
+
+    // Captured Hessian matvecs, one per (input, output, scale) triple.  The
+    // kernels, their launch shapes and every buffer address are fixed at
+    // capture; what changes them (new trajectories with their displacement
+    // bound, the velocity-is-zero branch, the precision) bumps m_GraphEpoch,
+    // and graphs of an older epoch are dropped.  A triple runs eagerly on its
+    // first two uses in an epoch (lazy allocations happen there, and a
+    // Newton step whose Krylov solve needs only one or two matvecs pays no
+    // capture), is captured on its third and replayed from then on.
+    struct MatVecGraph {
+        const float* in;
+        float* out;
+        bool scale;
+        uint64_t epoch;
+        int uses;
+        hipGraphExec_t exec;
+    };
+    std::vector<MatVecGraph> m_Graphs;
+    uint64_t m_GraphEpoch = 1;
+    hipStream_t m_CaptureStream = nullptr;
 
     RegOpt m_Opt;
     std::unique_ptr<Reducer> m_Red;

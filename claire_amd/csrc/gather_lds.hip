@@ -377,13 +377,13 @@ __global__ __launch_bounds__(256) void k_to_half(const float* __restrict__ in, _
 // of resident blocks (LDS allows 3 blocks per CU for D <= 2, 1 for D <= 4),
 // weighed against the per-block prologue of WIN planes.
 int num_cus() {
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
+    // one device per process; the first call (any thread) queries it
+    static const int cus = [] {
+        int dev = 0, n = 0;
         CLR_HIP_CHECK(hipGetDevice(&dev));
-        CLR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        if (cus <= 0) cus = 1;
-    }
+        CLR_HIP_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+        return n > 0 ? n : 1;
+    }();
     return cus;
 }
 

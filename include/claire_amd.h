@@ -114,6 +114,16 @@ int clr_set_precision(clr_ctx* ctx, int precision);
  * SemiLagrangian::ComputeTrajectory (src/SemiLagrangian.cpp:172-178).
  * Changing it drops the cached trajectories and everything computed on them. */
 int clr_set_rk_order(clr_ctx* ctx, int order);
+/* Launch mode of clr_hessian_matvec, with no counterpart in the reference
+ * (whose matvec is a sequence of host calls behind MatShell MATOP_MULT,
+ * src/Optimizer.cpp:434-437).  on = 1: each (input, output, scale) triple
+ * runs eagerly on its first two uses after the velocity, trajectory or
+ * precision changed, is captured into a HIP graph on its third use and is
+ * replayed from then on (at most 4 triples are kept); on = 0 (default):
+ * always eager.  Results are the same either way.  clr_matvec_graph_count
+ * reports the graphs currently held. */
+int clr_set_matvec_graphs(clr_ctx* ctx, int on);
+int clr_matvec_graph_count(clr_ctx* ctx, int* count);
 /* Device memory held by the context (bytes). */
 int clr_device_bytes(clr_ctx* ctx, size_t* bytes);
 

@@ -302,3 +302,43 @@ def test_rk4_characteristic(shape, oracle):
     assert relerr(m, m32) < 1e-5 and relerr(m, m64) < 1e-5
     assert_fp32_parity(H, H32, H64)
     assert np.abs(X2 - oracle.trajectory(v, NT, +1, prec="f32")).max() < 1e-6
+
+
+def test_hessian_matvec_graph_replay():
+    """clr_set_matvec_graphs: replays of the captured matvec equal the eager
+    matvec bit for bit, on every replay, with eager work and host
+    synchronizations between replays, for two buffer pairs, both scalings and
+    after a new velocity (which invalidates the captured graphs)."""
+    from claire_amd import DeviceArray
+
+    shape = (32, 16, 64)
+    print(f"\n[gpu case] matvec graphs {shape}", flush=True)
+    v, mT, mR, vt = _data(shape, seed=80)
+    v2 = synth.bandlimited_velocity(shape, kmax=4, cfl=1.5, nt=NT, seed=81)
+    vt2 = synth.bandlimited_velocity(shape, kmax=4, cfl=1.0, nt=NT, seed=82)
+    eager = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA)
+    graph = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA, graphs=True)
+    vec = (3,) + shape
+    ins = [DeviceArray(vec), DeviceArray(vec)]
+    outs = [DeviceArray(vec), DeviceArray(vec)]
+    try:
+        ins[0].upload(vt)
+        ins[1].upload(vt2)
+        for c in (eager, graph):
+            c.set_template_image(mT)
+            c.set_reference_image(mR)
+        for vel in (v, v2):
+            for c in (eager, graph):
+                c.set_velocity(vel)
+            want = {(k, s): eager.hessian_matvec((vt, vt2)[k], scale=s) for k in (0, 1) for s in (True, False)}
+            for rep in range(4):
+                for (k, s), w in want.items():
+                    graph.hessian_matvec_device(outs[k], ins[k], scale=s)
+                    assert np.array_equal(outs[k].download(), w), (rep, k, s)
+                    eager.solve_state()  # eager work between replays
+            assert graph.matvec_graph_count() == 4
+    finally:
+        for d in ins + outs:
+            d.free()
+        eager.close()
+        graph.close()
