@@ -44,6 +44,9 @@ def parse():
     p.add_argument("--no-parity", action="store_true")
     p.add_argument("--no-registration", action="store_true",
                    help="skip the 64^3 Gauss-Newton-Krylov registration leg")
+    p.add_argument("--no-batched", action="store_true",
+                   help="skip the 48-registration batched leg")
+    p.add_argument("--batch-workers", type=int, default=4)
     p.add_argument("--fp64", action="store_true", help="also score GPU and CPU FP32 against the FP64 oracle")
     p.add_argument("--calib-copy", action="store_true",
                    help="end with three 256 MiB device copies (reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -85,7 +88,7 @@ def registration_leg(args):
         mR = synth.rescale(c.solve_state()[nt])
         c.set_reference_image(mR)
         c.solve_gn(maxit=1)  # first-use allocations and code-object loads
-        for graphs in (True, False):
+        for graphs in (False, True, False, True):  # best of two solves per launch mode
             c.set_matvec_graphs(graphs)
             c.sync()
             t0 = time.perf_counter()
@@ -93,10 +96,13 @@ def registration_leg(args):
             c.sync()
             dt = time.perf_counter() - t0
             key = "gpu_graphs" if graphs else "gpu_eager"
+            if key in res and res[key]["seconds"] <= dt:
+                continue
             res[key] = {"seconds": round(dt, 4), "iterations": rep["iterations"], "matvecs": rep["matvecs"],
                         "objective_evals": rep["objective_evals"], "reason": rep["reason"],
                         "grad_reduction": rep["gnorm"] / rep["gnorm0"], "J": rep["J"]}
-            log(f"registration 64^3 ({key}): {dt:.3f}s, {rep['iterations']} its, {rep['matvecs']} matvecs")
+            log(f"registration 64^3 ({key}): {dt:.4f}s, {rep['iterations']} its, {rep['matvecs']} matvecs")
+        res["seconds"] = min(res["gpu_graphs"]["seconds"], res["gpu_eager"]["seconds"])
         c.set_velocity(v)
         m1 = c.solve_state()[nt]
         res["mismatch_gpu"] = float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT))
@@ -117,8 +123,103 @@ def registration_leg(args):
                            "reason": rep32["reason"], "grad_reduction": rep32["gnorm"] / rep32["gnorm0"],
                            "J": rep32["J"],
                            "mismatch": float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT))}
-        res["speedup_vs_cpu"] = round(dt / res["gpu_graphs"]["seconds"], 1)
+        res["speedup_vs_cpu"] = round(dt / res["seconds"], 1)
         log(f"registration 64^3 CPU FP32 port ({threads} threads): {dt:.2f}s")
+    return res
+
+
+def guarded(leg, args):
+    """Runs a secondary leg; a failure is reported in its place instead of
+    losing the headline line."""
+    try:
+        return leg(args)
+    except Exception as e:  # noqa: BLE001 -- reported, not hidden
+        log(f"{leg.__name__} failed: {e!r}")
+        return {"error": repr(e)}
+
+
+def batched_leg(args):
+    """48 independent registrations: 32×64³, 8×96³, 8×128³.  Each pair is a
+    30-ellipsoid phantom (radii 4–20 voxels, intensity U[0,1], σ = 1 voxel
+    smoothing, normalized to [0,1]) and its warp by a band-limited velocity
+    (|k| <= 8, CFL 2, nt = 4); H1 regularization β = (1e-2, 1e-4), nt = 4,
+    Gauss–Newton–Krylov with the default stopping rules.  All 48 solves run
+    through clr_solve_gn_batch (worker threads with one HIP stream each,
+    matvecs replayed from captured graphs); the same solves one after another
+    on one stream are timed beside it and must give identical velocities."""
+    from claire_amd import Claire, solve_gn_batch, synth
+
+    sizes = [64] * 32 + [96] * 8 + [128] * 8
+    nt, beta = 4, (1e-2, 1e-4, 0.0)
+    t0 = time.time()
+    problems, refs = [], []
+    for i, n in enumerate(sizes):
+        shape = (n, n, n)
+        mT = synth.ellipsoid_phantom(shape, count=30, seed=1000 + i)
+        v = synth.bandlimited_velocity(shape, kmax=8, cfl=2.0, nt=nt, seed=2000 + i)
+        c = Claire(shape, nt=nt, regnorm="h1", beta=beta, graphs=True)
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        mR = c.solve_state()[nt]
+        c.set_reference_image(mR)
+        problems.append(c)
+        refs.append((mT, mR))
+    log(f"batched: {len(sizes)} pairs set up in {time.time() - t0:.1f}s")
+    res = {"workload": "48 GN-Krylov registrations (32x64^3, 8x96^3, 8x128^3), ellipsoid phantoms warped by "
+                       "band-limited v |k|<=8 CFL 2, H1 beta=1e-2, nt=4",
+           "points": int(sum(n ** 3 for n in sizes))}
+    try:
+        solve_gn_batch(problems, workers=args.batch_workers, maxit=1)  # first-use allocations
+        for c in problems:
+            c.sync()
+        t0 = time.perf_counter()
+        got = solve_gn_batch(problems, workers=args.batch_workers)
+        dt_batch = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        seq = solve_gn_batch(problems, workers=1)
+        dt_seq = time.perf_counter() - t0
+        same = all(np.array_equal(a[0], b[0]) and a[1] == b[1] for a, b in zip(got, seq))
+        mism = []
+        for c, (v, rep), (mT, mR) in zip(problems, got, refs):
+            c.set_velocity(v)
+            m1 = c.solve_state()[nt]
+            mism.append(float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT)))
+        its = [rep["iterations"] for _, rep in got]
+        reasons = sorted({rep["reason"] for _, rep in got})
+        res.update({"workers": args.batch_workers, "seconds": round(dt_batch, 3),
+                    "registrations_per_s": round(len(sizes) / dt_batch, 2),
+                    "sequential_seconds": round(dt_seq, 3),
+                    "speedup_vs_sequential": round(dt_seq / dt_batch, 2),
+                    "batched_equals_sequential": bool(same),
+                    "newton_iterations": {"min": min(its), "max": max(its), "total": int(sum(its))},
+                    "reasons": reasons,
+                    "mismatch": {"min": min(mism), "median": float(np.median(mism)), "max": max(mism)}})
+        log(f"batched: {len(sizes)} registrations in {dt_batch:.2f}s ({len(sizes) / dt_batch:.1f}/s), "
+            f"sequential {dt_seq:.2f}s, identical={same}")
+        if not args.no_cpu_baseline:
+            from oracle import oracle_py
+
+            threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
+            oracle_py.set_num_threads(threads)
+            mT, mR = refs[0]
+            t0 = time.perf_counter()
+            v32, rep32, _ = oracle_py.gn_solve(np.zeros((3,) + mT.shape, np.float32), mT, mR, nt, "h1", beta,
+                                               prec="f32")
+            dt = time.perf_counter() - t0
+            vg, repg = got[0][0].astype(np.float64), got[0][1]
+            m1 = oracle_py.solve_state(v32, mT, nt, prec="f32")[nt]
+            res["cpu_fp32_sample"] = {
+                "pair": 0, "grid": list(mT.shape), "seconds": round(dt, 3), "cores": threads, "kind": "port",
+                "iterations": rep32["iterations"], "gpu_iterations": repg["iterations"],
+                "pcg_iterations": rep32["pcg_iterations"], "gpu_pcg_iterations": repg["pcg_iterations"],
+                "mismatch": float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT)), "gpu_mismatch": mism[0],
+                "grad_reduction": rep32["gnorm"] / rep32["gnorm0"],
+                "gpu_grad_reduction": repg["gnorm"] / repg["gnorm0"],
+                "velocity_relerr_gpu_vs_cpu": float(np.linalg.norm(vg - v32) / np.linalg.norm(v32))}
+            log(f"batched: CPU FP32 port on pair 0 (64^3): {dt:.2f}s")
+    finally:
+        for c in problems:
+            c.close()
     return res
 
 
@@ -262,7 +363,9 @@ def main():
                 out["relerr_vs_fp64"] = {"gpu_fp32": rel(Hgpu, H64), "cpu_fp32": rel(Hcpu, H64),
                                          "gpu_mixed": rel(Hmix, H64)}
     if not args.no_registration:
-        out["registration"] = registration_leg(args)
+        out["registration"] = guarded(registration_leg, args)
+    if not args.no_batched:
+        out["batched"] = guarded(batched_leg, args)
     if args.calib_copy:
         big = DeviceArray((64 * 1024 * 1024,))
         big2 = DeviceArray((64 * 1024 * 1024,))

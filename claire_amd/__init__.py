@@ -125,6 +125,37 @@ class GNReport(ctypes.Structure):
 GN_REASONS = {0: "continue", 1: "gttol", 2: "gatol", 3: "steptol", -1: "maxits", -2: "nan", -3: "linesearch"}
 
 
+def solve_gn_batch(problems, v0s=None, workers=4, **options):
+    """Gauss–Newton–Krylov registrations of many contexts at once
+    (clr_solve_gn_batch): `workers` host threads, one HIP stream each.
+    Returns one (v, report dict) per problem; raises on the first failure
+    after every solve was attempted."""
+    n = len(problems)
+    opt = GNOptions()
+    _check(lib().clr_gn_options_default(ctypes.byref(opt)))
+    for k, val in options.items():
+        if not hasattr(opt, k):
+            raise TypeError(f"unknown option {k}")
+        setattr(opt, k, val)
+    vs = []
+    for i, c in enumerate(problems):
+        d = c.buf("gn_v", (3,) + c.nx)
+        d.upload(np.zeros((3,) + c.nx, np.float32) if v0s is None else v0s[i])
+        vs.append(d)
+    ctxs = (ctypes.c_void_p * n)(*[c.ctx.value for c in problems])
+    vp = (ctypes.c_void_p * n)(*[d.ptr for d in vs])
+    reps = (GNReport * n)()
+    status = (ctypes.c_int * n)()
+    _check(lib().clr_solve_gn_batch(ctxs, vp, ctypes.c_int(n), ctypes.byref(opt), reps, status,
+                                    ctypes.c_int(int(workers))))
+    out = []
+    for i in range(n):
+        r = {name: getattr(reps[i], name) for name, _ in GNReport._fields_}
+        r["reason"] = GN_REASONS.get(r["reason"], r["reason"])
+        out.append((vs[i].download(), r))
+    return out
+
+
 class DeviceArray:
     """Device allocation owned by Python (clr_device_malloc/free)."""
 

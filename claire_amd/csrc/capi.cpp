@@ -1,9 +1,13 @@
 // capi.cpp -- the extern "C" boundary (include/claire_amd.h).
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <new>
+#include <set>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/claire_amd.h"
 #include "claire.hpp"
@@ -16,6 +20,7 @@ using claire_amd::Error;
 struct clr_ctx {
     std::unique_ptr<reg::Profiler> prof;
     std::unique_ptr<reg::CLAIRE> claire;
+    std::unique_ptr<reg::Optimizer> optimizer;  // work vectors kept across solves
     std::unique_ptr<claire_amd::FFTPlan3D> plan;  // for the raw FFT entry points
     reg::DeviceArray<claire_amd::c32> spec;
 };
@@ -54,6 +59,29 @@ const reg::RegOpt& O(clr_ctx* ctx) { return C(ctx).Opt(); }
 void check_hip_launch() {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error(claire_amd::kErrHip, std::string("kernel launch failed: ") + hipGetErrorString(e));
+}
+
+reg::GNOptions gn_options(const clr_gn_options& opt) {
+    reg::GNOptions o;
+    o.maxit = opt.maxit;
+    o.gttol = opt.gttol;
+    o.gatol = opt.gatol;
+    o.krylov_maxit = opt.krylov_maxit;
+    o.krylov_rtol = opt.krylov_rtol;
+    o.forcing = opt.forcing;
+    o.precond = opt.precond;
+    o.linesearch = opt.linesearch;
+    return o;
+}
+
+void run_gn(clr_ctx* ctx, float* v, const reg::GNOptions& o, clr_gn_report* report, double* history,
+            int history_len) {
+    if (!ctx->optimizer) ctx->optimizer.reset(new reg::Optimizer(&C(ctx)));
+    reg::GNReport r;
+    ctx->optimizer->Run(v, o, r, history, history_len);
+    check_hip_launch();
+    *report = clr_gn_report{r.iterations, r.reason, r.matvecs, r.pcg_iterations, r.objective_evals,
+                            r.gradient_evals, r.J0, r.D0, r.gnorm0, r.J, r.D, r.R, r.gnorm};
 }
 }  // namespace
 
@@ -330,21 +358,73 @@ int clr_solve_gn(clr_ctx* ctx, float* v, const clr_gn_options* opt, clr_gn_repor
         need(opt, "opt");
         need(report, "report");
         CLR_REQUIRE(history_len >= 0, claire_amd::kErrInvalidArgument, "negative history length");
-        reg::GNOptions o;
-        o.maxit = opt->maxit;
-        o.gttol = opt->gttol;
-        o.gatol = opt->gatol;
-        o.krylov_maxit = opt->krylov_maxit;
-        o.krylov_rtol = opt->krylov_rtol;
-        o.forcing = opt->forcing;
-        o.precond = opt->precond;
-        o.linesearch = opt->linesearch;
-        reg::Optimizer optimizer(&C(ctx));
-        reg::GNReport r;
-        optimizer.Run(v, o, r, history, history ? history_len : 0);
-        check_hip_launch();
-        *report = clr_gn_report{r.iterations, r.reason, r.matvecs, r.pcg_iterations, r.objective_evals,
-                                r.gradient_evals, r.J0, r.D0, r.gnorm0, r.J, r.D, r.R, r.gnorm};
+        run_gn(ctx, v, gn_options(*opt), report, history, history ? history_len : 0);
+    });
+}
+
+int clr_solve_gn_batch(clr_ctx* const* ctxs, float* const* v, int count, const clr_gn_options* opt,
+                       clr_gn_report* reports, int* status, int workers) {
+    return guard([&] {
+        need(opt, "opt");
+        CLR_REQUIRE(count >= 0, claire_amd::kErrInvalidArgument, "negative batch size");
+        CLR_REQUIRE(workers >= 1, claire_amd::kErrInvalidArgument, "workers must be >= 1");
+        if (count == 0) return;
+        need(ctxs, "ctxs");
+        need(v, "v");
+        need(reports, "reports");
+        std::set<const clr_ctx*> seen;
+        for (int i = 0; i < count; ++i) {
+            need(ctxs[i], "context");
+            need(ctxs[i]->claire.get(), "context");
+            need(v[i], "v");
+            CLR_REQUIRE(seen.insert(ctxs[i]).second, claire_amd::kErrInvalidArgument,
+                        "a context appears twice in the batch");
+        }
+        const reg::GNOptions o = gn_options(*opt);
+        int dev = 0;
+        CLR_HIP_CHECK(hipGetDevice(&dev));
+        std::atomic<int> next{0};
+        std::vector<int> rc(count, CLR_OK);
+        std::vector<std::string> err(count);
+        auto worker = [&]() {
+            hipStream_t st = nullptr;
+            const bool ready = hipSetDevice(dev) == hipSuccess &&
+                               hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+            for (int i = next.fetch_add(1); i < count; i = next.fetch_add(1)) {
+                if (!ready) {
+                    rc[i] = claire_amd::kErrHip;
+                    err[i] = "could not create a stream for the batch worker";
+                    continue;
+                }
+                reg::CLAIRE& c = *ctxs[i]->claire;
+                const hipStream_t own = c.Opt().stream;
+                try {
+                    CLR_HIP_CHECK(hipStreamSynchronize(own));  // work queued on the context's stream first
+                    c.SetStream(st);
+                    run_gn(ctxs[i], v[i], o, &reports[i], nullptr, 0);
+                    CLR_HIP_CHECK(hipStreamSynchronize(st));
+                } catch (const Error& e) {
+                    rc[i] = e.code;
+                    err[i] = e.what();
+                } catch (const std::exception& e) {
+                    rc[i] = claire_amd::kErrInvalidArgument;
+                    err[i] = e.what();
+                }
+                (void)hipStreamSynchronize(st);
+                c.SetStream(own);
+            }
+            if (st) (void)hipStreamDestroy(st);
+        };
+        std::vector<std::thread> pool;
+        const int nthreads = std::min(workers, count);
+        for (int w = 0; w < nthreads; ++w) pool.emplace_back(worker);
+        for (auto& t : pool) t.join();
+        int first = -1;
+        for (int i = 0; i < count; ++i) {
+            if (status) status[i] = rc[i];
+            if (rc[i] != CLR_OK && first < 0) first = i;
+        }
+        if (first >= 0) throw Error(rc[first], "batch entry " + std::to_string(first) + ": " + err[first]);
     });
 }
 

@@ -309,6 +309,8 @@ class CLAIRE {
     // Characteristic scheme of every later trajectory: 2 or 4
     // (SemiLagrangian::ComputeTrajectory, src/SemiLagrangian.cpp:172-178).
     int SetRKOrder(int order);
+    // Stream of every later launch (work already queued stays where it is).
+    void SetStream(hipStream_t s) { m_Opt.stream = s; }
     // Capture the Hessian matvec into HIP graphs and replay them (on/off).
     int SetMatVecGraphs(bool on);
     int MatVecGraphCount() const { return static_cast<int>(m_Graphs.size()); }

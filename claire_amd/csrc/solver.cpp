@@ -9,7 +9,7 @@ namespace reg {
 using namespace claire_amd;
 
 Optimizer::Optimizer(CLAIRE* problem)
-    : m_Prob(problem), m_N(3 * problem->Opt().domain.nl), m_Stream(problem->Opt().stream) {
+    : m_Prob(problem), m_N(3 * problem->Opt().domain.nl) {
     const size_t n = static_cast<size_t>(m_N);
     m_G.resize(n);
     m_D.resize(n);
@@ -34,9 +34,9 @@ int Optimizer::Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& 
     float* z = m_Z.get();
     float* p = m_P.get();
     float* w = m_W.get();
-    launch_fill(d, 0.0f, m_N, m_Stream);
-    launch_copy(g, r, m_N, m_Stream);
-    launch_scale(r, -1.0f, m_N, m_Stream);
+    launch_fill(d, 0.0f, m_N, Stream());
+    launch_copy(g, r, m_N, Stream());
+    launch_scale(r, -1.0f, m_N, Stream());
     const double bnorm = std::sqrt(Dot(r, r));
     const double tol = std::max(rtol * bnorm, 1e-16);
     const int kmax = std::max(0, opt.krylov_maxit - 1);
@@ -45,21 +45,21 @@ int Optimizer::Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& 
         if (opt.precond)
             m_Prob->ApplyInvRegularizationOperator(out, in, false);
         else
-            launch_copy(in, out, m_N, m_Stream);
+            launch_copy(in, out, m_N, Stream());
     };
     precond(r, z);
     double rz = Dot(r, z);
     int its = 0;
     for (int k = 1; k <= kmax; ++k) {
-        if (k == 1) launch_copy(z, p, m_N, m_Stream);
+        if (k == 1) launch_copy(z, p, m_N, Stream());
         m_Prob->HessianMatVec(w, p, true);
         rep.matvecs++;
         its = k;
         const double pw = Dot(p, w);
         if (!(pw > 0.0)) break;
         const double a = rz / pw;
-        launch_axpy(static_cast<float>(a), p, d, m_N, m_Stream);
-        launch_axpy(static_cast<float>(-a), w, r, m_N, m_Stream);
+        launch_axpy(static_cast<float>(a), p, d, m_N, Stream());
+        launch_axpy(static_cast<float>(-a), w, r, m_N, Stream());
         const double rnorm = std::sqrt(Dot(r, r));
         if (rnorm <= tol || !(rnorm < 1e6 * bnorm)) break;
         precond(r, z);
@@ -67,7 +67,7 @@ int Optimizer::Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& 
         const double beta = rz_new / rz;
         rz = rz_new;
         // p = z + β·p
-        launch_waxpy(static_cast<float>(beta), p, z, p, m_N, m_Stream);
+        launch_waxpy(static_cast<float>(beta), p, z, p, m_N, Stream());
     }
     return its;
 }
@@ -82,7 +82,7 @@ int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, 
 
     // ‖g₀‖ at v = 0 (InitializeOptimization, src/CLAIRE.cpp:320-335)
     m_Zero.resize(static_cast<size_t>(m_N));
-    launch_fill(m_Zero.get(), 0.0f, m_N, m_Stream);
+    launch_fill(m_Zero.get(), 0.0f, m_N, Stream());
     m_Prob->EvaluateGradient(g, m_Zero.get());
     rep.gradient_evals++;
     rep.gnorm0 = std::sqrt(Dot(g, g));
@@ -127,8 +127,8 @@ int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, 
         double gd = Dot(g, d);
         bool newton = true;
         if (!(gd < 0.0)) {
-            launch_copy(g, d, m_N, m_Stream);
-            launch_scale(d, -1.0f, m_N, m_Stream);
+            launch_copy(g, d, m_N, Stream());
+            launch_scale(d, -1.0f, m_N, Stream());
             gd = -gnorm * gnorm;
             newton = false;
         }
@@ -138,7 +138,7 @@ int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, 
             alpha = 1.0;
             const int trials = opt.linesearch ? 30 : 1;
             for (int k = 0; k < trials; ++k) {
-                launch_waxpy(static_cast<float>(alpha), d, v, vt, m_N, m_Stream);
+                launch_waxpy(static_cast<float>(alpha), d, v, vt, m_N, Stream());
                 m_Prob->EvaluateObjective(&Jt, &Dt, &Rt, vt);
                 rep.objective_evals++;
                 if (!opt.linesearch || (std::isfinite(Jt) && Jt <= J + 1e-4 * alpha * gd)) {
@@ -148,8 +148,8 @@ int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, 
                 alpha *= 0.5;
             }
             if (ok || !newton) break;
-            launch_copy(g, d, m_N, m_Stream);
-            launch_scale(d, -1.0f, m_N, m_Stream);
+            launch_copy(g, d, m_N, Stream());
+            launch_scale(d, -1.0f, m_N, Stream());
             gd = -gnorm * gnorm;
             newton = false;
         }
@@ -158,7 +158,7 @@ int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, 
             m_Prob->SetControlVariable(v);
             break;
         }
-        launch_copy(vt, v, m_N, m_Stream);
+        launch_copy(vt, v, m_N, Stream());
         J = Jt; D = Dt; R = Rt;
         m_Prob->EvaluateGradient(g, nullptr);  // at the accepted trial velocity
         rep.gradient_evals++;
@@ -169,7 +169,7 @@ int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, 
     }
     rep.iterations = iter;
     rep.J = J; rep.D = D; rep.R = R; rep.gnorm = gnorm;
-    CLR_HIP_CHECK(hipStreamSynchronize(m_Stream));
+    CLR_HIP_CHECK(hipStreamSynchronize(Stream()));
     return 0;
 }
 

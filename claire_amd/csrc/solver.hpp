@@ -46,7 +46,9 @@ class Optimizer {
     int Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& rep);
     CLAIRE* m_Prob;
     IntType m_N;
-    hipStream_t m_Stream;
+    // the problem's stream at the time of the call (a batch driver may move
+    // a problem to another stream between solves)
+    hipStream_t Stream() const { return m_Prob->Opt().stream; }
     DeviceArray<float> m_G, m_D, m_R, m_Z, m_P, m_W, m_Vt, m_Zero;
 };
 

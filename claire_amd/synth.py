@@ -126,3 +126,68 @@ def smooth_template_problem(nx, dtype=np.float32):
                   np.broadcast_to(np.sin(x1) * np.cos(x3) * np.sin(x3), shape),
                   np.broadcast_to(np.sin(x2) * np.cos(x1) * np.sin(x1), shape)])
     return rescale(mT.astype(dtype)), v.astype(dtype)
+
+
+def gaussian_smoothing(x, sigma):
+    """Preprocessing::GaussianSmoothing (src/Preprocessing.cpp:1956-2050):
+    multiply the spectrum by exp(−½·Σ_i k_i²·(σ_i·h_i)²), σ in grid points,
+    wavenumbers k > n/2 shifted to k − n (the Nyquist index keeps +n/2)."""
+    x = np.asarray(x, dtype=np.float64)
+    nx = x.shape
+    sig = np.broadcast_to(np.asarray(sigma, dtype=np.float64), (3,))
+    c = [(sig[a] * TWO_PI / nx[a]) ** 2 for a in range(3)]
+    k = []
+    for a, n in enumerate(nx):
+        kk = np.arange(n // 2 + 1 if a == 2 else n, dtype=np.float64)
+        if a < 2:
+            kk = np.where(kk > n // 2, kk - n, kk)
+        shape = [1, 1, 1]
+        shape[a] = kk.size
+        k.append(kk.reshape(shape))
+    mult = np.exp(-0.5 * (k[0] ** 2 * c[0] + k[1] ** 2 * c[1] + k[2] ** 2 * c[2]))
+    return np.fft.irfftn(np.fft.rfftn(x) * mult, s=nx, axes=(0, 1, 2))
+
+
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def ellipsoid_phantom(nx, count=30, seed=0, radii=(4.0, 20.0), sigma=1.0, dtype=np.float32):
+    """Synthetic phantom: `count` solid ellipsoids painted in order (later ones
+    overwrite earlier ones) on the periodic grid, each with centre uniform
+    over the grid, semi-axes U[radii] voxels, a uniformly random rotation and
+    intensity U[0,1]; then Gaussian smoothing with `sigma` voxels
+    (gaussian_smoothing) and min-max normalization to [0, 1]."""
+    rng = np.random.default_rng(seed)
+    nx = tuple(int(n) for n in nx)
+    f = np.zeros(nx, dtype=np.float64)
+    for _ in range(count):
+        c = rng.uniform(0.0, 1.0, 3) * np.asarray(nx)
+        r = rng.uniform(radii[0], radii[1], 3)
+        R = _rotation(rng)
+        val = rng.uniform(0.0, 1.0)
+        w = int(np.ceil(r.max())) + 1
+        ci = np.floor(c).astype(int)
+        off, disp = [], []
+        for a in range(3):
+            if 2 * w + 1 >= nx[a]:  # box wider than the grid: every index, nearest periodic image
+                o = np.arange(nx[a])
+                disp.append(np.mod(o - c[a] + 0.5 * nx[a], nx[a]) - 0.5 * nx[a])
+            else:
+                o = np.arange(ci[a] - w, ci[a] + w + 1)
+                disp.append(o - c[a])
+            off.append(o)
+        d = np.stack(np.meshgrid(*disp, indexing="ij"), -1)
+        q = d @ R  # coordinates in the ellipsoid's frame
+        inside = ((q / r) ** 2).sum(-1) <= 1.0
+        idx = np.ix_(*[np.mod(off[a], nx[a]) for a in range(3)])
+        blk = f[idx]
+        blk[inside] = val
+        f[idx] = blk
+    g = gaussian_smoothing(f, sigma)
+    lo, hi = g.min(), g.max()
+    return ((g - lo) / (hi - lo) if hi > lo else g - lo).astype(dtype)

@@ -217,6 +217,19 @@ typedef struct clr_gn_report {
 int clr_gn_options_default(clr_gn_options* opt);
 int clr_solve_gn(clr_ctx* ctx, float* v, const clr_gn_options* opt, clr_gn_report* report, double* history,
                  int history_len);
+/* Many independent registrations at once (no counterpart in the reference,
+ * which runs one registration per MPI job): the Gauss–Newton–Krylov solve of
+ * clr_solve_gn on each of `count` distinct contexts, spread over `workers`
+ * host threads, each with its own non-blocking HIP stream, so that the
+ * solves of small grids overlap on the device.  v[i] (device, 3·nl of
+ * context i) is the initial guess on entry and the solution on exit;
+ * reports[i] receives context i's report and status[i] (may be NULL) its
+ * error code.  Every solve is attempted; the return value is the first
+ * failing entry's code.  Each context runs on its worker's stream during the
+ * call and is back on its own stream afterwards; work already queued on its
+ * own stream is waited for first.  Results equal those of clr_solve_gn. */
+int clr_solve_gn_batch(clr_ctx* const* ctxs, float* const* v, int count, const clr_gn_options* opt,
+                       clr_gn_report* reports, int* status, int workers);
 
 /* Regularization::EvaluateGradient == HessianMatVec (3·nl → 3·nl). */
 int clr_reg_apply(clr_ctx* ctx, const float* v, float* out);

@@ -34,3 +34,24 @@ def test_null_arguments_are_rejected_without_a_gpu():
     lib = claire_amd.lib()
     assert lib.clr_create(None, None, 4, 0, None, None) == 1
     assert b"null" in lib.clr_last_error()
+
+
+def test_batch_arguments_are_checked_without_a_gpu():
+    lib = claire_amd.lib()
+    opt = claire_amd.GNOptions()
+    assert lib.clr_gn_options_default(ctypes.byref(opt)) == 0
+    # an empty batch does nothing
+    assert lib.clr_solve_gn_batch(None, None, 0, ctypes.byref(opt), None, None, 1) == 0
+    assert lib.clr_solve_gn_batch(None, None, 1, None, None, None, 1) == 1
+    assert lib.clr_solve_gn_batch(None, None, -1, ctypes.byref(opt), None, None, 1) == 1
+    assert lib.clr_solve_gn_batch(None, None, 0, ctypes.byref(opt), None, None, 0) == 1
+    assert b"workers" in lib.clr_last_error()
+    ctxs = (ctypes.c_void_p * 2)(None, None)
+    assert lib.clr_solve_gn_batch(ctxs, ctxs, 2, ctypes.byref(opt), (claire_amd.GNReport * 2)(), None, 2) == 1
+    assert b"null" in lib.clr_last_error()
+
+
+def test_rk_order_and_graph_setters_reject_a_null_context():
+    lib = claire_amd.lib()
+    assert lib.clr_set_rk_order(None, 4) == 1
+    assert lib.clr_set_matvec_graphs(None, 1) == 1

@@ -27,13 +27,14 @@ def _mismatch(m1, mT, mR):
     return float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT))
 
 
-@pytest.mark.parametrize("n,maxit,gttol", [(32, 10, 1e-12), (64, 50, 5e-2)])
-def test_gn_registration_matches_oracle(n, maxit, gttol, oracle):
+@pytest.mark.parametrize("n,maxit,gttol,regnorm", [(32, 10, 1e-12, "h1sn"), (64, 50, 5e-2, "h1sn"), (32, 50, 5e-2, "h1"),
+                                                     (32, 50, 5e-2, "h2sn")])
+def test_gn_registration_matches_oracle(n, maxit, gttol, regnorm, oracle):
     shape = (n, n, n)
-    print(f"\n[gpu case] GN registration {shape} maxit={maxit} gttol={gttol}", flush=True)
+    print(f"\n[gpu case] GN registration {shape} {regnorm} maxit={maxit} gttol={gttol}", flush=True)
     mT, mR = _problem(shape, oracle)
-    v64, rep64, hist64 = oracle.gn_solve(np.zeros((3,) + shape), mT, mR, NT, "h1sn", BETA, maxit=maxit, gttol=gttol)
-    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA)
+    v64, rep64, hist64 = oracle.gn_solve(np.zeros((3,) + shape), mT, mR, NT, regnorm, BETA, maxit=maxit, gttol=gttol)
+    c = Claire(shape, nt=NT, regnorm=regnorm, beta=BETA)
     try:
         c.set_template_image(mT.astype(np.float32))
         c.set_reference_image(mR.astype(np.float32))
@@ -53,3 +54,53 @@ def test_gn_registration_matches_oracle(n, maxit, gttol, oracle):
     assert abs(mis - mis64) <= 1e-3 * mis64
     assert abs(red - red64) <= 1e-3 * red64 or abs(red - red64) <= 1e-3 * 5e-2
     assert abs(rep["J"] - rep64["J"]) <= 1e-3 * rep64["J"]
+
+
+def _golden():
+    import importlib.util
+    import json
+    import os
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    spec = importlib.util.spec_from_file_location("make_registration_golden",
+                                                  os.path.join(here, "make_registration_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    with open(os.path.join(here, "registration_phantom.json")) as f:
+        return mod, json.load(f)
+
+
+_GOLDEN_MOD, _GOLDEN = _golden()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+@pytest.mark.parametrize("name", sorted(_GOLDEN))
+def test_phantom_registration_matches_fp64_golden(name, precision):
+    """Ellipsoid-phantom registrations against the FP64 oracle's stored end
+    results (tests/golden/make_registration_golden.py): 128³ with H2SN
+    β = 1e-3, and 64³ with RegularizationH2 as written, whose line search
+    fails after one Newton step in the oracle as well.  Plain FP32 and the
+    mixed form: stop reason, final mismatch, ‖g‖/‖g₀‖ and J at 3 digits."""
+    case, ref = _GOLDEN[name]["case"], _GOLDEN[name]["oracle_fp64"]
+    print(f"\n[gpu case] phantom registration {name} {precision}", flush=True)
+    mT, mR = _GOLDEN_MOD.inputs(case)
+    shape = mT.shape
+    c = Claire(shape, nt=case["nt"], regnorm=case["regnorm"], beta=(case["beta"], 0.0, 0.0), precision=precision)
+    try:
+        c.set_template_image(mT.astype(np.float32))
+        c.set_reference_image(mR.astype(np.float32))
+        v, rep, hist = c.solve_gn(maxit=case["maxit"], gttol=case["gttol"])
+        c.set_velocity(v)
+        m1 = c.solve_state()[case["nt"]]
+    finally:
+        c.close()
+    mis = _mismatch(m1, mT, mR)
+    red, red64 = rep["gnorm"] / rep["gnorm0"], ref["gnorm"] / ref["gnorm0"]
+    print(f"gpu {precision}: its {rep['iterations']} pcg {rep['pcg_iterations']} mismatch {mis:.6e} "
+          f"|g|/|g0| {red:.6e} J {rep['J']:.6e} ({rep['reason']})")
+    print(f"fp64 oracle: its {ref['iterations']} pcg {ref['pcg_iterations']} mismatch {ref['mismatch']:.6e} "
+          f"|g|/|g0| {red64:.6e} J {ref['J']:.6e} ({ref['reason']})")
+    assert rep["reason"] == ref["reason"]
+    assert abs(mis - ref["mismatch"]) <= 1e-3 * ref["mismatch"]
+    assert abs(red - red64) <= 1e-3 * red64 or abs(red - red64) <= 1e-3 * case["gttol"]
+    assert abs(rep["J"] - ref["J"]) <= 1e-3 * ref["J"]

@@ -377,3 +377,25 @@ def test_det_def_grad_converges_to_flow_determinant(oracle, inverse):
     assert errs[2] < 2e-2 and errs[1] < errs[0] / 3 and errs[2] < errs[1] / 3
     J0, st = oracle.det_def_grad(np.zeros((3,) + shape), 4, inverse)
     assert np.abs(J0 - 1.0).max() < 1e-12 and np.allclose(st, 1.0, rtol=0, atol=1e-12)
+
+
+def test_registration_golden_reproduces(oracle):
+    """The stored FP64 registration results (tests/golden) come from the
+    current oracle and inputs: the quick 64³ case is recomputed here."""
+    import importlib.util
+    import json
+    import os
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(here, "make_registration_golden.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    with open(os.path.join(here, "registration_phantom.json")) as f:
+        golden = json.load(f)
+    case, ref = golden["phantom64_h2"]["case"], golden["phantom64_h2"]["oracle_fp64"]
+    mT, mR = mk.inputs(case)
+    v, rep, _ = oracle.gn_solve(np.zeros((3,) + mT.shape), mT, mR, case["nt"], case["regnorm"],
+                                (case["beta"], 0.0, 0.0), maxit=case["maxit"], gttol=case["gttol"])
+    assert rep["reason"] == ref["reason"] and rep["iterations"] == ref["iterations"]
+    for k in ("J", "D", "R", "gnorm", "gnorm0"):
+        assert abs(rep[k] - ref[k]) <= 1e-9 * abs(ref[k]), k
