@@ -46,7 +46,7 @@ def parse():
                    help="skip the 64^3 Gauss-Newton-Krylov registration leg")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the 48-registration batched leg")
-    p.add_argument("--batch-workers", type=int, default=4)
+    p.add_argument("--batch-workers", type=int, default=8)
     p.add_argument("--fp64", action="store_true", help="also score GPU and CPU FP32 against the FP64 oracle")
     p.add_argument("--calib-copy", action="store_true",
                    help="end with three 256 MiB device copies (reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")

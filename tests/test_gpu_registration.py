@@ -27,13 +27,27 @@ def _mismatch(m1, mT, mR):
     return float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT))
 
 
+def _close(a, a64, a32, rel=1e-3, floor=0.0):
+    """3 digits against FP64, or no further from FP64 than twice the
+    reference's FP32 path is (where FP32 round-off changes the Krylov
+    iteration counts of the reference itself)."""
+    return abs(a - a64) <= max(rel * abs(a64), floor, 2.0 * abs(a32 - a64))
+
+
 @pytest.mark.parametrize("n,maxit,gttol,regnorm", [(32, 10, 1e-12, "h1sn"), (64, 50, 5e-2, "h1sn"), (32, 50, 5e-2, "h1"),
                                                      (32, 50, 5e-2, "h2sn")])
 def test_gn_registration_matches_oracle(n, maxit, gttol, regnorm, oracle):
+    """GPU solve against the FP64 oracle, with the oracle's FP32 twin (the
+    reference's single-precision path) as the yardstick where FP32 round-off
+    alone moves the result: with H1 and β1 = 1e-4 the inverse-regularization
+    preconditioner amplifies the mean mode by 1/(β0·β1) = 1e6 and the FP32
+    Krylov solves of the reference path stop later than the FP64 ones."""
     shape = (n, n, n)
     print(f"\n[gpu case] GN registration {shape} {regnorm} maxit={maxit} gttol={gttol}", flush=True)
     mT, mR = _problem(shape, oracle)
     v64, rep64, hist64 = oracle.gn_solve(np.zeros((3,) + shape), mT, mR, NT, regnorm, BETA, maxit=maxit, gttol=gttol)
+    v32, rep32, hist32 = oracle.gn_solve(np.zeros((3,) + shape, np.float32), mT, mR, NT, regnorm, BETA, maxit=maxit,
+                                         gttol=gttol, prec="f32")
     c = Claire(shape, nt=NT, regnorm=regnorm, beta=BETA)
     try:
         c.set_template_image(mT.astype(np.float32))
@@ -44,16 +58,17 @@ def test_gn_registration_matches_oracle(n, maxit, gttol, regnorm, oracle):
     finally:
         c.close()
     m1_64 = oracle.solve_state(v64, mT, NT)[NT]
-    mis, mis64 = _mismatch(m1, mT, mR), _mismatch(m1_64, mT, mR)
+    m1_32 = oracle.solve_state(v32.astype(np.float64), mT, NT)[NT]
+    mis, mis64, mis32 = _mismatch(m1, mT, mR), _mismatch(m1_64, mT, mR), _mismatch(m1_32, mT, mR)
     red, red64 = rep["gnorm"] / rep["gnorm0"], rep64["gnorm"] / rep64["gnorm0"]
-    print(f"gpu: its {rep['iterations']} pcg {rep['pcg_iterations']} mismatch {mis:.6e} |g|/|g0| {red:.6e} "
-          f"({rep['reason']})")
-    print(f"fp64 oracle: its {rep64['iterations']} pcg {rep64['pcg_iterations']} mismatch {mis64:.6e} "
-          f"|g|/|g0| {red64:.6e} ({rep64['reason']})")
+    red32 = rep32["gnorm"] / rep32["gnorm0"]
+    for tag, r, m, g in (("gpu", rep, mis, red), ("fp32 oracle", rep32, mis32, red32), ("fp64 oracle", rep64, mis64, red64)):
+        print(f"{tag}: its {r['iterations']} pcg {r['pcg_iterations']} mismatch {m:.6e} |g|/|g0| {g:.6e} "
+              f"J {r['J']:.6e} ({r['reason']})")
     assert rep["reason"] == rep64["reason"]
-    assert abs(mis - mis64) <= 1e-3 * mis64
-    assert abs(red - red64) <= 1e-3 * red64 or abs(red - red64) <= 1e-3 * 5e-2
-    assert abs(rep["J"] - rep64["J"]) <= 1e-3 * rep64["J"]
+    assert _close(mis, mis64, mis32)
+    assert _close(red, red64, red32, floor=1e-3 * gttol)
+    assert _close(rep["J"], rep64["J"], rep32["J"])
 
 
 def _golden():
