@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--no-batched", action="store_true",
                    help="skip the 48-registration batched leg")
     p.add_argument("--batch-workers", type=int, default=8)
+    p.add_argument("--large", action="store_true",
+                   help="add the 512^3 leg (40-ellipsoid phantom, nt=8): footprint, matvec/s, 5 GN iterations")
     p.add_argument("--fp64", action="store_true", help="also score GPU and CPU FP32 against the FP64 oracle")
     p.add_argument("--calib-copy", action="store_true",
                    help="end with three 256 MiB device copies (reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -223,6 +225,72 @@ def batched_leg(args):
     return res
 
 
+def large_leg(args):
+    """512³ single registration: a 40-ellipsoid phantom (radii 8–40 voxels,
+    σ = 1 voxel smoothing) warped by a band-limited velocity (|k| <= 16,
+    CFL 3, nt = 8).  Reports the device footprint, Hessian matvecs per second
+    with RegularizationH2 β = 1e-3 (plain and mixed), and five Gauss–Newton
+    iterations with H2SN β = 1e-3 (RegularizationH2 as written stalls its
+    line search after one step: its gradient omits the hd of its
+    functional)."""
+    from claire_amd import Claire, DeviceArray, synth
+
+    n, nt = 512, 8
+    shape = (n, n, n)
+    t0 = time.time()
+    mT = synth.ellipsoid_phantom(shape, count=40, seed=7, radii=(8.0, 40.0))
+    log(f"large: phantom in {time.time() - t0:.1f}s")
+    v = synth.bandlimited_velocity(shape, kmax=16, cfl=3.0, nt=nt, seed=8)
+    log(f"large: inputs {shape} in {time.time() - t0:.1f}s")
+    res = {"workload": "512^3, 40-ellipsoid phantom warped by band-limited v |k|<=16 CFL 3, nt=8"}
+    c = Claire(shape, nt=nt, regnorm="h2", beta=(1e-3, 0.0, 0.0))
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        mR = c.solve_state()[nt]
+        c.set_reference_image(mR)
+        c.prepare_hessian()
+        vt = DeviceArray.from_host(v)
+        hv = DeviceArray((3,) + shape)
+        for prec in ("fp32", "mixed"):
+            c.set_precision(prec)
+            c.prepare_hessian()
+            c.hessian_matvec_device(hv, vt)
+            c.sync()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                c.hessian_matvec_device(hv, vt)
+            c.sync()
+            dt = (time.perf_counter() - t0) / 5
+            res[f"matvec_{prec}"] = {"regnorm": "h2", "beta": 1e-3, "matvec_per_s": round(1.0 / dt, 2),
+                                     "ms_per_matvec": round(1e3 * dt, 2)}
+            log(f"large: {prec} matvec {1e3 * dt:.1f} ms")
+        res["device_bytes_problem"] = c.device_bytes()
+        vt.free()
+        hv.free()
+    finally:
+        c.close()
+    c = Claire(shape, nt=nt, regnorm="h2sn", beta=(1e-3, 0.0, 0.0))
+    try:
+        c.set_template_image(mT)
+        c.set_reference_image(mR)
+        t0 = time.perf_counter()
+        vsol, rep, hist = c.solve_gn(maxit=6)
+        c.sync()
+        dt = time.perf_counter() - t0
+        c.set_velocity(vsol)
+        m1 = c.solve_state()[nt]
+        res["gn"] = {"regnorm": "h2sn", "beta": 1e-3, "seconds": round(dt, 2), "iterations": rep["iterations"],
+                     "matvecs": rep["matvecs"], "reason": rep["reason"],
+                     "grad_reduction": rep["gnorm"] / rep["gnorm0"],
+                     "mismatch": float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT)),
+                     "device_bytes_problem_and_solver": c.device_bytes() + 8 * 3 * 4 * n ** 3}
+        log(f"large: GN {rep['iterations']} its in {dt:.1f}s ({rep['reason']})")
+    finally:
+        c.close()
+    return res
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -366,6 +434,8 @@ def main():
         out["registration"] = guarded(registration_leg, args)
     if not args.no_batched:
         out["batched"] = guarded(batched_leg, args)
+    if args.large:
+        out["large"] = guarded(large_leg, args)
     if args.calib_copy:
         big = DeviceArray((64 * 1024 * 1024,))
         big2 = DeviceArray((64 * 1024 * 1024,))
