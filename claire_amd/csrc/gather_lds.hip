@@ -73,15 +73,22 @@ constexpr int kBufferWord3 = 0x00020000;
 __device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, static_cast<int>(bytes), kBufferWord3);
 }
+// Cache policy of the buffer loads: 0 for the field planes (neighbouring
+// tiles re-read their halo rows from L2), kNonTemporal for the plan words and
+// the per-point coefficients, which are read once.
+constexpr int kNonTemporal = 2;
 __device__ __forceinline__ float bload(rsrc_t r, uint32_t voff, uint32_t soff) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ float bload_once(rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, kNonTemporal));
 }
 __device__ __forceinline__ void bstore(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
 }
 __device__ __forceinline__ PlanPt bload_plan(rsrc_t r, uint32_t voff, uint32_t soff) {
     typedef unsigned int u3 __attribute__((ext_vector_type(3)));
-    const u3 w = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+    const u3 w = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, kNonTemporal);
     PlanPt p;
     p.w[0] = w.x;
     p.w[1] = w.y;
@@ -176,7 +183,7 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
 #pragma unroll
         for (int k = 0; k < G::PTS; ++k) {
             qa[k] = bload_plan(rp, 3u * off[k], 3u * pb);
-            aa[k] = Epi::kHasAux ? bload(ra, off[k], pb) : 0.0f;
+            aa[k] = Epi::kHasAux ? bload_once(ra, off[k], pb) : 0.0f;
         }
     }
     __syncthreads();
@@ -200,7 +207,7 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
 #pragma unroll
             for (int k = 0; k < G::PTS; ++k) {
                 nxt[k] = bload_plan(rp, 3u * off[k], 3u * pb);
-                naux[k] = Epi::kHasAux ? bload(ra, off[k], pb) : 0.0f;
+                naux[k] = Epi::kHasAux ? bload_once(ra, off[k], pb) : 0.0f;
             }
         }
 
@@ -279,9 +286,20 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
 // writes m̃¹ = −s_0 (λ̃⁰ = s_0 when nt = 1) and s_1..s_{nt−1}.
 // 4 consecutive values as float4: FP32 directly, FP16 converted.
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ float4 ld4(const float* a, int64_t i) { return reinterpret_cast<const float4*>(a)[i]; }
+// 16-byte (FP32) or 8-byte (FP16) streaming loads with the non-temporal
+// hint: every element of these arrays is read once per launch, and the hint
+// keeps the streams from displacing reusable lines (measured on a read-8,
+// write-1 stream at 256³: 5.84 TB/s with the hint, 5.19 TB/s without).
+using f4v = float __attribute__((ext_vector_type(4)));
+using u2v = unsigned __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 ld4(const float* a, int64_t i) {
+    const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(a) + i);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ float4 ld4(const _Float16* a, int64_t i) {
-    const h4 h = reinterpret_cast<const h4*>(a)[i];
+    const u2v bits = __builtin_nontemporal_load(reinterpret_cast<const u2v*>(a) + i);
+    h4 h;
+    __builtin_memcpy(&h, &bits, sizeof(h));
     return make_float4(static_cast<float>(h.x), static_cast<float>(h.y), static_cast<float>(h.z),
                        static_cast<float>(h.w));
 }

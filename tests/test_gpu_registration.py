@@ -34,14 +34,19 @@ def _close(a, a64, a32, rel=1e-3, floor=0.0):
     return abs(a - a64) <= max(rel * abs(a64), floor, 2.0 * abs(a32 - a64))
 
 
-@pytest.mark.parametrize("n,maxit,gttol,regnorm", [(32, 10, 1e-12, "h1sn"), (64, 50, 5e-2, "h1sn"), (32, 50, 5e-2, "h1"),
+@pytest.mark.parametrize("n,maxit,gttol,regnorm", [(32, 5, 1e-12, "h1sn"), (64, 50, 5e-2, "h1sn"), (32, 50, 5e-2, "h1"),
                                                      (32, 50, 5e-2, "h2sn")])
 def test_gn_registration_matches_oracle(n, maxit, gttol, regnorm, oracle):
     """GPU solve against the FP64 oracle, with the oracle's FP32 twin (the
     reference's single-precision path) as the yardstick where FP32 round-off
     alone moves the result: with H1 and β1 = 1e-4 the inverse-regularization
     preconditioner amplifies the mean mode by 1/(β0·β1) = 1e6 and the FP32
-    Krylov solves of the reference path stop later than the FP64 ones."""
+    Krylov solves of the reference path stop later than the FP64 ones.
+
+    The deep case (gttol 1e-12) stops at maxit = 5, before the noise floor:
+    run further, the FP64 solve itself ends on either the step-length test or
+    a failed line search, depending on the host's libm (1e-16 differences in
+    J decide it)."""
     shape = (n, n, n)
     print(f"\n[gpu case] GN registration {shape} {regnorm} maxit={maxit} gttol={gttol}", flush=True)
     mT, mR = _problem(shape, oracle)
