@@ -155,3 +155,27 @@ def test_reference_image_size_128x150x128(oracle):
     print(f"Hessian relerr vs FP64: gpu {e_gpu:.3e}  reference FP32 path {e_ref:.3e}  gpu vs ref {relerr(H, H32):.3e}")
     # FP32 conditioning at 2.5M points: both FP32 paths sit ~2e-5 from FP64
     assert e_gpu <= max(1e-5, 2.0 * e_ref)
+
+
+def test_maximum_size_512():
+    """The largest supported grid, 512³ (packed plan offsets use all 9 bits,
+    the LDS gathers' 32-bit byte offsets reach 1.5 GiB): FFT round trip, and
+    the LDS-staged gather along a trajectory equal to the direct gather on
+    the same foot points."""
+    from claire_amd import STATE
+
+    shape = (512, 512, 512)
+    print(f"\n[gpu case] maximum size {shape}", flush=True)
+    c = Claire(shape, nt=1, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0))
+    try:
+        f = synth.gaussian_blobs(shape, count=4, seed=5)
+        back = c.fft_c2r(c.fft_r2c(f)) / np.float32(f.size)
+        assert relerr(back, f) < 3e-6
+        v = synth.bandlimited_velocity(shape, kmax=8, cfl=2.0, nt=1, seed=6)
+        c.set_velocity(v)
+        X = c.trajectory(STATE)
+        a = c.interp_scalar(f, STATE)
+        b = c.interp_points(f, X)
+        assert relerr(a, b) < 1e-6
+    finally:
+        c.close()
