@@ -86,7 +86,19 @@ void Profiler::reset() {
 // ---------------------------------------------------------------------------
 // Reducer (FP64)
 // ---------------------------------------------------------------------------
-Reducer::Reducer(const RegOpt* opt) : m_Opt(opt), m_Partial(kReduceBlocks), m_Result(1) {}
+Reducer::Reducer(const RegOpt* opt) : m_Opt(opt), m_Partial(kReduceBlocks), m_Result(kSlots) {}
+
+void Reducer::DotTo(const float* a, const float* b, IntType n, int slot) {
+    CLR_REQUIRE(slot >= 0 && slot < kSlots, kErrInvalidArgument, "reduction slot out of range");
+    KernelScope k(m_Opt->prof, "reduce_dot", 8.0 * n);
+    launch_dot(a, b, n, m_Partial.get(), m_Result.get() + slot, m_Opt->stream);
+}
+
+void Reducer::Fetch(double* out, int nslots) {
+    CLR_REQUIRE(nslots >= 1 && nslots <= kSlots, kErrInvalidArgument, "reduction slot count out of range");
+    CLR_HIP_CHECK(hipMemcpyAsync(out, m_Result.get(), sizeof(double) * nslots, hipMemcpyDeviceToHost, m_Opt->stream));
+    CLR_HIP_CHECK(hipStreamSynchronize(m_Opt->stream));
+}
 
 double Reducer::fetch() {
     double h = 0.0;
@@ -235,20 +247,16 @@ void SpectralOps::Grad(float* const g[3], const float* f) {
     { KernelScope k(m_Opt->prof, "fft_x3_r2c", bytes_x3_r2c(d)); m_Plan->x3_r2c(f, s[0], st); }
     { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[0], -1, st); }
     { KernelScope k(m_Opt->prof, "fft_x1_grad", 32.0 * d.nlc); m_Plan->x1_grad(s[0], s, st); }
-    for (int c = 0; c < 3; ++c) {
-        { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[c], +1, st); }
-        { KernelScope k(m_Opt->prof, "fft_x3_c2r", bytes_x3_c2r(d, false)); m_Plan->x3_c2r(s[c], g[c], st); }
-    }
+    { KernelScope k(m_Opt->prof, "fft_x2", 3.0 * bytes_col(d)); m_Plan->x2_n(s, 3, +1, st); }
+    { KernelScope k(m_Opt->prof, "fft_x3_c2r", 3.0 * bytes_x3_c2r(d, false)); m_Plan->x3_c2r_n(s, g, 3, st); }
 }
 
 void SpectralOps::Div(float* out, const float* const v[3]) {
     const Dims& d = m_Opt->domain;
     hipStream_t st = m_Opt->stream;
     c32* s[3] = {m_Spec[0].get(), m_Spec[1].get(), m_Spec[2].get()};
-    for (int c = 0; c < 3; ++c) {
-        { KernelScope k(m_Opt->prof, "fft_x3_r2c", bytes_x3_r2c(d)); m_Plan->x3_r2c(v[c], s[c], st); }
-        { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[c], -1, st); }
-    }
+    { KernelScope k(m_Opt->prof, "fft_x3_r2c", 3.0 * bytes_x3_r2c(d)); m_Plan->x3_r2c_n(v, s, 3, st); }
+    { KernelScope k(m_Opt->prof, "fft_x2", 3.0 * bytes_col(d)); m_Plan->x2_n(s, 3, -1, st); }
     { KernelScope k(m_Opt->prof, "fft_x1_div", 32.0 * d.nlc); m_Plan->x1_div(s, s[0], st); }
     { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[0], +1, st); }
     { KernelScope k(m_Opt->prof, "fft_x3_c2r", bytes_x3_c2r(d, false)); m_Plan->x3_c2r(s[0], out, st); }
@@ -259,15 +267,13 @@ void SpectralOps::Multiplier3(const SpecParams& sp, const float* const in[3], fl
     const Dims& d = m_Opt->domain;
     hipStream_t st = m_Opt->stream;
     c32* s[3] = {m_Spec[0].get(), m_Spec[1].get(), m_Spec[2].get()};
-    for (int c = 0; c < 3; ++c) {
-        { KernelScope k(m_Opt->prof, "fft_x3_r2c", bytes_x3_r2c(d)); m_Plan->x3_r2c(in[c], s[c], st); }
-        { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[c], -1, st); }
-    }
+    { KernelScope k(m_Opt->prof, "fft_x3_r2c", 3.0 * bytes_x3_r2c(d)); m_Plan->x3_r2c_n(in, s, 3, st); }
+    { KernelScope k(m_Opt->prof, "fft_x2", 3.0 * bytes_col(d)); m_Plan->x2_n(s, 3, -1, st); }
     { KernelScope k(m_Opt->prof, "fft_x1_mult", 3.0 * bytes_col(d)); m_Plan->x1_multiplier(s, s, 3, sp, st); }
-    for (int c = 0; c < 3; ++c) {
-        const float* a = add ? add[c] : nullptr;
-        { KernelScope k(m_Opt->prof, "fft_x2", bytes_col(d)); m_Plan->x2(s[c], +1, st); }
-        { KernelScope k(m_Opt->prof, "fft_x3_c2r", bytes_x3_c2r(d, a != nullptr)); m_Plan->x3_c2r(s[c], out[c], st, alpha, a, beta); }
+    { KernelScope k(m_Opt->prof, "fft_x2", 3.0 * bytes_col(d)); m_Plan->x2_n(s, 3, +1, st); }
+    {
+        KernelScope k(m_Opt->prof, "fft_x3_c2r", 3.0 * bytes_x3_c2r(d, add != nullptr));
+        m_Plan->x3_c2r_n(s, out, 3, st, alpha, add, beta);
     }
 }
 
@@ -326,17 +332,21 @@ int Regularization::EvaluateFunctional(double* R, const VecField* v) {
     float* w[3] = {work.m_X1, work.m_X2, work.m_X3};
     const int norm = Norm();
     double acc = 0.0;
+    double dots[Reducer::kSlots];
     if (norm == kRegH1 || norm == kRegH1SN) {
-        // Σ_i ‖∇v_i‖² (src/RegularizationH1SN.cpp:84-141)
+        // Σ_i ‖∇v_i‖² (src/RegularizationH1SN.cpp:84-141); the nine products
+        // (and ‖v‖² for H1) are read back together and summed in order
         for (int c = 0; c < 3; ++c) {
             const float* vc = c == 0 ? v->m_X1 : (c == 1 ? v->m_X2 : v->m_X3);
             m_Spec->Grad(w, vc);
-            for (int k = 0; k < 3; ++k) acc += m_Red->Dot(w[k], w[k], nl);
+            for (int k = 0; k < 3; ++k) m_Red->DotTo(w[k], w[k], nl, 3 * c + k);
         }
+        if (norm == kRegH1) m_Red->DotTo(v->m_X1, v->m_X1, 3 * nl, 9);
+        m_Red->Fetch(dots, norm == kRegH1 ? 10 : 9);
+        for (int i = 0; i < 9; ++i) acc += dots[i];
         if (norm == kRegH1) {
             // ½·hd·β0·(H1v + β1·L2v), src/RegularizationH1.cpp:140-147
-            const double l2 = m_Red->Dot(v->m_X1, v->m_X1, 3 * nl);
-            *R = 0.5 * hd * m_Opt->beta[0] * (acc + m_Opt->beta[1] * l2);
+            *R = 0.5 * hd * m_Opt->beta[0] * (acc + m_Opt->beta[1] * dots[9]);
         } else {
             *R = 0.5 * hd * m_Opt->beta[0] * acc;
         }
@@ -350,7 +360,9 @@ int Regularization::EvaluateFunctional(double* R, const VecField* v) {
         sp.beta1 = std::sqrt(m_Opt->beta[1]);
     }
     m_Spec->Multiplier3(sp, in, w, 1.0f, nullptr, 0.0f);
-    for (int k = 0; k < 3; ++k) acc += m_Red->Dot(w[k], w[k], nl);
+    for (int k = 0; k < 3; ++k) m_Red->DotTo(w[k], w[k], nl, k);
+    m_Red->Fetch(dots, 3);
+    for (int k = 0; k < 3; ++k) acc += dots[k];
     if (norm == kRegH2)
         *R = 0.5 * hd * acc;                         // src/RegularizationH2.cpp:164-166
     else

@@ -145,6 +145,12 @@ class Reducer {
     double Min(const float* a, IntType n);
     double Max(const float* a, IntType n);
     double Sum(const float* a, IntType n);
+    // Several dot products into result slots (0 <= slot < kSlots), read back
+    // with one synchronization by Fetch: the host waits once instead of
+    // once per product.
+    static constexpr int kSlots = 16;
+    void DotTo(const float* a, const float* b, IntType n, int slot);
+    void Fetch(double* out, int nslots);
 
   private:
     double fetch();

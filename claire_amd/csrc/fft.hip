@@ -315,9 +315,26 @@ __device__ __forceinline__ float spec_multiplier(const SpecParams& p, int w1, in
 // ---------------------------------------------------------------------------
 // x3 pass, real -> half complex.  Rows 2p and 2p+1 are packed as a + i·b.
 // ---------------------------------------------------------------------------
+// Up to three fields per launch: blockIdx.y selects the field (a vector
+// field's components go through one launch instead of three).
+struct R2CPtrs {
+    const float* in[3];
+    c32* out[3];
+};
+struct C2RPtrs {
+    const c32* in[3];
+    float* out[3];
+    const float* add[3];
+};
+struct ColPtrs {
+    const c32* in[3];
+    c32* out[3];
+};
+
 template <int N>
-__global__ __launch_bounds__(Geo<N>::THREADS) void k_row_r2c(const float* __restrict__ in, c32* __restrict__ out,
-                                                             int64_t nrows, const c32* __restrict__ tw) {
+__global__ __launch_bounds__(Geo<N>::THREADS) void k_row_r2c(R2CPtrs ptr, int64_t nrows, const c32* __restrict__ tw) {
+    const float* __restrict__ in = ptr.in[blockIdx.y];
+    c32* __restrict__ out = ptr.out[blockIdx.y];
     using G = Geo<N>;
     __shared__ c32 s[G::LDS_ELEMS];
     constexpr int V = G::V, T = G::T, R0 = First<typename Plan<N>::Fwd>::value,
@@ -366,9 +383,11 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_row_r2c(const float* __rest
 // (FFTW c2r convention).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(Geo<N>::THREADS) void k_row_c2r(const c32* __restrict__ in, float* __restrict__ out,
-                                                             int64_t nrows, const c32* __restrict__ tw, float alpha,
-                                                             const float* __restrict__ add, float beta) {
+__global__ __launch_bounds__(Geo<N>::THREADS) void k_row_c2r(C2RPtrs ptr, int64_t nrows, const c32* __restrict__ tw,
+                                                             float alpha, float beta) {
+    const c32* __restrict__ in = ptr.in[blockIdx.y];
+    float* __restrict__ out = ptr.out[blockIdx.y];
+    const float* __restrict__ add = ptr.add[blockIdx.y];
     using G = Geo<N>;
     __shared__ c32 s[G::LDS_ELEMS];
     constexpr int V = G::V, T = G::T, R0 = First<typename Plan<N>::Inv>::value,
@@ -470,8 +489,9 @@ __device__ __forceinline__ int64_t col_base(const ColGeom& g, int64_t cg) {
 }
 
 template <int N, int DIR>
-__global__ __launch_bounds__(Geo<N>::THREADS) void k_col(const c32* __restrict__ in, c32* __restrict__ out,
-                                                         ColGeom geo, const c32* __restrict__ tw) {
+__global__ __launch_bounds__(Geo<N>::THREADS) void k_col(ColPtrs ptr, ColGeom geo, const c32* __restrict__ tw) {
+    const c32* in = ptr.in[blockIdx.y];  // may equal out (in place): no __restrict__
+    c32* out = ptr.out[blockIdx.y];
     using G = Geo<N>;
     __shared__ c32 s[G::LDS_ELEMS];
     using L = typename std::conditional<(DIR < 0), typename Plan<N>::Fwd, typename Plan<N>::Inv>::type;
@@ -646,37 +666,44 @@ void dispatch_n(int n, A&&... a) {
 
 template <int N>
 struct RowR2C {
-    static void go(const float* in, c32* out, int64_t nrows, const c32* tw, hipStream_t st) {
+    static void go(const R2CPtrs& p, int ncomp, int64_t nrows, const c32* tw, hipStream_t st) {
         using G = Geo<N>;
         const int64_t npairs = (nrows + 1) / 2;
         const int64_t blocks = (npairs + G::B - 1) / G::B;
-        hipLaunchKernelGGL(k_row_r2c<N>, dim3(static_cast<unsigned>(blocks)), dim3(G::THREADS), 0, st, in, out, nrows, tw);
+        hipLaunchKernelGGL(k_row_r2c<N>, dim3(static_cast<unsigned>(blocks), ncomp), dim3(G::THREADS), 0, st, p, nrows,
+                           tw);
     }
 };
 
 template <int N>
 struct RowC2R {
-    static void go(const c32* in, float* out, int64_t nrows, const c32* tw, float alpha, const float* add, float beta,
-                   hipStream_t st) {
+    static void go(const C2RPtrs& p, int ncomp, int64_t nrows, const c32* tw, float alpha, float beta, hipStream_t st) {
         using G = Geo<N>;
         const int64_t npairs = (nrows + 1) / 2;
         const int64_t blocks = (npairs + G::B - 1) / G::B;
-        hipLaunchKernelGGL(k_row_c2r<N>, dim3(static_cast<unsigned>(blocks)), dim3(G::THREADS), 0, st, in, out, nrows, tw,
-                           alpha, add, beta);
+        hipLaunchKernelGGL(k_row_c2r<N>, dim3(static_cast<unsigned>(blocks), ncomp), dim3(G::THREADS), 0, st, p, nrows,
+                           tw, alpha, beta);
     }
 };
 
 template <int N>
 struct ColPass {
-    static void go(const c32* in, c32* out, ColGeom geo, int dir, const c32* tw, hipStream_t st) {
+    static void go(const ColPtrs& p, int ncomp, ColGeom geo, int dir, const c32* tw, hipStream_t st) {
         using G = Geo<N>;
-        const int64_t blocks = (geo.ncols + G::B - 1) / G::B;
+        const dim3 grid(static_cast<unsigned>((geo.ncols + G::B - 1) / G::B), ncomp);
         if (dir < 0)
-            hipLaunchKernelGGL((k_col<N, -1>), dim3(static_cast<unsigned>(blocks)), dim3(G::THREADS), 0, st, in, out, geo, tw);
+            hipLaunchKernelGGL((k_col<N, -1>), grid, dim3(G::THREADS), 0, st, p, geo, tw);
         else
-            hipLaunchKernelGGL((k_col<N, +1>), dim3(static_cast<unsigned>(blocks)), dim3(G::THREADS), 0, st, in, out, geo, tw);
+            hipLaunchKernelGGL((k_col<N, +1>), grid, dim3(G::THREADS), 0, st, p, geo, tw);
     }
 };
+
+ColPtrs col_ptrs(c32* const data[], int ncomp) {
+    CLR_REQUIRE(ncomp >= 1 && ncomp <= 3, kErrInvalidArgument, "1 to 3 fields per FFT pass");
+    ColPtrs p{};
+    for (int c = 0; c < ncomp; ++c) p.in[c] = p.out[c] = data[c];
+    return p;
+}
 
 template <int N>
 struct X1Fused {
@@ -737,30 +764,59 @@ FFTPlan3D::~FFTPlan3D() {
 }
 
 void FFTPlan3D::r2c(const float* in, c32* out, hipStream_t st) const {
-    const int64_t nrows = static_cast<int64_t>(dims.n[0]) * dims.n[1];
-    dispatch_n<RowR2C>(dims.n[2], in, out, nrows, tw[2], st);
-    dispatch_n<ColPass>(dims.n[1], out, out, geom_x2(dims), -1, tw[1], st);
-    dispatch_n<ColPass>(dims.n[0], out, out, geom_x1(dims), -1, tw[0], st);
+    x3_r2c(in, out, st);
+    x2(out, -1, st);
+    x1(out, -1, st);
 }
 
 void FFTPlan3D::c2r(c32* in, float* out, hipStream_t st, float alpha, const float* add, float beta) const {
-    const int64_t nrows = static_cast<int64_t>(dims.n[0]) * dims.n[1];
-    dispatch_n<ColPass>(dims.n[0], in, in, geom_x1(dims), +1, tw[0], st);
-    dispatch_n<ColPass>(dims.n[1], in, in, geom_x2(dims), +1, tw[1], st);
-    dispatch_n<RowC2R>(dims.n[2], in, out, nrows, tw[2], alpha, add, beta, st);
+    x1(in, +1, st);
+    x2(in, +1, st);
+    x3_c2r(in, out, st, alpha, add, beta);
+}
+
+void FFTPlan3D::x3_r2c_n(const float* const in[], c32* const out[], int ncomp, hipStream_t st) const {
+    CLR_REQUIRE(ncomp >= 1 && ncomp <= 3, kErrInvalidArgument, "1 to 3 fields per FFT pass");
+    R2CPtrs p{};
+    for (int c = 0; c < ncomp; ++c) {
+        p.in[c] = in[c];
+        p.out[c] = out[c];
+    }
+    dispatch_n<RowR2C>(dims.n[2], p, ncomp, static_cast<int64_t>(dims.n[0]) * dims.n[1], tw[2], st);
+}
+void FFTPlan3D::x3_c2r_n(const c32* const in[], float* const out[], int ncomp, hipStream_t st, float alpha,
+                         const float* const add[], float beta) const {
+    CLR_REQUIRE(ncomp >= 1 && ncomp <= 3, kErrInvalidArgument, "1 to 3 fields per FFT pass");
+    C2RPtrs p{};
+    for (int c = 0; c < ncomp; ++c) {
+        p.in[c] = in[c];
+        p.out[c] = out[c];
+        p.add[c] = add ? add[c] : nullptr;
+    }
+    dispatch_n<RowC2R>(dims.n[2], p, ncomp, static_cast<int64_t>(dims.n[0]) * dims.n[1], tw[2], alpha, beta, st);
+}
+void FFTPlan3D::x2_n(c32* const data[], int ncomp, int dir, hipStream_t st) const {
+    dispatch_n<ColPass>(dims.n[1], col_ptrs(data, ncomp), ncomp, geom_x2(dims), dir, tw[1], st);
 }
 
 void FFTPlan3D::x3_r2c(const float* in, c32* out, hipStream_t st) const {
-    dispatch_n<RowR2C>(dims.n[2], in, out, static_cast<int64_t>(dims.n[0]) * dims.n[1], tw[2], st);
+    const float* i[1] = {in};
+    c32* o[1] = {out};
+    x3_r2c_n(i, o, 1, st);
 }
 void FFTPlan3D::x3_c2r(const c32* in, float* out, hipStream_t st, float alpha, const float* add, float beta) const {
-    dispatch_n<RowC2R>(dims.n[2], in, out, static_cast<int64_t>(dims.n[0]) * dims.n[1], tw[2], alpha, add, beta, st);
+    const c32* i[1] = {in};
+    float* o[1] = {out};
+    const float* a[1] = {add};
+    x3_c2r_n(i, o, 1, st, alpha, a, beta);
 }
 void FFTPlan3D::x2(c32* data, int dir, hipStream_t st) const {
-    dispatch_n<ColPass>(dims.n[1], data, data, geom_x2(dims), dir, tw[1], st);
+    c32* d[1] = {data};
+    x2_n(d, 1, dir, st);
 }
 void FFTPlan3D::x1(c32* data, int dir, hipStream_t st) const {
-    dispatch_n<ColPass>(dims.n[0], data, data, geom_x1(dims), dir, tw[0], st);
+    c32* d[1] = {data};
+    dispatch_n<ColPass>(dims.n[0], col_ptrs(d, 1), 1, geom_x1(dims), dir, tw[0], st);
 }
 
 void FFTPlan3D::x1_multiplier(const c32* const in[3], c32* const out[3], int ncomp, const SpecParams& sp,

@@ -35,6 +35,12 @@ class FFTPlan3D {
                 float beta = 0.f) const;
     void x2(c32* data, int dir, hipStream_t st) const;
     void x1(c32* data, int dir, hipStream_t st) const;
+    // The same passes over 1 to 3 fields in one launch (add may be null;
+    // add[c] may equal out[c] but no other field's out).
+    void x3_r2c_n(const float* const in[], c32* const out[], int ncomp, hipStream_t st) const;
+    void x3_c2r_n(const c32* const in[], float* const out[], int ncomp, hipStream_t st, float alpha = 1.f,
+                  const float* const add[] = nullptr, float beta = 0.f) const;
+    void x2_n(c32* const data[], int ncomp, int dir, hipStream_t st) const;
     // Fused x1 passes (forward x1, k-space operator, inverse x1).
     void x1_multiplier(const c32* const in[3], c32* const out[3], int ncomp, const SpecParams& sp, hipStream_t st) const;
     void x1_grad(const c32* in, c32* const out[3], hipStream_t st) const;

@@ -60,10 +60,17 @@ int Optimizer::Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& 
         const double a = rz / pw;
         launch_axpy(static_cast<float>(a), p, d, m_N, Stream());
         launch_axpy(static_cast<float>(-a), w, r, m_N, Stream());
-        const double rnorm = std::sqrt(Dot(r, r));
-        if (rnorm <= tol || !(rnorm < 1e6 * bnorm)) break;
+        // z = M⁻¹r is applied before the convergence test so that ‖r‖² and
+        // rᵀz come back with one synchronization (on the last iteration z
+        // is not used)
         precond(r, z);
-        const double rz_new = Dot(r, z);
+        m_Prob->Red()->DotTo(r, r, m_N, 0);
+        m_Prob->Red()->DotTo(r, z, m_N, 1);
+        double rr_rz[2];
+        m_Prob->Red()->Fetch(rr_rz, 2);
+        const double rnorm = std::sqrt(rr_rz[0]);
+        if (rnorm <= tol || !(rnorm < 1e6 * bnorm)) break;
+        const double rz_new = rr_rz[1];
         const double beta = rz_new / rz;
         rz = rz_new;
         // p = z + β·p
