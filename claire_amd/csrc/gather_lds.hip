@@ -23,6 +23,11 @@
 // tiles and keeps the shared halo rows in its own L2.
 #include <algorithm>
 #include <cstdlib>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 #include "gather_device.hpp"
 
@@ -439,16 +444,79 @@ int choose_chunk(int n1, int64_t tiles) {
     return best;
 }
 
+// Chunk-length autotuning.  The first launch of a gather shape (bound D,
+// grid, number of fields) times the heuristic's choice and a few other
+// lengths on the launch stream and keeps the fastest for every later launch
+// of that shape.  The chunk length only changes how the planes are split
+// among blocks, not the arithmetic of any point, so every candidate writes
+// the same output.  Tuning is skipped while the stream is being captured into
+// a graph; CLAIRE_AMD_GATHER_TUNE=0 turns it off (the heuristic is used).
+struct ShapeKey {
+    int d, n1, n2, n3, nz;
+    bool operator<(const ShapeKey& o) const {
+        return std::tie(d, n1, n2, n3, nz) < std::tie(o.d, o.n1, o.n2, o.n3, o.nz);
+    }
+};
+std::mutex g_tune_mu;
+std::map<ShapeKey, int> g_tuned;
+
+bool tuning_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("CLAIRE_AMD_GATHER_TUNE");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
+}
+
 template <int D, class Epi>
 void launch_lds(const GatherDims& g, const uint32_t* P, const float* f, int64_t fzstride, int nz, const Epi& epi,
                 hipStream_t st) {
     const int nt3 = (g.n3 + LdsGeo<D>::LT3 - 1) / LdsGeo<D>::LT3;
     const int nt2 = (g.n2 + kLT2 - 1) / kLT2;
-    const int chunk = choose_chunk<D>(g.n1, static_cast<int64_t>(nt3) * nt2 * nz);
-    const int nch = (g.n1 + chunk - 1) / chunk;
-    const int nb = nt3 * nt2 * nch;
-    hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
-                       reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
+    auto launch = [&](int chunk) {
+        const int nch = (g.n1 + chunk - 1) / chunk;
+        const int nb = nt3 * nt2 * nch;
+        hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
+                           reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
+    };
+    if (chunk_override() >= 4) return launch(std::min(chunk_override(), g.n1));
+    const ShapeKey key{D, g.n1, g.n2, g.n3, nz};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        const auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) return launch(it->second);
+    }
+    const int guess = choose_chunk<D>(g.n1, static_cast<int64_t>(nt3) * nt2 * nz);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (!tuning_enabled() || hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+        return launch(guess);
+    std::vector<int> cand{guess};
+    for (int c : {16, 24, 32, 40, 48})
+        if (c <= g.n1 && std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+    launch(guess);  // warm-up; its output is the result
+    int best = guess;
+    if (cand.size() > 1) {
+        hipEvent_t ev[2];
+        CLR_HIP_CHECK(hipEventCreate(&ev[0]));
+        CLR_HIP_CHECK(hipEventCreate(&ev[1]));
+        float best_ms = std::numeric_limits<float>::max();
+        for (int c : cand) {
+            CLR_HIP_CHECK(hipEventRecord(ev[0], st));
+            launch(c);
+            CLR_HIP_CHECK(hipEventRecord(ev[1], st));
+            CLR_HIP_CHECK(hipEventSynchronize(ev[1]));
+            float ms = 0.f;
+            CLR_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+    }
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned[key] = best;
 }
 
 template <class Epi>
