@@ -122,6 +122,7 @@ class GNReport(ctypes.Structure):
                [(n, ctypes.c_double) for n in ("J0", "D0", "gnorm0", "J", "D", "R", "gnorm")]
 
 
+MODELS = {"compressible": 0, "stokes": 1, "relaxed_stokes": 2}
 GN_REASONS = {0: "continue", 1: "gttol", 2: "gatol", 3: "steptol", -1: "maxits", -2: "nan", -3: "linesearch"}
 
 
@@ -200,7 +201,7 @@ class Claire:
     """One registration problem (GN, semi-Lagrangian, nc = 1) on the GPU."""
 
     def __init__(self, nx, nt=4, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0), stream=None, precision="fp32",
-                 rk_order=2, graphs=False):
+                 rk_order=2, graphs=False, model="compressible"):
         self.nx = tuple(int(n) for n in nx)
         self.nt = int(nt)
         self.nl = int(np.prod(self.nx))
@@ -217,6 +218,8 @@ class Claire:
             self.set_rk_order(rk_order)
         if graphs:
             self.set_matvec_graphs(True)
+        if model != "compressible":
+            self.set_reg_model(model)
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -339,6 +342,17 @@ class Claire:
         """"fp32" (the reference's path) or "mixed" (FP16 storage of the
         Newton-invariant gradient fields in the Hessian matvec)."""
         _check(lib().clr_set_precision(self.ctx, ctypes.c_int({"fp32": 0, "mixed": 1}[precision])))
+
+    def set_reg_model(self, model):
+        """"compressible" (CLAIRE), "stokes" (CLAIREStokes) or
+        "relaxed_stokes" (CLAIREDivReg)."""
+        _check(lib().clr_set_reg_model(self.ctx, ctypes.c_int(MODELS[model])))
+
+    def apply_projection(self, v):
+        """The model's ApplyProjection of a vector field (3, n1, n2, n3)."""
+        d = self._in("proj", v, (3,) + self.nx)
+        _check(lib().clr_apply_projection(self.ctx, _vp(d.ptr)))
+        return d.download()
 
     def set_matvec_graphs(self, on):
         """Replay Hessian matvecs as captured HIP graphs (same results)."""

@@ -327,6 +327,18 @@ int clr_set_precision(clr_ctx* ctx, int precision) {
     return guard([&] { C(ctx).SetPrecision(precision); });
 }
 
+int clr_set_reg_model(clr_ctx* ctx, int model) {
+    return guard([&] { C(ctx).SetRegModel(model); });
+}
+
+int clr_apply_projection(clr_ctx* ctx, float* v) {
+    return guard([&] {
+        need(v, "v");
+        C(ctx).ApplyProjection(v);
+        check_hip_launch();
+    });
+}
+
 int clr_set_matvec_graphs(clr_ctx* ctx, int on) {
     return guard([&] { C(ctx).SetMatVecGraphs(on != 0); });
 }

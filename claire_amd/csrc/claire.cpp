@@ -277,6 +277,23 @@ void SpectralOps::Multiplier3(const SpecParams& sp, const float* const in[3], fl
     }
 }
 
+void SpectralOps::Project(const SpecParams& sp, float* const b[3]) {
+    const Dims& d = m_Opt->domain;
+    hipStream_t st = m_Opt->stream;
+    c32* s[3] = {m_Spec[0].get(), m_Spec[1].get(), m_Spec[2].get()};
+    const float* in[3] = {b[0], b[1], b[2]};
+    { KernelScope k(m_Opt->prof, "fft_x3_r2c", 3.0 * bytes_x3_r2c(d)); m_Plan->x3_r2c_n(in, s, 3, st); }
+    { KernelScope k(m_Opt->prof, "fft_x2", 3.0 * bytes_col(d)); m_Plan->x2_n(s, 3, -1, st); }
+    { KernelScope k(m_Opt->prof, "fft_x1_project", 3.0 * bytes_col(d)); m_Plan->x1_project(s, sp, st); }
+    { KernelScope k(m_Opt->prof, "fft_x2", 3.0 * bytes_col(d)); m_Plan->x2_n(s, 3, +1, st); }
+    {
+        // b + F⁻¹[x̂]: the c2r epilogue adds the field in place (the AXPY of
+        // src/CLAIREStokes.cpp:536, src/CLAIREDivReg.cpp:452)
+        KernelScope k(m_Opt->prof, "fft_x3_c2r", 3.0 * bytes_x3_c2r(d, true));
+        m_Plan->x3_c2r_n(s, b, 3, st, 1.0f, in, 1.0f);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Regularization (src/Regularization*.cpp)
 // ---------------------------------------------------------------------------
@@ -614,7 +631,10 @@ void CLAIRE::EnsureTrajectories() {
     // (src/CLAIRE.cpp:2614-2635 / 3907-3927 compute ∇·v and (∇·v)(X) per solve)
     const float* vc[3] = {m_VelocityField->m_X1, m_VelocityField->m_X2, m_VelocityField->m_X3};
     m_Spectral->Div(m_Scalar.get(), vc);
-    {
+    if (m_Opt.regmodel == kModelStokes) {
+        // incompressible adjoints are pure transport (src/CLAIREStokes.cpp:169-406): c = 1
+        launch_fill(m_AdjCoeff.get(), 1.0f, m_Opt.domain.nl, m_Opt.stream);
+    } else {
         // bytes: X 12 + div v gathered 4 + div v 4 + C 4 per point
         KernelScope k(m_Opt.prof, "gather_adjoint_coeff", 24.0 * m_Opt.domain.nl);
         launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Plan("adjoint").X, m_Scalar.get(),
@@ -830,6 +850,7 @@ int CLAIRE::SolveIncAdjointFrom(const float* mtilde) {
     launch_copy(mtilde, m_Lam.get(), nl, m_Opt.stream);
     launch_scale(m_Lam.get(), -1.0f, nl, m_Opt.stream);
     IncAdjointCore(m_Lam.get(), m_IncBodyForce.get(), false);
+    ProjectField(m_IncBodyForce.get());  // src/CLAIRE.cpp:3467-3468
     // λ̃ at t = 0: the last slot (constant in time when v = 0)
     launch_copy(m_VelocityIsZero ? m_Lam.get() : m_Lam.get() + m_Opt.nt * nl, m_IncAdjointT0.get(), nl, m_Opt.stream);
     return 0;
@@ -917,6 +938,7 @@ void CLAIRE::HessianMatVecBody(float* Hvtilde, const VecField* vt, bool scale) {
     IncStateCore(vt, m_Lam.get(), half);
     // \tilde{λ} and \tilde{b}·hd (src/CLAIRE.cpp:1348, 3307-3491)
     IncAdjointCore(m_Lam.get(), m_IncBodyForce.get(), half);
+    ProjectField(m_IncBodyForce.get());  // K[b̃], src/CLAIRE.cpp:3467-3468
     // Hṽ = β·A[ṽ] + b̃ (src/CLAIRE.cpp:1356-1361); 1/hd when !scale (src/CLAIRE.cpp:1283-1286)
     const float a = scale ? 1.0f : 1.0f / m_Opt.GetLebesgueMeasure();
     m_Regularization->ApplyFused(&hv, vt, a, m_IncBodyForce.get(), a);
@@ -931,7 +953,39 @@ int CLAIRE::SolveAdjointEquation() {
     // λ¹ = mR − m^nt (src/DistanceMeasureSL2.cpp:198-205)
     launch_sub(m_ReferenceImage.get(), m_State.get() + m_Opt.nt * nl, m_Lam.get(), nl, m_Opt.stream);
     IncAdjointCore(m_Lam.get(), m_BodyForce.get(), false);
+    ProjectField(m_BodyForce.get());  // src/CLAIRE.cpp:2337-2338
     launch_copy(m_VelocityIsZero ? m_Lam.get() : m_Lam.get() + m_Opt.nt * nl, m_Adjoint.get(), nl, m_Opt.stream);
+    return 0;
+}
+
+void CLAIRE::ProjectField(VecField* b) {
+    if (m_Opt.regmodel == kModelCompressible) return;
+    SpecParams sp{};
+    sp.op = m_Opt.regmodel == kModelStokes ? kOpProjStokes : kOpProjRelaxed;
+    sp.beta0 = m_Opt.beta[0];
+    sp.beta1 = m_Opt.beta[1];
+    sp.beta2 = m_Opt.beta[2];
+    sp.hd = m_Opt.GetLebesgueMeasure();
+    sp.scale = m_Opt.ComputeFFTScale();
+    float* c[3] = {b->m_X1, b->m_X2, b->m_X3};
+    m_Spectral->Project(sp, c);
+}
+
+int CLAIRE::ApplyProjection(float* v) {
+    CLR_REQUIRE(v != nullptr, kErrInvalidArgument, "null vector field");
+    VecField view(&m_Opt, v);
+    ProjectField(&view);
+    return 0;
+}
+
+int CLAIRE::SetRegModel(int model) {
+    CLR_REQUIRE(model == kModelCompressible || model == kModelStokes || model == kModelRelaxedStokes,
+                kErrInvalidArgument, "registration model must be 0 (compressible), 1 (Stokes) or 2 (relaxed Stokes)");
+    if (model != m_Opt.regmodel) {
+        m_Opt.regmodel = model;
+        m_TrajValid = m_StateValid = m_GradMValid = m_GradMXValid = m_HalfValid = false;
+        ++m_GraphEpoch;
+    }
     return 0;
 }
 
@@ -944,7 +998,10 @@ int CLAIRE::EvaluateObjective(double* J, double* D, double* R, const float* v) {
     // ½·hd·Σ(mR − m^nt)²/nc (src/DistanceMeasureSL2.cpp:124-147)
     *D = 0.5 * hd * m_Red->SqDist(m_ReferenceImage.get(), m_State.get() + m_Opt.nt * nl, nl);
     *R = 0.0;
-    if (!m_VelocityIsZero) m_Regularization->EvaluateFunctional(R, m_VelocityField.get());
+    if (!m_VelocityIsZero) {
+        m_Regularization->EvaluateFunctional(R, m_VelocityField.get());
+        if (m_Opt.regmodel == kModelRelaxedStokes) *R += DivergencePenalty();
+    }
     *J = *D + *R;
     return 0;
 }
@@ -958,6 +1015,26 @@ int CLAIRE::EvaluateGradient(float* g, const float* v) {
     // g = β·A[v] + b (src/CLAIRE.cpp:1021-1026)
     m_Regularization->ApplyFused(&gv, m_VelocityField.get(), 1.0f, m_BodyForce.get(), 1.0f);
     return 0;
+}
+
+// CLAIREDivReg::EvaluteRegularizationDIV (src/CLAIREDivReg.cpp:190-260):
+// ½·hd·β2·(‖∂3 d‖² + ‖∂2 d‖² + ‖∂1 d‖² + ‖d‖²), d = ∇·v (the reference
+// stores the gradient with its components reversed and sums in that order).
+double CLAIRE::DivergencePenalty() {
+    const int64_t nl = m_Opt.domain.nl;
+    const float* vc[3] = {m_VelocityField->m_X1, m_VelocityField->m_X2, m_VelocityField->m_X3};
+    float* d = m_Scalar.get();
+    m_Spectral->Div(d, vc);
+    float* gd[3] = {m_WorkVecField1->m_X3, m_WorkVecField1->m_X2, m_WorkVecField1->m_X1};
+    m_Spectral->Grad(gd, d);
+    m_Red->DotTo(m_WorkVecField1->m_X1, m_WorkVecField1->m_X1, nl, 0);
+    m_Red->DotTo(m_WorkVecField1->m_X2, m_WorkVecField1->m_X2, nl, 1);
+    m_Red->DotTo(m_WorkVecField1->m_X3, m_WorkVecField1->m_X3, nl, 2);
+    m_Red->DotTo(d, d, nl, 3);
+    double r[4];
+    m_Red->Fetch(r, 4);
+    const double sum = ((r[0] + r[1]) + r[2]) + r[3];
+    return 0.5 * static_cast<double>(m_Opt.GetLebesgueMeasure()) * static_cast<double>(m_Opt.beta[2]) * sum;
 }
 
 int CLAIRE::ApplyInvRegularizationOperator(float* Ainvx, const float* x, bool applysqrt) {

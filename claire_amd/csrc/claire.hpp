@@ -126,6 +126,9 @@ struct RegOpt {
     int precision = 0;
     // Replay the Hessian matvec as a captured HIP graph (CLAIRE::HessianMatVec).
     int graphs = 0;
+    // Registration model (m_RegModel): compressible (CLAIRE), Stokes
+    // (CLAIREStokes) or relaxed Stokes (CLAIREDivReg).
+    int regmodel = claire_amd::kModelCompressible;
     // Order of the characteristic scheme, 2 or 4 (m_PDESolver.rkorder,
     // src/RegOpt.cpp:674-676, default 2 at src/RegOpt.cpp:1087).
     int rkorder = 2;
@@ -193,6 +196,9 @@ class SpectralOps {
     // out_d = alpha·F⁻¹(m(k)·F(in_d)) + beta·add_d, d = 0..2
     void Multiplier3(const claire_amd::SpecParams& sp, const float* const in[3], float* const out[3], float alpha,
                      const float* const add[3], float beta);
+    // b ← b + F⁻¹[x̂] with the projection of sp.op (kOpProjStokes or
+    // kOpProjRelaxed), in place on the three components.
+    void Project(const claire_amd::SpecParams& sp, float* const b[3]);
     c32* Spectrum(int i) const { return m_Spec[i].get(); }
 
   private:
@@ -315,6 +321,12 @@ class CLAIRE {
     // Characteristic scheme of every later trajectory: 2 or 4
     // (SemiLagrangian::ComputeTrajectory, src/SemiLagrangian.cpp:172-178).
     int SetRKOrder(int order);
+    // Registration model of every later solve (kModelCompressible,
+    // kModelStokes, kModelRelaxedStokes); drops what was computed before.
+    int SetRegModel(int model);
+    // The model's ApplyProjection on a flat vector field, in place (a no-op
+    // for the compressible model, src/CLAIRE.cpp:4044-4048).
+    int ApplyProjection(float* v);
     // Stream of every later launch (work already queued stays where it is).
     void SetStream(hipStream_t s) { m_Opt.stream = s; }
     // Capture the Hessian matvec into HIP graphs and replay them (on/off).
@@ -335,6 +347,8 @@ class CLAIRE {
     void IncAdjointCore(const float* lt0, VecField* bt, bool half);
     void EnsureHalfGradients();
     void HessianMatVecBody(float* Hvtilde, const VecField* vt, bool scale);
+    void ProjectField(VecField* b);
+    double DivergencePenalty();
     void CaptureMatVec(float* Hvtilde, const VecField* vt, bool scale, hipGraphExec_t* exec);
     void DropGraphs();
 

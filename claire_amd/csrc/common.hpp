@@ -64,14 +64,20 @@ enum SpecOp : int {
     kOpRegInverse = 1,   // inverse
     kOpRegInvSqrt = 2,   // inverse square root
     kOpH2Functional = 3, // scale·√β0·(lapik + √β1)
-    kOpScaledLaplace = 4 // scale·lapik
+    kOpScaledLaplace = 4,// scale·lapik
+    kOpProjStokes = 5,   // CLAIREStokes::ApplyProjection
+    kOpProjRelaxed = 6   // CLAIREDivReg::ApplyProjection
 };
+
+// Registration models (RegOpt m_RegModel).
+enum RegModelKind : int { kModelCompressible = 0, kModelStokes = 1, kModelRelaxedStokes = 2 };
 
 struct SpecParams {
     int op;
     int norm;
     float beta0, beta1;
     float hd, scale;
+    float beta2;  // divergence weight (projection of the relaxed model)
 };
 
 }  // namespace claire_amd

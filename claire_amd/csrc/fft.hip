@@ -532,7 +532,29 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_col(ColPtrs ptr, ColGeom ge
 //   kFuseDiv   out[0] = F1⁻¹(Σ_d i·w_d·scale·F1(in[d]))
 // Columns of the x1 pass are (k2, k3) with cg = k2·nh + k3.
 // ---------------------------------------------------------------------------
-enum FuseMode { kFuseMult = 0, kFuseGrad = 1, kFuseDiv = 2 };
+enum FuseMode { kFuseMult = 0, kFuseGrad = 1, kFuseDiv = 2, kFuseProject = 3 };
+
+// Per-mode factor m of the models' projections, FP32/FP64 exactly as the
+// reference writes it: ĉ = scale·m·(g·b̂), x̂_i = g_i·ĉ, b ← b + F⁻¹x̂.
+//   Stokes (src/CLAIREStokes.cpp:457-462): m = −1/|w|², the Nyquist index
+//   kept at +n/2 in |w|², −1 at w = 0.
+//   Relaxed (src/CLAIREDivReg.cpp:362-404): lapik = −|g|² (Nyquist zeroed),
+//   m = opik·(1/lapik), opik = 1/(β0·(1/(β2·(−lapik + 1))) + 1).
+__device__ __forceinline__ float proj_factor(const SpecParams& p, int k1, int k2, int k3, int n1, int n2, int n3) {
+    const int x1 = k1 > n1 / 2 ? k1 - n1 : k1, x2 = k2 > n2 / 2 ? k2 - n2 : k2, x3 = k3 > n3 / 2 ? k3 - n3 : k3;
+    if (p.op == kOpProjStokes) {
+        const float lap = static_cast<float>(static_cast<long>(x1) * x1 + static_cast<long>(x2) * x2 +
+                                             static_cast<long>(x3) * x3);
+        return lap == 0.0f ? -1.0f : static_cast<float>(-1.0 / static_cast<double>(lap));
+    }
+    const int z1 = k1 == n1 / 2 ? 0 : x1, z2 = k2 == n2 / 2 ? 0 : x2, z3 = k3 == n3 / 2 ? 0 : x3;
+    const float lapik = -static_cast<float>(static_cast<long>(z1) * z1 + static_cast<long>(z2) * z2 +
+                                            static_cast<long>(z3) * z3);
+    const float lapinvik = lapik == 0.0f ? -1.0f : static_cast<float>(1.0 / static_cast<double>(lapik));
+    float opik = static_cast<float>(1.0 / (static_cast<double>(p.beta2) * (static_cast<double>(-lapik) + 1.0)));
+    opik = static_cast<float>(1.0 / (static_cast<double>(p.beta0) * static_cast<double>(opik) + 1.0));
+    return opik * lapinvik;
+}
 
 struct Ptr3 {
     const c32* in[3];
@@ -614,7 +636,7 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_x1_fused(Ptr3 ptr, ColGeom 
             }
             inv_store(v, ptr.out[d]);
         }
-    } else {
+    } else if constexpr (MODE == kFuseDiv) {
         c32 acc[V];
 #pragma unroll
         for (int i = 0; i < V; ++i) acc[i] = {0.f, 0.f};
@@ -640,6 +662,35 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_x1_fused(Ptr3 ptr, ColGeom 
 #pragma unroll
         for (int i = 0; i < V; ++i) acc[i] = {acc[i].x * sc, acc[i].y * sc};
         inv_store(acc, ptr.out[0]);
+    } else {
+        // projection of a vector field: all three components at every mode
+        c32 a[V], b[V], e[V];
+        load_fwd(ptr.in[0], a);
+        load_fwd(ptr.in[1], b);
+        load_fwd(ptr.in[2], e);
+        const float sc = sp.scale;
+        const float g2 = static_cast<float>(w2), g3 = static_cast<float>(w3);
+#pragma unroll
+        for (int q = 0; q < V / RL; ++q) {
+            const int j = t + T * q;
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                const int k1 = j + r * (N / RL);
+                const float g1 = static_cast<float>(wavenum(k1, N));
+                const float m = proj_factor(sp, k1, k2, k3, N, n2, n3);
+                const int i = q * RL + r;
+                float cre = -sc * (g1 * a[i].x + g2 * b[i].x + g3 * e[i].x);
+                float cim = sc * (g1 * a[i].y + g2 * b[i].y + g3 * e[i].y);
+                cre *= m;
+                cim *= m;
+                b[i] = {-g2 * cre, g2 * cim};
+                e[i] = {-g3 * cre, g3 * cim};
+                a[i] = {cre * -g1, cim * g1};
+            }
+        }
+        inv_store(a, ptr.out[0]);
+        inv_store(b, ptr.out[1]);
+        inv_store(e, ptr.out[2]);
     }
 }
 
@@ -718,6 +769,10 @@ struct X1Fused {
                 break;
             case kFuseGrad:
                 hipLaunchKernelGGL((k_x1_fused<N, kFuseGrad>), grid, dim3(G::THREADS), 0, st, p, geo, tw, sp, n2, n3, nh);
+                break;
+            case kFuseProject:
+                hipLaunchKernelGGL((k_x1_fused<N, kFuseProject>), grid, dim3(G::THREADS), 0, st, p, geo, tw, sp, n2, n3,
+                                   nh);
                 break;
             default:
                 hipLaunchKernelGGL((k_x1_fused<N, kFuseDiv>), grid, dim3(G::THREADS), 0, st, p, geo, tw, sp, n2, n3, nh);
@@ -837,6 +892,14 @@ void FFTPlan3D::x1_grad(const c32* in, c32* const out[3], hipStream_t st) const 
     SpecParams sp{};
     sp.scale = dims.scale;
     dispatch_n<X1Fused>(dims.n[0], static_cast<int>(kFuseGrad), p, geom_x1(dims), tw[0], sp, dims.n[1], dims.n[2],
+                        dims.nh, 3, st);
+}
+
+void FFTPlan3D::x1_project(c32* const data[3], const SpecParams& sp, hipStream_t st) const {
+    CLR_REQUIRE(sp.op == kOpProjStokes || sp.op == kOpProjRelaxed, kErrInvalidArgument, "not a projection");
+    Ptr3 p{};
+    for (int d = 0; d < 3; ++d) p.in[d] = p.out[d] = data[d];
+    dispatch_n<X1Fused>(dims.n[0], static_cast<int>(kFuseProject), p, geom_x1(dims), tw[0], sp, dims.n[1], dims.n[2],
                         dims.nh, 3, st);
 }
 

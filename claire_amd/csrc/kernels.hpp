@@ -45,6 +45,9 @@ class FFTPlan3D {
     void x1_multiplier(const c32* const in[3], c32* const out[3], int ncomp, const SpecParams& sp, hipStream_t st) const;
     void x1_grad(const c32* in, c32* const out[3], hipStream_t st) const;
     void x1_div(const c32* const in[3], c32* out, hipStream_t st) const;
+    // Projection of a vector field (sp.op kOpProjStokes or kOpProjRelaxed):
+    // x̂_i = g_i·scale·m·(g·b̂) in place of the three spectra.
+    void x1_project(c32* const data[3], const SpecParams& sp, hipStream_t st) const;
 
     Dims dims;
 

@@ -114,6 +114,30 @@ int clr_set_precision(clr_ctx* ctx, int precision);
  * SemiLagrangian::ComputeTrajectory (src/SemiLagrangian.cpp:172-178).
  * Changing it drops the cached trajectories and everything computed on them. */
 int clr_set_rk_order(clr_ctx* ctx, int order);
+/* Registration model, replacing the choice of problem class in
+ * CLAIREInterface (src/CLAIREInterface.cpp:685-690, RegOpt m_RegModel):
+ *   CLR_MODEL_COMPRESSIBLE   CLAIRE (default);
+ *   CLR_MODEL_STOKES         CLAIREStokes: incompressible, the adjoint and
+ *                            incremental adjoint are pure transport
+ *                            (src/CLAIREStokes.cpp:169-406) and both body
+ *                            forces are Leray-projected (ApplyProjection,
+ *                            src/CLAIREStokes.cpp:407-545);
+ *   CLR_MODEL_RELAXED_STOKES CLAIREDivReg: compressible adjoints, body forces
+ *                            projected with the β-weighted multiplier
+ *                            (src/CLAIREDivReg.cpp:313-460) and the divergence
+ *                            penalty ½·hd·β2·(‖∇(∇·v)‖² + ‖∇·v‖²) added to the
+ *                            objective (src/CLAIREDivReg.cpp:109-260); β2 is
+ *                            beta[2] of clr_create.
+ * Changing the model drops what was computed for the previous one. */
+#define CLR_MODEL_COMPRESSIBLE 0
+#define CLR_MODEL_STOKES 1
+#define CLR_MODEL_RELAXED_STOKES 2
+int clr_set_reg_model(clr_ctx* ctx, int model);
+/* The model's ApplyProjection on a vector field v (device, 3·nl), in place:
+ * v ← v + F⁻¹[g·scale·m·(g·v̂)] (CLAIREStokes::ApplyProjection,
+ * CLAIREDivReg::ApplyProjection; a no-op for the compressible model,
+ * src/CLAIRE.cpp:4044-4048). */
+int clr_apply_projection(clr_ctx* ctx, float* v);
 /* Launch mode of clr_hessian_matvec, with no counterpart in the reference
  * (whose matvec is a sequence of host calls behind MatShell MATOP_MULT,
  * src/Optimizer.cpp:434-437).  on = 1: each (input, output, scale) triple

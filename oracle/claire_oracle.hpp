@@ -288,6 +288,103 @@ void div(const Grid<T>& g, const T* v1, const T* v2, const T* v3, T* d) {
 }
 
 // ---------------------------------------------------------------------------
+// Registration models (RegOpt m_RegModel; CLAIREInterface picks the class,
+// src/CLAIREInterface.cpp:685-690): 0 COMPRESSIBLE (CLAIRE), 1 STOKES
+// (CLAIREStokes: pure-transport adjoints, projected body forces), 2
+// RELAXEDSTOKES (CLAIREDivReg: compressible adjoints, body forces projected
+// with a β-weighted multiplier, divergence penalty in the objective).  Test
+// infrastructure sets it per call.
+// ---------------------------------------------------------------------------
+enum RegModel { kCompressible = 0, kStokes = 1, kRelaxedStokes = 2 };
+
+inline int& reg_model() {
+    static int model = kCompressible;
+    return model;
+}
+
+// β of the model's projection (m_RegNorm.beta: β0 and, for RELAXEDSTOKES,
+// the divergence weight β2); the same values as the regularization's.
+inline double* reg_model_beta() {
+    static double b[3] = {0.0, 0.0, 0.0};
+    return b;
+}
+
+// ApplyProjection, in place: b ← b + F⁻¹[x̂] with, per mode,
+//   ĉ = scale·m·(g·b̂),  x̂_i = g_i·ĉ,
+// where g is the Nyquist-zeroed wave vector and
+//   STOKES (src/CLAIREStokes.cpp:446-513)        m = −1/|w|² (w with the
+//          Nyquist index kept at +n/2), m = −1 at w = 0;
+//   RELAXEDSTOKES (src/CLAIREDivReg.cpp:354-430) m = opik·lapinvik with
+//          lapik = −|g|², lapinvik = 1/lapik (−1 at 0),
+//          opik = 1/(β0·(1/(β2·(|g|² + 1))) + 1).
+// The statement order and the double literals follow the reference: the
+// real part is formed as −scale·Σ g_i·Re b̂_i and each output real part as
+// −g_i·ĉ_re, which is g_i·scale·m·Σ … .
+template <class T>
+void project(const Grid<T>& g, int model, const T* beta, T* const b[3]) {
+    if (model == kCompressible) return;
+    using C = std::complex<T>;
+    std::vector<C> h0(g.nlc), h1(g.nlc), h2(g.nlc);
+    r2c3d(g, b[0], h0.data());
+    r2c3d(g, b[1], h1.data());
+    r2c3d(g, b[2], h2.data());
+    const T scale = g.scale;
+    const int n0 = g.n[0], n1 = g.n[1], nh = g.nh;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k0 = 0; k0 < n0; ++k0)
+        for (int k1 = 0; k1 < n1; ++k1)
+            for (int k2 = 0; k2 < nh; ++k2) {
+                const long kk[3] = {k0, k1, k2};
+                long wx[3];
+                for (int a = 0; a < 3; ++a) wx[a] = kk[a] > g.n[a] / 2 ? kk[a] - g.n[a] : kk[a];
+                T m;
+                if (model == kStokes) {
+                    T lapinvik = static_cast<T>(wx[0] * wx[0] + wx[1] * wx[1] + wx[2] * wx[2]);
+                    lapinvik = lapinvik == T(0) ? T(-1) : static_cast<T>(-1.0 / lapinvik);
+                    m = lapinvik;
+                } else {
+                    long wz[3];
+                    for (int a = 0; a < 3; ++a) wz[a] = kk[a] == g.n[a] / 2 ? 0 : wx[a];
+                    const T lapik = -static_cast<T>(wz[0] * wz[0] + wz[1] * wz[1] + wz[2] * wz[2]);
+                    const T lapinvik = lapik == T(0) ? T(-1) : static_cast<T>(1.0 / lapik);
+                    T opik = static_cast<T>(1.0 / (beta[2] * (-lapik + 1.0)));
+                    opik = static_cast<T>(1.0 / (beta[0] * opik + 1.0));
+                    m = opik * lapinvik;
+                }
+                T gr[3];
+                for (int a = 0; a < 3; ++a) gr[a] = static_cast<T>(kk[a] == g.n[a] / 2 ? 0 : wx[a]);
+                const int64_t i = (static_cast<int64_t>(k0) * n1 + k1) * nh + k2;
+                const C x1 = h0[i], x2 = h1[i], x3 = h2[i];
+                T cre = -scale * (gr[0] * x1.real() + gr[1] * x2.real() + gr[2] * x3.real());
+                T cim = scale * (gr[0] * x1.imag() + gr[1] * x2.imag() + gr[2] * x3.imag());
+                cre *= m;
+                cim *= m;
+                h1[i] = C(-gr[1] * cre, gr[1] * cim);
+                h2[i] = C(-gr[2] * cre, gr[2] * cim);
+                h0[i] = C(cre * -gr[0], cim * gr[0]);
+            }
+    std::vector<T> x(g.nl);
+    C* hs[3] = {h0.data(), h1.data(), h2.data()};
+    for (int d = 0; d < 3; ++d) {
+        c2r3d(g, hs[d], x.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t l = 0; l < g.nl; ++l) b[d][l] += x[l];
+    }
+}
+
+// K[b]·hd, the end of SolveAdjointEquation / SolveIncAdjointEquation
+// (src/CLAIRE.cpp:2337-2341, 3467-3471): the model's projection, then the
+// scaling by hd.
+template <class T>
+void project_and_scale(const Grid<T>& g, T* const b[3]) {
+    const T beta[3] = {static_cast<T>(reg_model_beta()[0]), static_cast<T>(reg_model_beta()[1]),
+                       static_cast<T>(reg_model_beta()[2])};
+    project(g, reg_model(), beta, b);
+    for (int d = 0; d < 3; ++d)
+        for (int64_t i = 0; i < g.nl; ++i) b[d][i] *= g.hd;
+}
+
+// ---------------------------------------------------------------------------
 // Regularization operators (per-mode multipliers on each velocity component).
 // ---------------------------------------------------------------------------
 enum RegNorm { kH1 = 0, kH2 = 1, kH1SN = 2, kH2SN = 3 };
@@ -676,10 +773,13 @@ void solve_adjoint(const Grid<T>& g, int nt, const T* const v[3], const T* m, co
         }
     } else {
         const T ht = T(1) / static_cast<T>(nt);
+        const bool stokes = reg_model() == kStokes;  // CLAIREStokes: pure transport (src/CLAIREStokes.cpp:169-296)
         std::vector<T> X(3 * nl), divv(nl), divvx(nl), lx(nl);
         trajectory(g, v, ht, -1, X.data());
-        div(g, v[0], v[1], v[2], divv.data());
-        interp(g, divv.data(), X.data(), divvx.data());
+        if (!stokes) {
+            div(g, v[0], v[1], v[2], divv.data());
+            interp(g, divv.data(), X.data(), divvx.data());
+        }
         T scale = ht;
         for (int j = 0; j < nt; ++j) {
             const T* mj = m + static_cast<int64_t>(nt - j) * nl;
@@ -690,9 +790,13 @@ void solve_adjoint(const Grid<T>& g, int nt, const T* const v[3], const T* m, co
             for (int64_t i = 0; i < nl; ++i) {
                 const T lam = lambda[i];
                 const T lamx = lx[i];
-                const T rhs0 = lamx * divvx[i];
-                const T rhs1 = (lamx + ht * rhs0) * divv[i];
-                lambda[i] = static_cast<T>(lamx + 0.5 * ht * (rhs0 + rhs1));
+                if (stokes) {
+                    lambda[i] = lamx;
+                } else {
+                    const T rhs0 = lamx * divvx[i];
+                    const T rhs1 = (lamx + ht * rhs0) * divv[i];
+                    lambda[i] = static_cast<T>(lamx + 0.5 * ht * (rhs0 + rhs1));
+                }
                 b[0][i] += scale * g1[i] * lam;
                 b[1][i] += scale * g2[i] * lam;
                 b[2][i] += scale * g3[i] * lam;
@@ -708,8 +812,7 @@ void solve_adjoint(const Grid<T>& g, int nt, const T* const v[3], const T* m, co
             b[2][i] += static_cast<T>(0.5 * scale * g3[i] * lam);
         }
     }
-    for (int d = 0; d < 3; ++d)
-        for (int64_t i = 0; i < nl; ++i) b[d][i] *= g.hd;
+    project_and_scale(g, b);
 }
 
 // Incremental state equation, GN, SL (SolveIncStateEquationSL,
@@ -773,10 +876,13 @@ void solve_inc_adjoint_x(const Grid<T>& g, int nt, const T* const v[3], const T*
     } else {
         const T ht = T(1) / static_cast<T>(nt);
         const T hthalf = static_cast<T>(0.5) * ht;
+        const bool stokes = reg_model() == kStokes;  // src/CLAIREStokes.cpp:297-406
         std::vector<T> divv(nl), divvx(nl), lx(nl);
         const T* X = Xa;
-        div(g, v[0], v[1], v[2], divv.data());
-        interp(g, divv.data(), X, divvx.data());
+        if (!stokes) {
+            div(g, v[0], v[1], v[2], divv.data());
+            interp(g, divv.data(), X, divvx.data());
+        }
         T scale = ht;
         for (int j = 0; j < nt; ++j) {
             const T* mj = m + static_cast<int64_t>(nt - j) * nl;
@@ -787,9 +893,13 @@ void solve_inc_adjoint_x(const Grid<T>& g, int nt, const T* const v[3], const T*
             for (int64_t i = 0; i < nl; ++i) {
                 const T lt = ltilde[i];
                 const T ltx = lx[i];
-                const T rhs0 = ltx * divvx[i];
-                const T rhs1 = (ltx + ht * rhs0) * divv[i];
-                ltilde[i] = ltx + hthalf * (rhs0 + rhs1);
+                if (stokes) {
+                    ltilde[i] = ltx;
+                } else {
+                    const T rhs0 = ltx * divvx[i];
+                    const T rhs1 = (ltx + ht * rhs0) * divv[i];
+                    ltilde[i] = ltx + hthalf * (rhs0 + rhs1);
+                }
                 bt[0][i] += scale * g1[i] * lt;
                 bt[1][i] += scale * g2[i] * lt;
                 bt[2][i] += scale * g3[i] * lt;
@@ -805,8 +915,7 @@ void solve_inc_adjoint_x(const Grid<T>& g, int nt, const T* const v[3], const T*
             bt[2][i] += static_cast<T>(0.5 * scale * g3[i] * lt);
         }
     }
-    for (int d = 0; d < 3; ++d)
-        for (int64_t i = 0; i < nl; ++i) bt[d][i] *= g.hd;
+    project_and_scale(g, bt);
 }
 
 template <class T>
@@ -872,6 +981,23 @@ void gradient(const Grid<T>& g, int nt, int norm, const T* beta, const T* const 
         for (int64_t i = 0; i < nl; ++i) gout[d][i] += b[d][i];
 }
 
+// CLAIREDivReg::EvaluteRegularizationDIV (src/CLAIREDivReg.cpp:190-260):
+// Rw = ½·hd·β2·(‖∂3 d‖² + ‖∂2 d‖² + ‖∂1 d‖² + ‖d‖²), d = ∇·v; the gradient
+// is stored with its components reversed (grad into (x3, x2, x1)), so the
+// products are summed in that order.
+template <class T>
+double divergence_penalty(const Grid<T>& g, const T* beta, const T* const v[3]) {
+    std::vector<T> d(g.nl), g1(g.nl), g2(g.nl), g3(g.nl);
+    div(g, v[0], v[1], v[2], d.data());
+    grad(g, d.data(), g3.data(), g2.data(), g1.data());
+    double r = 0.0;
+    r += dot(g1.data(), g1.data(), g.nl);
+    r += dot(g2.data(), g2.data(), g.nl);
+    r += dot(g3.data(), g3.data(), g.nl);
+    r += dot(d.data(), d.data(), g.nl);
+    return 0.5 * static_cast<double>(g.hd) * static_cast<double>(beta[2]) * r;
+}
+
 // Objective J = D + R (CLAIRE::EvaluateObjective src/CLAIRE.cpp:793-865;
 // R skipped when v = 0).
 template <class T>
@@ -881,6 +1007,8 @@ void objective(const Grid<T>& g, int nt, int norm, const T* beta, const T* const
     solve_state(g, nt, v, mT, m.data());
     *D = distance_sl2(g, nt, m.data(), mR);
     *R = velocity_is_zero(g, v) ? 0.0 : reg_functional(g, norm, beta, v);
+    // CLAIREDivReg::EvaluateObjective adds the divergence penalty (src/CLAIREDivReg.cpp:109-188)
+    if (reg_model() == kRelaxedStokes && !velocity_is_zero(g, v)) *R += divergence_penalty(g, beta, v);
     *J = *D + *R;
 }
 

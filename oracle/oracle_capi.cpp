@@ -49,6 +49,18 @@ int oracle_set_rk_order(int order) {
     return 0;
 }
 
+// Registration model (0 compressible, 1 Stokes, 2 relaxed Stokes) and the
+// β of its projection, used by every solve that follows.
+int oracle_set_reg_model(int model, const double* beta) {
+    if (model < 0 || model > 2) {
+        g_err = "model must be 0, 1 or 2";
+        return 1;
+    }
+    oracle::reg_model() = model;
+    for (int i = 0; i < 3; ++i) oracle::reg_model_beta()[i] = beta ? beta[i] : 0.0;
+    return 0;
+}
+
 void oracle_set_num_threads(int n) {
 #ifdef _OPENMP
     if (n > 0) omp_set_num_threads(n);
@@ -157,6 +169,14 @@ void oracle_set_num_threads(int n) {
     int oracle_interp_points_##SFX(const int* nx, const T* f, int64_t npts, const T* X, T* out) {                 \
         return guard([&] {                                                                                         \
             for (int64_t i = 0; i < npts; ++i) out[i] = oracle::interp_point(f, nx, X + 3 * i);                    \
+        });                                                                                                        \
+    }                                                                                                              \
+    int oracle_project_##SFX(const int* nx, int model, const double* beta, T* b1, T* b2, T* b3) {                  \
+        return guard([&] {                                                                                         \
+            oracle::Grid<T> g(nx);                                                                                  \
+            const T bt[3] = {static_cast<T>(beta[0]), static_cast<T>(beta[1]), static_cast<T>(beta[2])};           \
+            T* b[3] = {b1, b2, b3};                                                                                 \
+            oracle::project(g, model, bt, b);                                                                       \
         });                                                                                                        \
     }                                                                                                              \
     int oracle_trajectory_##SFX(const int* nx, int nt, const T* v1, const T* v2, const T* v3, int sigma, T* X) {  \

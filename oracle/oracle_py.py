@@ -71,6 +71,39 @@ class rk_order:
         return False
 
 
+MODELS = {"compressible": 0, "stokes": 1, "relaxed_stokes": 2}
+
+
+def set_reg_model(model, beta=(0.0, 0.0, 0.0)) -> None:
+    """Registration model of every later solve ("compressible", "stokes",
+    "relaxed_stokes") and the β of its projection."""
+    if lib().oracle_set_reg_model(int(MODELS[model]), _beta(beta)) != 0:
+        raise ValueError(lib().oracle_last_error().decode())
+
+
+class reg_model:
+    """Context manager that runs a block with the given registration model."""
+
+    def __init__(self, model, beta=(0.0, 0.0, 0.0)):
+        self.model, self.beta = model, beta
+
+    def __enter__(self):
+        set_reg_model(self.model, self.beta)
+        return self
+
+    def __exit__(self, *exc):
+        set_reg_model("compressible")
+        return False
+
+
+def project(b, model, beta, prec="f64"):
+    """The model's ApplyProjection on a vector field (3, n1, n2, n3)."""
+    out = np.array(_c(b, prec), copy=True)
+    _call("oracle_project", prec, _nx(out.shape[1:]), ctypes.c_int(MODELS[model]), _beta(beta), _p(out[0]),
+          _p(out[1]), _p(out[2]))
+    return out
+
+
 def _dt(prec):
     return np.float64 if prec == "f64" else np.float32
 
