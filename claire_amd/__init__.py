@@ -343,6 +343,11 @@ class Claire:
         Newton-invariant gradient fields in the Hessian matvec)."""
         _check(lib().clr_set_precision(self.ctx, ctypes.c_int({"fp32": 0, "mixed": 1}[precision])))
 
+    def set_matvec_type(self, kind):
+        """"default" (HessMatVec), "precond" (PrecondHessMatVec) or
+        "precond_sym" (PrecondHessMatVecSym)."""
+        _check(lib().clr_set_matvec_type(self.ctx, ctypes.c_int({"default": 0, "precond": 1, "precond_sym": 2}[kind])))
+
     def set_reg_model(self, model):
         """"compressible" (CLAIRE), "stokes" (CLAIREStokes) or
         "relaxed_stokes" (CLAIREDivReg)."""

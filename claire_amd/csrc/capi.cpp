@@ -327,6 +327,10 @@ int clr_set_precision(clr_ctx* ctx, int precision) {
     return guard([&] { C(ctx).SetPrecision(precision); });
 }
 
+int clr_set_matvec_type(clr_ctx* ctx, int type) {
+    return guard([&] { C(ctx).SetMatVecType(type); });
+}
+
 int clr_set_reg_model(clr_ctx* ctx, int model) {
     return guard([&] { C(ctx).SetRegModel(model); });
 }

@@ -931,10 +931,40 @@ void CLAIRE::CaptureMatVec(float* Hvtilde, const VecField* vt, bool scale, hipGr
     CLR_HIP_CHECK(e);
 }
 
+int CLAIRE::SetMatVecType(int type) {
+    CLR_REQUIRE(type >= 0 && type <= 2, kErrInvalidArgument,
+                "matvec type must be 0 (default), 1 (preconditioned) or 2 (symmetric preconditioned)");
+    if (type != m_Opt.matvectype) {
+        m_Opt.matvectype = type;
+        ++m_GraphEpoch;
+    }
+    return 0;
+}
+
 void CLAIRE::HessianMatVecBody(float* Hvtilde, const VecField* vt, bool scale) {
     VecField hv(&m_Opt, Hvtilde);
-    // \tilde{m} (src/CLAIRE.cpp:1345) with λ̃⁰ = −m̃^nt fused into the last step
     const bool half = m_Opt.precision == 1;
+    const float hd = m_Opt.GetLebesgueMeasure();
+    const int64_t n3 = 3 * m_Opt.domain.nl;
+    if (m_Opt.matvectype != 0) {
+        // PrecondHessMatVec (src/CLAIRE.cpp:1392-1451): hd·ṽ + (βA)⁻¹b̃[ṽ];
+        // PrecondHessMatVecSym (src/CLAIRE.cpp:1463-1540):
+        // hd·ṽ + (βA)^{-1/2}b̃[(βA)^{-1/2}ṽ]
+        const bool sym = m_Opt.matvectype == 2;
+        const VecField* in = vt;
+        if (sym) {
+            m_Regularization->ApplyInverse(m_WorkVecField1.get(), vt, true);
+            in = m_WorkVecField1.get();
+        }
+        IncStateCore(in, m_Lam.get(), half);
+        IncAdjointCore(m_Lam.get(), m_IncBodyForce.get(), half);
+        ProjectField(m_IncBodyForce.get());
+        m_Regularization->ApplyInverse(&hv, m_IncBodyForce.get(), sym);
+        launch_waxpy(hd, vt->m_X1, Hvtilde, Hvtilde, n3, m_Opt.stream);
+        if (!scale) launch_scale(Hvtilde, 1.0f / hd, n3, m_Opt.stream);  // src/CLAIRE.cpp:1283-1286
+        return;
+    }
+    // \tilde{m} (src/CLAIRE.cpp:1345) with λ̃⁰ = −m̃^nt fused into the last step
     IncStateCore(vt, m_Lam.get(), half);
     // \tilde{λ} and \tilde{b}·hd (src/CLAIRE.cpp:1348, 3307-3491)
     IncAdjointCore(m_Lam.get(), m_IncBodyForce.get(), half);

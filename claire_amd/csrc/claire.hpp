@@ -129,6 +129,9 @@ struct RegOpt {
     // Registration model (m_RegModel): compressible (CLAIRE), Stokes
     // (CLAIREStokes) or relaxed Stokes (CLAIREDivReg).
     int regmodel = claire_amd::kModelCompressible;
+    // Hessian operator (m_KrylovMethod.matvectype, include/RegOpt.hpp:94-98):
+    // 0 DEFAULTMATVEC, 1 PRECONDMATVEC, 2 PRECONDMATVECSYM.
+    int matvectype = 0;
     // Order of the characteristic scheme, 2 or 4 (m_PDESolver.rkorder,
     // src/RegOpt.cpp:674-676, default 2 at src/RegOpt.cpp:1087).
     int rkorder = 2;
@@ -321,6 +324,10 @@ class CLAIRE {
     // Characteristic scheme of every later trajectory: 2 or 4
     // (SemiLagrangian::ComputeTrajectory, src/SemiLagrangian.cpp:172-178).
     int SetRKOrder(int order);
+    // Operator applied by HessianMatVec: 0 the GN Hessian (HessMatVec),
+    // 1 hd·ṽ + (βA)⁻¹b̃[ṽ] (PrecondHessMatVec), 2 hd·ṽ +
+    // (βA)^{-1/2}b̃[(βA)^{-1/2}ṽ] (PrecondHessMatVecSym).
+    int SetMatVecType(int type);
     // Registration model of every later solve (kModelCompressible,
     // kModelStokes, kModelRelaxedStokes); drops what was computed before.
     int SetRegModel(int model);

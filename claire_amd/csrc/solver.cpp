@@ -34,15 +34,31 @@ int Optimizer::Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& 
     float* z = m_Z.get();
     float* p = m_P.get();
     float* w = m_W.get();
+    const int matvec = m_Prob->Opt().matvectype;
+    CLR_REQUIRE(matvec != 1, kErrInvalidArgument,
+                "the preconditioned Hessian (PRECONDMATVEC) is not symmetric; this driver's Krylov method is CG");
+    const bool sym = matvec == 2;
     launch_fill(d, 0.0f, m_N, Stream());
     launch_copy(g, r, m_N, Stream());
     launch_scale(r, -1.0f, m_N, Stream());
+    // PreKrylovSolve (src/CLAIREBase.cpp:941-953): the right-hand side of the
+    // symmetrically preconditioned system is (βA)^{-1/2}(−g)
+    if (sym) m_Prob->ApplyInvRegularizationOperator(r, r, true);
     const double bnorm = std::sqrt(Dot(r, r));
     const double tol = std::max(rtol * bnorm, 1e-16);
     const int kmax = std::max(0, opt.krylov_maxit - 1);
-    if (!(bnorm > tol)) return 0;
+    // PostKrylovSolve (src/CLAIREBase.cpp:959-969): d ← (βA)^{-1/2}d
+    auto post = [&]() {
+        if (sym) m_Prob->ApplyInvRegularizationOperator(d, d, true);
+    };
+    if (!(bnorm > tol)) {
+        post();
+        return 0;
+    }
+    // the symmetrically preconditioned operator is solved without a further
+    // preconditioner (the reference pairs it with its two-level scheme)
     auto precond = [&](const float* in, float* out) {
-        if (opt.precond)
+        if (opt.precond && !sym)
             m_Prob->ApplyInvRegularizationOperator(out, in, false);
         else
             launch_copy(in, out, m_N, Stream());
@@ -76,6 +92,7 @@ int Optimizer::Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& 
         // p = z + β·p
         launch_waxpy(static_cast<float>(beta), p, z, p, m_N, Stream());
     }
+    post();
     return its;
 }
 

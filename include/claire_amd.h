@@ -114,6 +114,21 @@ int clr_set_precision(clr_ctx* ctx, int precision);
  * SemiLagrangian::ComputeTrajectory (src/SemiLagrangian.cpp:172-178).
  * Changing it drops the cached trajectories and everything computed on them. */
 int clr_set_rk_order(clr_ctx* ctx, int order);
+/* Operator applied by clr_hessian_matvec, replacing m_KrylovMethod.matvectype
+ * (include/RegOpt.hpp:94-98, dispatched in CLAIRE::HessianMatVec,
+ * src/CLAIRE.cpp:1251-1277):
+ *   CLR_MATVEC_DEFAULT      the GN Hessian R''ṽ + b̃ (HessMatVec);
+ *   CLR_MATVEC_PRECOND      hd·ṽ + (βA)⁻¹b̃[ṽ] (PrecondHessMatVec,
+ *                           src/CLAIRE.cpp:1392-1451; not symmetric);
+ *   CLR_MATVEC_PRECOND_SYM  hd·ṽ + (βA)^{-1/2}b̃[(βA)^{-1/2}ṽ]
+ *                           (PrecondHessMatVecSym, src/CLAIRE.cpp:1463-1540).
+ * clr_solve_gn accepts DEFAULT and PRECOND_SYM; with PRECOND_SYM it applies
+ * (βA)^{-1/2} to the Krylov right-hand side and solution
+ * (PreKrylovSolve/PostKrylovSolve, src/CLAIREBase.cpp:941-969). */
+#define CLR_MATVEC_DEFAULT 0
+#define CLR_MATVEC_PRECOND 1
+#define CLR_MATVEC_PRECOND_SYM 2
+int clr_set_matvec_type(clr_ctx* ctx, int type);
 /* Registration model, replacing the choice of problem class in
  * CLAIREInterface (src/CLAIREInterface.cpp:685-690, RegOpt m_RegModel):
  *   CLR_MODEL_COMPRESSIBLE   CLAIRE (default);

@@ -302,6 +302,14 @@ inline int& reg_model() {
     return model;
 }
 
+// Hessian operator (m_KrylovMethod.matvectype): 0 DEFAULTMATVEC (HessMatVec),
+// 1 PRECONDMATVEC (PrecondHessMatVec), 2 PRECONDMATVECSYM
+// (PrecondHessMatVecSym).  Test infrastructure sets it per call.
+inline int& matvec_type() {
+    static int t = 0;
+    return t;
+}
+
 // β of the model's projection (m_RegNorm.beta: β0 and, for RELAXEDSTOKES,
 // the divergence weight β2); the same values as the regularization's.
 inline double* reg_model_beta() {
@@ -939,6 +947,30 @@ void hessian_matvec_x(const Grid<T>& g, int nt, int norm, const T* beta, const T
     const int64_t nl = g.nl;
     std::vector<T> mt(nl), lt(nl), b0(nl), b1(nl), b2(nl);
     T* bt[3] = {b0.data(), b1.data(), b2.data()};
+    if (matvec_type() != 0) {
+        // PrecondHessMatVec (src/CLAIRE.cpp:1392-1451): hd·ṽ + (βA)⁻¹b̃[ṽ];
+        // PrecondHessMatVecSym (src/CLAIRE.cpp:1463-1540):
+        // hd·ṽ + (βA)^{-1/2}b̃[(βA)^{-1/2}ṽ]
+        const bool sym = matvec_type() == 2;
+        std::vector<T> w0(nl), w1(nl), w2(nl);
+        T* w[3] = {w0.data(), w1.data(), w2.data()};
+        const T* in[3] = {vt[0], vt[1], vt[2]};
+        if (sym) {
+            reg_inverse(g, norm, beta, vt, w, true);
+            for (int d = 0; d < 3; ++d) in[d] = w[d];
+        }
+        solve_inc_state_x(g, nt, Xs, m, in, mt.data());
+        solve_inc_adjoint_x(g, nt, v, Xa, m, mt.data(), lt.data(), bt);
+        const T* btc[3] = {bt[0], bt[1], bt[2]};
+        reg_inverse(g, norm, beta, btc, hv, sym);
+        for (int d = 0; d < 3; ++d)
+            for (int64_t i = 0; i < nl; ++i) hv[d][i] = g.hd * vt[d][i] + hv[d][i];
+        if (!scale) {
+            for (int d = 0; d < 3; ++d)
+                for (int64_t i = 0; i < nl; ++i) hv[d][i] *= T(1) / g.hd;
+        }
+        return;
+    }
     solve_inc_state_x(g, nt, Xs, m, vt, mt.data());
     solve_inc_adjoint_x(g, nt, v, Xa, m, mt.data(), lt.data(), bt);
     reg_apply(g, norm, beta, vt, hv);
@@ -1154,8 +1186,13 @@ void gn_solve(const Grid<T>& g, int nt, int norm, const T* beta, T* const v[3], 
             const double q = gnorm / kg0;
             rtol = std::max(1e-16, std::min(0.5, o.forcing == 1 ? q : std::sqrt(q)));
         }
+        // PRECONDMATVECSYM: (βA)^{-1/2} on the right-hand side and the solution
+        // (PreKrylovSolve / PostKrylovSolve, src/CLAIREBase.cpp:941-969), no
+        // further preconditioner; PRECONDMATVEC is not symmetric (no CG)
+        if (matvec_type() == 1) throw std::invalid_argument("PRECONDMATVEC needs a nonsymmetric Krylov method");
+        const bool sym = matvec_type() == 2;
         auto apply_pc = [&](const std::array<const T*, 3>& in, const std::array<T*, 3>& out) {
-            if (o.precond) {
+            if (o.precond && !sym) {
                 reg_inverse(g, norm, beta, in.data(), out.data(), false);
             } else {
                 for (int c = 0; c < 3; ++c) std::copy(in[c], in[c] + nl, out[c]);
@@ -1163,6 +1200,11 @@ void gn_solve(const Grid<T>& g, int nt, int norm, const T* beta, T* const v[3], 
         };
         // KSPCG, x0 = 0, b = −g
         for (size_t i = 0; i < d.size(); ++i) { d[i] = T(0); r[i] = -gv[i]; }
+        if (sym) {
+            std::vector<T> tmp = r;
+            auto Tm = ptrs(tmp);
+            reg_inverse(g, norm, beta, cst(Tm).data(), Rr.data(), true);
+        }
         const double bnorm = std::sqrt(dot3(g, cst(Rr).data(), cst(Rr).data()));
         const double tol = std::max(rtol * bnorm, 1e-16);
         const int kmax = std::max(0, o.krylov_maxit - 1);
@@ -1192,6 +1234,11 @@ void gn_solve(const Grid<T>& g, int nt, int norm, const T* beta, T* const v[3], 
                 rz = rz_new;
                 for (size_t i = 0; i < p.size(); ++i) p[i] = z[i] + static_cast<T>(bta) * p[i];
             }
+        }
+        if (sym) {
+            std::vector<T> tmp = d;
+            auto Tm = ptrs(tmp);
+            reg_inverse(g, norm, beta, cst(Tm).data(), Dd.data(), true);
         }
         rep.pcg_iterations += its;
         if (h) h[6] = its;

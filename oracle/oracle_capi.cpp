@@ -49,6 +49,17 @@ int oracle_set_rk_order(int order) {
     return 0;
 }
 
+// Hessian operator type of every later matvec (0 default, 1 preconditioned,
+// 2 symmetric preconditioned).
+int oracle_set_matvec_type(int t) {
+    if (t < 0 || t > 2) {
+        g_err = "matvec type must be 0, 1 or 2";
+        return 1;
+    }
+    oracle::matvec_type() = t;
+    return 0;
+}
+
 // Registration model (0 compressible, 1 Stokes, 2 relaxed Stokes) and the
 // β of its projection, used by every solve that follows.
 int oracle_set_reg_model(int model, const double* beta) {

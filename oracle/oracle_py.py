@@ -96,6 +96,25 @@ class reg_model:
         return False
 
 
+MATVEC_TYPES = {"default": 0, "precond": 1, "precond_sym": 2}
+
+
+class matvec_type:
+    """Context manager: Hessian operator type of the matvecs in a block."""
+
+    def __init__(self, kind):
+        self.kind = kind
+
+    def __enter__(self):
+        if lib().oracle_set_matvec_type(MATVEC_TYPES[self.kind]) != 0:
+            raise ValueError(lib().oracle_last_error().decode())
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_matvec_type(0)
+        return False
+
+
 def project(b, model, beta, prec="f64"):
     """The model's ApplyProjection on a vector field (3, n1, n2, n3)."""
     out = np.array(_c(b, prec), copy=True)

@@ -124,3 +124,56 @@ def test_model_registration(model, oracle):
     assert abs(rep["J"] - rep64["J"]) <= max(1e-3 * rep64["J"], 2.0 * abs(rep32["J"] - rep64["J"]))
     red, red64, red32 = (r["gnorm"] / r["gnorm0"] for r in (rep, rep64, rep32))
     assert abs(red - red64) <= max(1e-3 * red64, 5e-5, 2.0 * abs(red32 - red64))
+
+
+@pytest.mark.parametrize("kind", ["precond", "precond_sym"])
+@pytest.mark.parametrize("scale", [True, False])
+def test_preconditioned_matvec_types(kind, scale, oracle):
+    """PrecondHessMatVec / PrecondHessMatVecSym (clr_set_matvec_type) against
+    the oracle with the same operator type."""
+    shape = (32, 16, 64)
+    print(f"\n[gpu case] matvec {kind} scale={scale}", flush=True)
+    v, mT, mR, vt = _data(shape, seed=120)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA)
+    try:
+        c.set_matvec_type(kind)
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        H = c.hessian_matvec(vt, scale=scale)
+    finally:
+        c.close()
+    with oracle.matvec_type(kind):
+        H32 = oracle.hessian_matvec(v, oracle.solve_state(v, mT, NT, prec="f32"), vt, NT, "h1sn", BETA, scale=scale,
+                                    prec="f32")
+        H64 = oracle.hessian_matvec(v, oracle.solve_state(v, mT, NT), vt, NT, "h1sn", BETA, scale=scale)
+    assert_fp32_parity(H, H32, H64)
+
+
+def test_registration_with_the_symmetric_preconditioned_operator(oracle):
+    shape = (32, 32, 32)
+    print(f"\n[gpu case] GN registration precond_sym {shape}", flush=True)
+    mT, vstar = synth.smooth_template_problem(shape, np.float64)
+    mR = synth.rescale(oracle.solve_state(vstar, mT, NT)[NT])
+    beta = (1e-2, 1e-4, 0.0)
+    with oracle.matvec_type("precond_sym"):
+        v64, rep64, _ = oracle.gn_solve(np.zeros((3,) + shape), mT, mR, NT, "h1sn", beta, maxit=10)
+        v32, rep32, _ = oracle.gn_solve(np.zeros((3,) + shape, np.float32), mT, mR, NT, "h1sn", beta, maxit=10,
+                                        prec="f32")
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=beta)
+    try:
+        c.set_matvec_type("precond_sym")
+        c.set_template_image(mT.astype(np.float32))
+        c.set_reference_image(mR.astype(np.float32))
+        v, rep, _ = c.solve_gn(maxit=10)
+        c.set_matvec_type("precond")
+        from claire_amd import ClaireError
+
+        with pytest.raises(ClaireError, match="not symmetric"):
+            c.solve_gn(maxit=2)
+    finally:
+        c.close()
+    for tag, r in (("gpu", rep), ("fp32 oracle", rep32), ("fp64 oracle", rep64)):
+        print(f"{tag}: its {r['iterations']} pcg {r['pcg_iterations']} J {r['J']:.6e} "
+              f"|g|/|g0| {r['gnorm'] / r['gnorm0']:.6e} ({r['reason']})")
+    assert rep["reason"] == rep64["reason"]
+    assert abs(rep["J"] - rep64["J"]) <= max(1e-3 * rep64["J"], 2.0 * abs(rep32["J"] - rep64["J"]))

@@ -399,3 +399,44 @@ def test_registration_golden_reproduces(oracle):
     assert rep["reason"] == ref["reason"] and rep["iterations"] == ref["iterations"]
     for k in ("J", "D", "R", "gnorm", "gnorm0"):
         assert abs(rep[k] - ref[k]) <= 1e-9 * abs(ref[k]), k
+
+
+@pytest.mark.parametrize("norm", ["h1", "h1sn", "h2sn"])
+def test_preconditioned_matvecs_are_compositions(oracle, norm):
+    """PrecondHessMatVec = (βA)⁻¹H and PrecondHessMatVecSym =
+    (βA)^{-1/2}H(βA)^{-1/2} (src/CLAIRE.cpp:1380-1540), because (βA)⁻¹R'' is
+    hd·I on the modes the operators act on: checked against the default
+    matvec and the inverse regularization, on zero-mean fields without
+    Nyquist content."""
+    shape, nt, beta = (16, 16, 16), 4, (1e-2, 1e-4, 0.0)
+    v = synth.bandlimited_velocity(shape, kmax=3, cfl=1.0, nt=nt, seed=21).astype(np.float64)
+    mT = synth.gaussian_blobs(shape, count=5, seed=22).astype(np.float64)
+    # FP64 band-limited field: no content on the Nyquist indices, where the
+    # seminorms' multiplier vanishes but their inverse does not
+    vt = synth.bandlimited_velocity(shape, kmax=4, cfl=1.0, nt=nt, seed=23, dtype=np.float64)
+    vt -= vt.mean(axis=(1, 2, 3), keepdims=True)
+    m = oracle.solve_state(v, mT, nt)
+    H = oracle.hessian_matvec(v, m, vt, nt, norm, beta)
+    with oracle.matvec_type("precond"):
+        P = oracle.hessian_matvec(v, m, vt, nt, norm, beta)
+    assert relerr(P, oracle.reg_inverse(H, norm, beta)) < 1e-12
+    half = oracle.reg_inverse(vt, norm, beta, applysqrt=True)
+    Hh = oracle.hessian_matvec(v, m, half, nt, norm, beta)
+    with oracle.matvec_type("precond_sym"):
+        S = oracle.hessian_matvec(v, m, vt, nt, norm, beta)
+    assert relerr(S, oracle.reg_inverse(Hh, norm, beta, applysqrt=True)) < 1e-12
+
+
+def test_gauss_newton_with_the_symmetric_preconditioned_operator(oracle):
+    """PRECONDMATVECSYM in the GN driver: (βA)^{-1/2} on the right-hand side
+    and the solution; the Newton steps decrease J and reach gttol."""
+    shape, nt, beta = (16, 16, 16), 4, (1e-2, 1e-4, 0.0)
+    mT, vstar = synth.smooth_template_problem(shape, np.float64)
+    mR = synth.rescale(oracle.solve_state(vstar, mT, nt)[nt])
+    with oracle.matvec_type("precond_sym"):
+        v, rep, hist = oracle.gn_solve(np.zeros((3,) + shape), mT, mR, nt, "h1sn", beta, maxit=20, gttol=5e-2)
+    assert rep["reason"] == "gttol" and rep["J"] < rep["J0"]
+    assert np.all(np.diff(hist[:, 1]) <= 0)
+    with oracle.matvec_type("precond"):
+        with pytest.raises(RuntimeError, match="nonsymmetric"):
+            oracle.gn_solve(np.zeros((3,) + shape), mT, mR, nt, "h1sn", beta, maxit=2)
