@@ -123,6 +123,7 @@ class GNReport(ctypes.Structure):
 
 
 MODELS = {"compressible": 0, "stokes": 1, "relaxed_stokes": 2}
+PDE_TYPES = {"continuity": 0, "transport": 1}
 GN_REASONS = {0: "continue", 1: "gttol", 2: "gatol", 3: "steptol", -1: "maxits", -2: "nan", -3: "linesearch"}
 
 
@@ -201,7 +202,7 @@ class Claire:
     """One registration problem (GN, semi-Lagrangian, nc = 1) on the GPU."""
 
     def __init__(self, nx, nt=4, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0), stream=None, precision="fp32",
-                 rk_order=2, graphs=False, model="compressible"):
+                 rk_order=2, graphs=False, model="compressible", pde="transport"):
         self.nx = tuple(int(n) for n in nx)
         self.nt = int(nt)
         self.nl = int(np.prod(self.nx))
@@ -220,6 +221,8 @@ class Claire:
             self.set_matvec_graphs(True)
         if model != "compressible":
             self.set_reg_model(model)
+        if pde != "transport":
+            self.set_pde_type(pde)
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -372,6 +375,13 @@ class Claire:
         """Characteristic scheme of every later trajectory: 2 or 4 (the
         reference's -rkorder option)."""
         _check(lib().clr_set_rk_order(self.ctx, ctypes.c_int(int(order))))
+
+    def set_pde_type(self, pde):
+        """Constraint of every later state solve: "transport" (default) or
+        "continuity" (the reference's m_PDESolver.pdetype)."""
+        if pde not in PDE_TYPES:
+            raise ValueError(f"pde type must be one of {sorted(PDE_TYPES)}")
+        _check(lib().clr_set_pde_type(self.ctx, ctypes.c_int(PDE_TYPES[pde])))
 
     def solve_gn(self, v0=None, hist_len=128, **options):
         """Gauss–Newton–Krylov registration (clr_solve_gn) from v0 (zero by

@@ -358,6 +358,10 @@ int clr_set_rk_order(clr_ctx* ctx, int order) {
     return guard([&] { C(ctx).SetRKOrder(order); });
 }
 
+int clr_set_pde_type(clr_ctx* ctx, int type) {
+    return guard([&] { C(ctx).SetPDEType(type); });
+}
+
 int clr_gn_options_default(clr_gn_options* opt) {
     return guard([&] {
         need(opt, "opt");

@@ -584,7 +584,8 @@ size_t CLAIRE::DeviceBytes() const {
                           nt * 3 * nl + nt * nl + nt * nl + nl + 2 * 3 * nl;
     const size_t halves = m_Opt.precision == 1 ? (2 * nt + 1) * 3 * nl : 0;
     const size_t rk4 = m_Opt.rkorder == 4 ? 3 * nl : 0;  // RK4 stage sum
-    return (floats + rk4) * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
+    const size_t cont = m_Opt.pdetype == 0 ? nl : 0;      // continuity-equation coefficient
+    return (floats + rk4 + cont) * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
 }
 
 void CLAIRE::RequireVelocity() const { CLR_REQUIRE(m_HaveVelocity, kErrState, "velocity field not set"); }
@@ -640,6 +641,16 @@ void CLAIRE::EnsureTrajectories() {
         launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Plan("adjoint").X, m_Scalar.get(),
                              m_Opt.GetTimeStepSize(), 1.0f, m_AdjCoeff.get(), m_Opt.stream);
     }
+    if (m_Opt.pdetype == 0) {
+        // continuity-equation state update m^{j+1} = c·m^j(X_s) with
+        // c = 1 − ht/2·(dX + (1 − ht·dX)·d), d = ∇·v, dX = d(X_s): the α = −1
+        // form of the coefficient on the state trajectory
+        // (SolveContinuityEquationSL, src/CLAIRE.cpp:2813-2835)
+        m_StateCoeff.resize(static_cast<size_t>(m_Opt.domain.nl));
+        KernelScope k(m_Opt.prof, "gather_state_coeff", 24.0 * m_Opt.domain.nl);
+        launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Plan("state").X, m_Scalar.get(),
+                             m_Opt.GetTimeStepSize(), -1.0f, m_StateCoeff.get(), m_Opt.stream);
+    }
     m_TrajValid = true;
     ++m_GraphEpoch;
 }
@@ -654,6 +665,14 @@ int CLAIRE::SolveStateEquation() {
     launch_copy(m_TemplateImage.get(), m, nl, m_Opt.stream);  // m^0 = mT (src/CLAIRE.cpp:1896)
     if (m_VelocityIsZero) {
         for (int j = 1; j <= nt; ++j) launch_copy(m, m + j * nl, nl, m_Opt.stream);  // src/CLAIRE.cpp:1909-1920
+    } else if (m_Opt.pdetype == 0) {
+        // SolveContinuityEquationSL (src/CLAIRE.cpp:2735-2854)
+        const TrajectoryPlan tp = m_SL->Plan("state");
+        const GatherDims g = gather_dims(m_Opt.domain);
+        for (int j = 0; j < nt; ++j) {
+            KernelScope k(m_Opt.prof, "continuity_step", 24.0 * nl);
+            launch_plan_adjoint_step(tp, g, m_StateCoeff.get(), m + j * nl, m + (j + 1) * nl, m_Opt.stream);
+        }
     } else {
         for (int j = 0; j < nt; ++j)  // src/CLAIRE.cpp:2153-2163
             m_SL->Gather(m + (j + 1) * nl, m + j * nl, "state");
@@ -750,6 +769,16 @@ int CLAIRE::SetRKOrder(int order) {
     if (order != m_Opt.rkorder) {
         m_Opt.rkorder = order;
         m_TrajValid = m_StateValid = m_GradMValid = m_GradMXValid = m_HalfValid = false;
+    }
+    return 0;
+}
+
+int CLAIRE::SetPDEType(int type) {
+    CLR_REQUIRE(type == 0 || type == 1, kErrInvalidArgument, "pde type must be 0 (continuity) or 1 (transport)");
+    if (type != m_Opt.pdetype) {
+        m_Opt.pdetype = type;
+        m_TrajValid = m_StateValid = m_GradMValid = m_GradMXValid = m_HalfValid = false;
+        ++m_GraphEpoch;
     }
     return 0;
 }

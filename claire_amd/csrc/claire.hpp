@@ -135,6 +135,9 @@ struct RegOpt {
     // Order of the characteristic scheme, 2 or 4 (m_PDESolver.rkorder,
     // src/RegOpt.cpp:674-676, default 2 at src/RegOpt.cpp:1087).
     int rkorder = 2;
+    // Constraint of the state solve (m_PDESolver.pdetype, include/RegOpt.hpp:42-46):
+    // 0 continuity equation, 1 transport equation (default, src/RegOpt.cpp:1089).
+    int pdetype = 1;
     // include/RegOpt.hpp:485-500, src/RegOpt.cpp:2505-2512
     float GetLebesgueMeasure() const { return domain.hd; }
     float GetTimeStepSize() const { return 1.0f / static_cast<float>(nt); }
@@ -324,6 +327,9 @@ class CLAIRE {
     // Characteristic scheme of every later trajectory: 2 or 4
     // (SemiLagrangian::ComputeTrajectory, src/SemiLagrangian.cpp:172-178).
     int SetRKOrder(int order);
+    // Constraint of every later state solve: 0 continuity, 1 transport
+    // (CLAIRE::SolveStateEquation, src/CLAIRE.cpp:1923-1926).
+    int SetPDEType(int type);
     // Operator applied by HessianMatVec: 0 the GN Hessian (HessMatVec),
     // 1 hd·ṽ + (βA)⁻¹b̃[ṽ] (PrecondHessMatVec), 2 hd·ṽ +
     // (βA)^{-1/2}b̃[(βA)^{-1/2}ṽ] (PrecondHessMatVecSym).
@@ -398,6 +404,7 @@ class CLAIRE {
     DeviceArray<float> m_Lam;            // adjoint time series λ^0..λ^nt, (nt+1)·nl
     DeviceArray<float> m_Scalar;         // nl work
     DeviceArray<float> m_AdjCoeff;       // nl: adjoint update coefficient at every point
+    DeviceArray<float> m_StateCoeff;     // nl: continuity-equation state coefficient (pdetype 0 only)
     DeviceArray<float> m_VtX;            // ṽ(X_s), 3·nl
     DeviceArray<uint16_t> m_GradM16;     // half copies of m_GradM / m_GradMX (mixed form)
     DeviceArray<uint16_t> m_GradMX16;

@@ -114,6 +114,19 @@ int clr_set_precision(clr_ctx* ctx, int precision);
  * SemiLagrangian::ComputeTrajectory (src/SemiLagrangian.cpp:172-178).
  * Changing it drops the cached trajectories and everything computed on them. */
 int clr_set_rk_order(clr_ctx* ctx, int order);
+/* Constraint of every later state solve, replacing RegOpt
+ * m_PDESolver.pdetype (include/RegOpt.hpp:42-46, default TRANSPORTEQ at
+ * src/RegOpt.cpp:1089; clairetools sets CONTINUITYEQ for its Ravens maps,
+ * apps/clairetools.cpp:469-470), read only by CLAIRE::SolveStateEquation
+ * (src/CLAIRE.cpp:1923-1926):
+ *   CLR_PDE_CONTINUITY  ∂t m + ∇·(m v) = 0 (SolveContinuityEquationSL,
+ *                       src/CLAIRE.cpp:2735-2854);
+ *   CLR_PDE_TRANSPORT   ∂t m + v·∇m = 0 (SolveStateEquationSL).
+ * The values are the reference enum's.  As in the reference, the adjoint,
+ * incremental and Hessian solves stay the transport forms. */
+#define CLR_PDE_CONTINUITY 0
+#define CLR_PDE_TRANSPORT 1
+int clr_set_pde_type(clr_ctx* ctx, int type);
 /* Operator applied by clr_hessian_matvec, replacing m_KrylovMethod.matvectype
  * (include/RegOpt.hpp:94-98, dispatched in CLAIRE::HessianMatVec,
  * src/CLAIRE.cpp:1251-1277):

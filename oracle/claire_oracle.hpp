@@ -732,6 +732,43 @@ bool velocity_is_zero(const Grid<T>& g, const T* const v[3]) {
 // State equation m^{j+1} = m^j(X_s), m^0 = mT; v = 0 copies
 // (CLAIRE::SolveStateEquation src/CLAIRE.cpp:1896-1920,
 //  SolveStateEquationSL src/CLAIRE.cpp:2116-2170).  m has (nt+1)·nl entries.
+// Hyperbolic constraint of the state solve (m_PDESolver.pdetype,
+// include/RegOpt.hpp:42-46): 0 the continuity equation ∂t m + ∇·(m v) = 0,
+// 1 the transport equation ∂t m + v·∇m = 0 (default, src/RegOpt.cpp:1089).
+// Only the state solve reads it (CLAIRE::SolveStateEquation,
+// src/CLAIRE.cpp:1923-1926).  Test infrastructure sets it per call.
+enum PDEType { kContinuityEq = 0, kTransportEq = 1 };
+
+inline int& pde_type() {
+    static int t = kTransportEq;
+    return t;
+}
+
+// SL solve of the continuity equation, CLAIRE::SolveContinuityEquationSL
+// (src/CLAIRE.cpp:2735-2854), on the state trajectory X: for j = 0..nt−1
+//   mx = m^j(X), rhs0 = −mx·(∇·v)(X), rhs1 = −(mx + ht·rhs0)·∇·v,
+//   m^{j+1} = mx + ht/2·(rhs0 + rhs1);   m^0 is already in place.
+template <class T>
+void solve_continuity(const Grid<T>& g, int nt, const T* const v[3], const T* X, T* m) {
+    const int64_t nl = g.nl;
+    const T ht = T(1) / static_cast<T>(nt);
+    const T hthalf = static_cast<T>(0.5) * ht;
+    std::vector<T> divv(nl), divvx(nl), mx(nl);
+    div(g, v[0], v[1], v[2], divv.data());
+    interp(g, divv.data(), X, divvx.data());
+    for (int j = 0; j < nt; ++j) {
+        interp(g, m + j * nl, X, mx.data());
+        T* next = m + (j + 1) * nl;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            const T x = mx[i];
+            const T rhs0 = -x * divvx[i];
+            const T rhs1 = -(x + ht * rhs0) * divv[i];
+            next[i] = x + hthalf * (rhs0 + rhs1);
+        }
+    }
+}
+
 template <class T>
 void solve_state(const Grid<T>& g, int nt, const T* const v[3], const T* mT, T* m) {
     std::copy(mT, mT + g.nl, m);
@@ -742,6 +779,10 @@ void solve_state(const Grid<T>& g, int nt, const T* const v[3], const T* mT, T* 
     const T ht = T(1) / static_cast<T>(nt);
     std::vector<T> X(3 * g.nl);
     trajectory(g, v, ht, +1, X.data());
+    if (pde_type() == kContinuityEq) {
+        solve_continuity(g, nt, v, X.data(), m);
+        return;
+    }
     for (int j = 0; j < nt; ++j) interp(g, m + j * g.nl, X.data(), m + (j + 1) * g.nl);
 }
 

@@ -49,6 +49,17 @@ int oracle_set_rk_order(int order) {
     return 0;
 }
 
+// Constraint of every later state solve: 0 continuity equation, 1 transport
+// equation (RegOpt m_PDESolver.pdetype).
+int oracle_set_pde_type(int t) {
+    if (t != 0 && t != 1) {
+        g_err = "pde type must be 0 (continuity) or 1 (transport)";
+        return 1;
+    }
+    oracle::pde_type() = t;
+    return 0;
+}
+
 // Hessian operator type of every later matvec (0 default, 1 preconditioned,
 // 2 symmetric preconditioned).
 int oracle_set_matvec_type(int t) {

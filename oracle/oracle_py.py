@@ -71,6 +71,31 @@ class rk_order:
         return False
 
 
+PDE_TYPES = {"continuity": 0, "transport": 1}
+
+
+def set_pde_type(name: str) -> None:
+    """Constraint of every later state solve: "transport" (default) or
+    "continuity"."""
+    if lib().oracle_set_pde_type(PDE_TYPES[name]) != 0:
+        raise ValueError(lib().oracle_last_error().decode())
+
+
+class pde_type:
+    """Context manager that runs a block with the given state constraint."""
+
+    def __init__(self, name: str):
+        self.name = name
+
+    def __enter__(self):
+        set_pde_type(self.name)
+        return self
+
+    def __exit__(self, *exc):
+        set_pde_type("transport")
+        return False
+
+
 MODELS = {"compressible": 0, "stokes": 1, "relaxed_stokes": 2}
 
 

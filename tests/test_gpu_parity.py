@@ -304,6 +304,40 @@ def test_rk4_characteristic(shape, oracle):
     assert np.abs(X2 - oracle.trajectory(v, NT, +1, prec="f32")).max() < 1e-6
 
 
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_continuity_state_equation(shape, oracle):
+    """pdetype CONTINUITYEQ (SolveContinuityEquationSL, src/CLAIRE.cpp:2735-2854):
+    the state solve against the FP32 and FP64 oracles run with the same
+    constraint, the objective on that state, the Hessian matvec (whose
+    incremental solves stay the transport forms, as in the reference), and
+    the switch back to the transport solve."""
+    print(f"\n[gpu case] continuity state {shape}", flush=True)
+    v, mT, mR, vt = _data(shape, seed=90)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA, pde="continuity")
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        c.set_reference_image(mR)
+        m = c.solve_state()
+        J = c.evaluate_objective()
+        H = c.hessian_matvec(vt)
+        c.set_pde_type("transport")
+        mt = c.solve_state()
+    finally:
+        c.close()
+    with oracle.pde_type("continuity"):
+        m32 = oracle.solve_state(v, mT, NT, prec="f32")
+        m64 = oracle.solve_state(v, mT, NT)
+        J64 = oracle.objective(v, mT, mR, NT, "h1sn", BETA)
+    H32 = oracle.hessian_matvec(v, m32, vt, NT, "h1sn", BETA, prec="f32")
+    H64 = oracle.hessian_matvec(v, m64, vt, NT, "h1sn", BETA)
+    assert_fp32_parity(m, m32, m64)
+    assert abs(J[0] - J64[0]) <= 1e-5 * abs(J64[0])
+    assert_fp32_parity(H, H32, H64)
+    assert relerr(mt, oracle.solve_state(v, mT, NT, prec="f32")) < 1e-5
+    assert relerr(m[NT], mt[NT]) > 1e-3  # the constraint did change
+
+
 def test_hessian_matvec_graph_replay():
     """clr_set_matvec_graphs: replays of the captured matvec equal the eager
     matvec bit for bit, on every replay, with eager work and host

@@ -344,6 +344,12 @@ def test_gauss_newton_krylov_invariants(oracle):
 def _flow_determinant(shape, A, sign, steps=400):
     """exp(sign·∫₀¹ (∇·v)(y(s)) ds) along dy/ds = −v(y), y(0) = x, for the
     analytic v = A·(sin x1 cos x2, sin x2 cos x3, sin x3 cos x1), by RK4."""
+    return _flow(shape, A, sign, steps)[0]
+
+
+def _flow(shape, A, sign, steps=400):
+    """(exp(sign·∫₀¹ (∇·v)(y(s)) ds), y(1)) along dy/ds = −v(y), y(0) = x,
+    for v = A·(sin x1 cos x2, sin x2 cos x3, sin x3 cos x1), by RK4."""
     x = np.stack(np.meshgrid(*[np.arange(n) * 2 * np.pi / n for n in shape], indexing="ij")).reshape(3, -1)
 
     def vel(y):
@@ -360,7 +366,7 @@ def _flow_determinant(shape, A, sign, steps=400):
         k4 = -vel(y + h * k3)
         integral = integral + h / 6 * (dv(y) + 2 * dv(y + h / 2 * k1) + 2 * dv(y + h / 2 * k2) + dv(y + h * k3))
         y = y + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
-    return np.exp(sign * integral).reshape(shape)
+    return np.exp(sign * integral).reshape(shape), y.reshape((3,) + tuple(shape))
 
 
 @pytest.mark.parametrize("inverse", [False, True])
@@ -440,3 +446,51 @@ def test_gauss_newton_with_the_symmetric_preconditioned_operator(oracle):
     with oracle.matvec_type("precond"):
         with pytest.raises(RuntimeError, match="nonsymmetric"):
             oracle.gn_solve(np.zeros((3,) + shape), mT, mR, nt, "h1sn", beta, maxit=2)
+
+
+def _continuity_initial(x):
+    return 1.0 + 0.4 * np.sin(x[0] + x[1]) * np.cos(x[2])
+
+
+def test_continuity_state_converges_to_characteristic_solution(oracle):
+    """Continuity-equation state (src/CLAIRE.cpp:2735-2854) against the
+    exact solution of ∂t m + ∇·(m v) = 0 by characteristics,
+    m(x, 1) = m0(y(1))·exp(−∫₀¹ (∇·v)(y(s)) ds), y the back-traced path:
+    second order in ht until the tricubic interpolation error of the 32³
+    grid (≈1e-3, accumulated over the nt steps) takes over; a dropped or
+    sign-flipped source term is O(1)."""
+    shape, A = (32, 32, 32), 0.5
+    x = np.stack(_grid(shape))
+    v = A * np.stack([np.sin(x[0]) * np.cos(x[1]), np.sin(x[1]) * np.cos(x[2]), np.sin(x[2]) * np.cos(x[0])])
+    decay, foot = _flow(shape, A, -1.0)
+    exact = _continuity_initial(foot) * decay
+    m0 = _continuity_initial(x)
+    with oracle.pde_type("continuity"):
+        errs = [np.abs(oracle.solve_state(v, m0, nt)[nt] - exact).max() for nt in (4, 8, 16)]
+    assert errs[1] < errs[0] / 3 and errs[2] < errs[1] and errs[2] < 2e-3
+    transport = np.abs(oracle.solve_state(v, m0, 16)[16] - exact).max()
+    assert transport > 10 * errs[2]  # the transport solve misses the source term
+
+
+def test_continuity_state_conserves_mass_and_reduces_to_transport(oracle):
+    """∫m(·, t) is invariant under the continuity equation (to the
+    discretization error, which falls with nt); for ∇·v = 0 the source terms
+    vanish and the solve equals the transport solve (src/CLAIRE.cpp:2153-2163)
+    to round-off; m0 ≡ 1 gives det(∇y⁻¹), whose SL update has the same form
+    (src/DeformationFields.cpp:714-721); v = 0 copies m0."""
+    shape = (32, 32, 32)
+    x = np.stack(_grid(shape))
+    m0 = _continuity_initial(x)
+    v = 0.5 * np.stack([np.sin(x[0]) * np.cos(x[1]), np.sin(x[1]) * np.cos(x[2]), np.sin(x[2]) * np.cos(x[0])])
+    with oracle.pde_type("continuity"):
+        drift = [abs(oracle.solve_state(v, m0, nt)[nt].mean() / m0.mean() - 1.0) for nt in (4, 16)]
+        vfree = 0.7 * np.stack([np.sin(x[1]), np.cos(x[2]), np.sin(x[0] + 0.3)])
+        cont = oracle.solve_state(vfree, m0, 4)
+        ones = oracle.solve_state(v, np.ones(shape), 4)[4]
+        still = oracle.solve_state(np.zeros((3,) + shape), m0, 4)
+    assert drift[1] < 1e-3 and drift[1] < drift[0]
+    assert np.abs(cont - oracle.solve_state(vfree, m0, 4)).max() < 1e-12
+    assert np.abs(ones - oracle.det_def_grad(v, 4, True)[0]).max() < 1e-12
+    assert np.array_equal(still, np.broadcast_to(m0, still.shape))
+    with pytest.raises(KeyError):
+        oracle.set_pde_type("burgers")
