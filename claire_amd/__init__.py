@@ -124,6 +124,7 @@ class GNReport(ctypes.Structure):
 
 MODELS = {"compressible": 0, "stokes": 1, "relaxed_stokes": 2}
 PDE_TYPES = {"continuity": 0, "transport": 1}
+PDE_SOLVERS = {"rk2": 0, "sl": 2}
 GN_REASONS = {0: "continue", 1: "gttol", 2: "gatol", 3: "steptol", -1: "maxits", -2: "nan", -3: "linesearch"}
 
 
@@ -202,7 +203,7 @@ class Claire:
     """One registration problem (GN, semi-Lagrangian, nc = 1) on the GPU."""
 
     def __init__(self, nx, nt=4, regnorm="h1sn", beta=(1e-2, 1e-4, 0.0), stream=None, precision="fp32",
-                 rk_order=2, graphs=False, model="compressible", pde="transport"):
+                 rk_order=2, graphs=False, model="compressible", pde="transport", pde_solver="sl"):
         self.nx = tuple(int(n) for n in nx)
         self.nt = int(nt)
         self.nl = int(np.prod(self.nx))
@@ -223,6 +224,8 @@ class Claire:
             self.set_reg_model(model)
         if pde != "transport":
             self.set_pde_type(pde)
+        if pde_solver != "sl":
+            self.set_pde_solver(pde_solver)
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -382,6 +385,13 @@ class Claire:
         if pde not in PDE_TYPES:
             raise ValueError(f"pde type must be one of {sorted(PDE_TYPES)}")
         _check(lib().clr_set_pde_type(self.ctx, ctypes.c_int(PDE_TYPES[pde])))
+
+    def set_pde_solver(self, solver):
+        """Time integrator of every later transport solve: "sl" (default) or
+        "rk2" (the reference's -pdesolver option)."""
+        if solver not in PDE_SOLVERS:
+            raise ValueError(f"pde solver must be one of {sorted(PDE_SOLVERS)}")
+        _check(lib().clr_set_pde_solver(self.ctx, ctypes.c_int(PDE_SOLVERS[solver])))
 
     def solve_gn(self, v0=None, hist_len=128, **options):
         """Gauss–Newton–Krylov registration (clr_solve_gn) from v0 (zero by

@@ -362,6 +362,10 @@ int clr_set_pde_type(clr_ctx* ctx, int type) {
     return guard([&] { C(ctx).SetPDEType(type); });
 }
 
+int clr_set_pde_solver(clr_ctx* ctx, int type) {
+    return guard([&] { C(ctx).SetPDESolver(type); });
+}
+
 int clr_gn_options_default(clr_gn_options* opt) {
     return guard([&] {
         need(opt, "opt");

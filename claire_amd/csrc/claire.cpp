@@ -585,7 +585,8 @@ size_t CLAIRE::DeviceBytes() const {
     const size_t halves = m_Opt.precision == 1 ? (2 * nt + 1) * 3 * nl : 0;
     const size_t rk4 = m_Opt.rkorder == 4 ? 3 * nl : 0;  // RK4 stage sum
     const size_t cont = m_Opt.pdetype == 0 ? nl : 0;      // continuity-equation coefficient
-    return (floats + rk4 + cont) * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
+    const size_t rk2 = m_Opt.pdesolver == 0 ? 6 * nl : 0;  // RK2 integrator work
+    return (floats + rk4 + cont + rk2) * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
 }
 
 void CLAIRE::RequireVelocity() const { CLR_REQUIRE(m_HaveVelocity, kErrState, "velocity field not set"); }
@@ -660,7 +661,8 @@ int CLAIRE::SolveStateEquation() {
     CLR_REQUIRE(m_HaveTemplate, kErrState, "template image not set");
     const int64_t nl = m_Opt.domain.nl;
     const int nt = m_Opt.nt;
-    EnsureTrajectories();
+    // the SL solves and the continuity equation need the trajectories
+    if (m_Opt.pdesolver == 2 || m_Opt.pdetype == 0) EnsureTrajectories();
     float* m = m_State.get();
     launch_copy(m_TemplateImage.get(), m, nl, m_Opt.stream);  // m^0 = mT (src/CLAIRE.cpp:1896)
     if (m_VelocityIsZero) {
@@ -673,6 +675,8 @@ int CLAIRE::SolveStateEquation() {
             KernelScope k(m_Opt.prof, "continuity_step", 24.0 * nl);
             launch_plan_adjoint_step(tp, g, m_StateCoeff.get(), m + j * nl, m + (j + 1) * nl, m_Opt.stream);
         }
+    } else if (m_Opt.pdesolver == 0) {
+        Rk2StateSteps();  // SolveStateEquationRK2 (src/CLAIRE.cpp:1995-2105)
     } else {
         for (int j = 0; j < nt; ++j)  // src/CLAIRE.cpp:2153-2163
             m_SL->Gather(m + (j + 1) * nl, m + j * nl, "state");
@@ -783,12 +787,124 @@ int CLAIRE::SetPDEType(int type) {
     return 0;
 }
 
-int CLAIRE::PrepareHessian() {
-    EnsureTrajectories();
+int CLAIRE::SetPDESolver(int type) {
+    // RK2A ("stabilized RK2") has no transport solvers in the reference:
+    // SolveStateEquation and the adjoint/incremental solves throw "PDE solver
+    // not implemented" for it (src/CLAIRE.cpp:1927-1944, 2315-2330)
+    CLR_REQUIRE(type == 0 || type == 2, kErrInvalidArgument,
+                "pde solver must be 0 (rk2) or 2 (sl); rk2a has no transport solvers");
+    if (type != m_Opt.pdesolver) {
+        m_Opt.pdesolver = type;
+        if (type == 0) m_Rk2Work.resize(static_cast<size_t>(6 * m_Opt.domain.nl));
+        m_StateValid = m_GradMValid = m_GradMXValid = m_HalfValid = false;
+        ++m_GraphEpoch;
+    }
+    return 0;
+}
+
+void CLAIRE::EnsureMatVecCaches() {
+    if (m_Opt.pdesolver == 0) {
+        EnsureStateGradients();
+        return;
+    }
     EnsureGradientsAtStateTrajectory();
     if (m_Opt.precision == 1) EnsureHalfGradients();
+}
+
+int CLAIRE::PrepareHessian() {
+    if (m_Opt.pdesolver == 2) EnsureTrajectories();
+    EnsureMatVecCaches();
     CLR_HIP_CHECK(hipStreamSynchronize(m_Opt.stream));
     return 0;
+}
+
+void CLAIRE::Rk2StateSteps() {
+    const int64_t nl = m_Opt.domain.nl;
+    const int nt = m_Opt.nt;
+    const float ht = m_Opt.GetTimeStepSize();
+    float* w = m_Rk2Work.get();
+    float* gw[3] = {w, w + nl, w + 2 * nl};
+    const Vec3Ptr G{{gw[0], gw[1], gw[2]}};
+    const Vec3Ptr V{{m_VelocityField->m_X1, m_VelocityField->m_X2, m_VelocityField->m_X3}};
+    float* rhs0 = w + 3 * nl;
+    float* mbar = w + 4 * nl;
+    float* m = m_State.get();
+    for (int j = 0; j < nt; ++j) {
+        m_Spectral->Grad(gw, m + j * nl);  // ∇m^j
+        {
+            KernelScope k(m_Opt.prof, "rk2_state_stage", 36.0 * nl);
+            launch_rk2_state_stage1(nl, ht, G, V, m + j * nl, rhs0, mbar, m_Opt.stream);
+        }
+        m_Spectral->Grad(gw, mbar);  // ∇m̄
+        KernelScope k(m_Opt.prof, "rk2_state_stage", 36.0 * nl);
+        launch_rk2_state_stage2(nl, 0.5f * ht, G, V, m + j * nl, rhs0, m + (j + 1) * nl, m_Opt.stream);
+    }
+}
+
+void CLAIRE::Rk2IncState(const VecField* vt, float* lt0_out) {
+    const int64_t nl = m_Opt.domain.nl;
+    const int nt = m_Opt.nt;
+    const float ht = m_Opt.GetTimeStepSize();
+    const Vec3Ptr VT{{vt->m_X1, vt->m_X2, vt->m_X3}};
+    auto grad_m = [&](int j) {
+        const float* gj = m_GradM.get() + 3 * static_cast<int64_t>(j) * nl;
+        return Vec3Ptr{{gj, gj + nl, gj + 2 * nl}};
+    };
+    if (m_VelocityIsZero) {
+        // src/CLAIRE.cpp:3022-3052
+        KernelScope k(m_Opt.prof, "rk2_inc_stage", 28.0 * nl);
+        launch_rk2_inc_static(nl, nt, ht, grad_m(0), VT, lt0_out, true, m_Opt.stream);
+        return;
+    }
+    float* w = m_Rk2Work.get();
+    float* gw[3] = {w, w + nl, w + 2 * nl};
+    const Vec3Ptr G{{gw[0], gw[1], gw[2]}};
+    const Vec3Ptr V{{m_VelocityField->m_X1, m_VelocityField->m_X2, m_VelocityField->m_X3}};
+    float* rhs0 = w + 3 * nl;
+    float* mtbar = w + 4 * nl;
+    float* mt = m_IncState[0].get();  // m̃^j (GN: one slot, src/CLAIRE.cpp:3074-3078)
+    for (int j = 0; j < nt; ++j) {
+        const bool first = (j == 0), last = (j == nt - 1);
+        if (!first) m_Spectral->Grad(gw, mt);  // ∇m̃^j (zero for j = 0)
+        {
+            KernelScope k(m_Opt.prof, "rk2_inc_stage", 48.0 * nl);
+            launch_rk2_inc_stage1(nl, ht, !first, G, V, grad_m(j), VT, mt, rhs0, mtbar, m_Opt.stream);
+        }
+        m_Spectral->Grad(gw, mtbar);  // ∇m̃̄
+        KernelScope k(m_Opt.prof, "rk2_inc_stage", 64.0 * nl);
+        launch_rk2_inc_stage2(nl, 0.5f * ht, !first, G, V, grad_m(j + 1), VT, mt, rhs0, last ? lt0_out : mt, last,
+                              m_Opt.stream);
+    }
+}
+
+void CLAIRE::Rk2AdjointSteps() {
+    const int64_t nl = m_Opt.domain.nl;
+    const int nt = m_Opt.nt;
+    const float ht = m_Opt.GetTimeStepSize();
+    float* w = m_Rk2Work.get();
+    float* ww[3] = {w, w + nl, w + 2 * nl};
+    const float* wc[3] = {ww[0], ww[1], ww[2]};
+    const Vec3Out W{{ww[0], ww[1], ww[2]}};
+    const Vec3Ptr V{{m_VelocityField->m_X1, m_VelocityField->m_X2, m_VelocityField->m_X3}};
+    float* rhs0 = w + 3 * nl;
+    float* rhs1 = w + 4 * nl;
+    float* lam = m_Lam.get();
+    {
+        KernelScope k(m_Opt.prof, "rk2_adj_stage", 28.0 * nl);
+        launch_rk2_adj_flux(nl, lam, V, W, m_Opt.stream);
+    }
+    for (int j = 0; j < nt; ++j) {
+        const float* lj = lam + j * nl;
+        m_Spectral->Div(rhs0, wc);  // ∇·(λ^j v)
+        {
+            KernelScope k(m_Opt.prof, "rk2_adj_stage", 32.0 * nl);
+            launch_rk2_adj_mid(nl, ht, lj, rhs0, V, W, m_Opt.stream);
+        }
+        m_Spectral->Div(rhs1, wc);  // ∇·(λ̄ v)
+        const bool next = j + 1 < nt;
+        KernelScope k(m_Opt.prof, "rk2_adj_stage", (next ? 40.0 : 16.0) * nl);
+        launch_rk2_adj_end(nl, 0.5f * ht, lj, rhs0, rhs1, lam + (j + 1) * nl, next, V, W, m_Opt.stream);
+    }
 }
 
 // λ^0 given in m_Lam slot 0; writes b (×hd) and, when lam_out is set, λ at t = 0.
@@ -803,13 +919,21 @@ void CLAIRE::IncAdjointCore(const float* lt0, VecField* bt, bool half) {
         launch_body_force_static(nl, m_Opt.GetLebesgueMeasure(), m_Lam.get(), m_GradM.get(), b, m_Opt.stream);
         return;
     }
-    const TrajectoryPlan tp = m_SL->Plan("adjoint");
-    // λ^{j+1} = c·λ^j(X_a) (src/CLAIRE.cpp:2668-2673, 3955-3962), j = 0..nt−1
-    for (int j = 0; j < nt; ++j) {
-        // bytes: plan 12 + c 4 + gathered λ 4 + output 4 per point
-        KernelScope k(m_Opt.prof, "adjoint_step", 24.0 * nl);
-        launch_plan_adjoint_step(tp, g, m_AdjCoeff.get(), m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl,
-                                 m_Opt.stream);
+    if (m_Opt.pdesolver == 0) {
+        // SolveAdjointEquationRK2 / SolveIncAdjointEquationGNRK2
+        // (src/CLAIRE.cpp:2365-2548, 3504-3683): the same λ^j·∇m^{nt−j}
+        // trapezoid body force, λ^{j+1} by RK2 steps of the flux form
+        Rk2AdjointSteps();
+        half = false;
+    } else {
+        const TrajectoryPlan tp = m_SL->Plan("adjoint");
+        // λ^{j+1} = c·λ^j(X_a) (src/CLAIRE.cpp:2668-2673, 3955-3962), j = 0..nt−1
+        for (int j = 0; j < nt; ++j) {
+            // bytes: plan 12 + c 4 + gathered λ 4 + output 4 per point
+            KernelScope k(m_Opt.prof, "adjoint_step", 24.0 * nl);
+            launch_plan_adjoint_step(tp, g, m_AdjCoeff.get(), m_Lam.get() + j * nl, m_Lam.get() + (j + 1) * nl,
+                                     m_Opt.stream);
+        }
     }
     if (half) {
         // bytes: λ^0..λ^nt 4(nt+1) + ∇m (half) 6(nt+1) + b 12
@@ -828,6 +952,12 @@ void CLAIRE::IncAdjointCore(const float* lt0, VecField* bt, bool half) {
 void CLAIRE::IncStateCore(const VecField* vt, float* out, bool half) {
     const int64_t nl = m_Opt.domain.nl;
     const int nt = m_Opt.nt;
+    if (m_Opt.pdesolver == 0) {
+        // SolveIncStateEquationRK2 (src/CLAIRE.cpp:2962-3135)
+        EnsureStateGradients();
+        Rk2IncState(vt, out);
+        return;
+    }
     EnsureGradientsAtStateTrajectory();
     const GatherDims g = gather_dims(m_Opt.domain);
     const TrajectoryPlan ts = m_SL->Plan("state");
@@ -873,7 +1003,7 @@ int CLAIRE::SolveIncStateEquation() {
 int CLAIRE::SolveIncAdjointFrom(const float* mtilde) {
     RequireVelocity();
     if (!m_StateValid) SolveStateEquation();
-    EnsureGradientsAtStateTrajectory();
+    EnsureMatVecCaches();
     const int64_t nl = m_Opt.domain.nl;
     // λ̃⁰ = −m̃ (src/DistanceMeasureSL2.cpp:266-272)
     launch_copy(mtilde, m_Lam.get(), nl, m_Opt.stream);
@@ -901,8 +1031,7 @@ int CLAIRE::HessianMatVec(float* Hvtilde, const float* vtilde, bool scale) {
     CLR_REQUIRE(Hvtilde != nullptr, kErrInvalidArgument, "null output vector");
     CLR_REQUIRE(Hvtilde != vt->m_X1, kErrInvalidArgument, "HessianMatVec cannot run in place");
     // Newton-invariant caches, computed eagerly (never inside a capture)
-    EnsureGradientsAtStateTrajectory();
-    if (m_Opt.precision == 1) EnsureHalfGradients();
+    EnsureMatVecCaches();
     if (!m_Opt.graphs || (m_Opt.prof && m_Opt.prof->enabled())) {
         HessianMatVecBody(Hvtilde, vt, scale);
         return 0;
@@ -972,7 +1101,7 @@ int CLAIRE::SetMatVecType(int type) {
 
 void CLAIRE::HessianMatVecBody(float* Hvtilde, const VecField* vt, bool scale) {
     VecField hv(&m_Opt, Hvtilde);
-    const bool half = m_Opt.precision == 1;
+    const bool half = MixedKernels();
     const float hd = m_Opt.GetLebesgueMeasure();
     const int64_t n3 = 3 * m_Opt.domain.nl;
     if (m_Opt.matvectype != 0) {

@@ -138,6 +138,9 @@ struct RegOpt {
     // Constraint of the state solve (m_PDESolver.pdetype, include/RegOpt.hpp:42-46):
     // 0 continuity equation, 1 transport equation (default, src/RegOpt.cpp:1089).
     int pdetype = 1;
+    // Time integrator of the transport solves (m_PDESolver.type,
+    // include/RegOpt.hpp:31-35): 0 RK2, 2 SL (default, src/RegOpt.cpp:1083).
+    int pdesolver = 2;
     // include/RegOpt.hpp:485-500, src/RegOpt.cpp:2505-2512
     float GetLebesgueMeasure() const { return domain.hd; }
     float GetTimeStepSize() const { return 1.0f / static_cast<float>(nt); }
@@ -330,6 +333,9 @@ class CLAIRE {
     // Constraint of every later state solve: 0 continuity, 1 transport
     // (CLAIRE::SolveStateEquation, src/CLAIRE.cpp:1923-1926).
     int SetPDEType(int type);
+    // Time integrator of every later transport solve: 0 explicit RK2
+    // ("-pdesolver rk2"), 2 semi-Lagrangian (src/RegOpt.cpp:653-658).
+    int SetPDESolver(int type);
     // Operator applied by HessianMatVec: 0 the GN Hessian (HessMatVec),
     // 1 hd·ṽ + (βA)⁻¹b̃[ṽ] (PrecondHessMatVec), 2 hd·ṽ +
     // (βA)^{-1/2}b̃[(βA)^{-1/2}ṽ] (PrecondHessMatVecSym).
@@ -359,6 +365,15 @@ class CLAIRE {
     void IncStateCore(const VecField* vt, float* lt0_out, bool half);
     void IncAdjointCore(const float* lt0, VecField* bt, bool half);
     void EnsureHalfGradients();
+    // The Newton-invariant fields the incremental solves of the selected
+    // integrator read (and their half copies in the mixed SL form).
+    void EnsureMatVecCaches();
+    bool MixedKernels() const { return m_Opt.precision == 1 && m_Opt.pdesolver == 2; }
+    // Explicit RK2 integrators (src/CLAIRE.cpp:1995-2105, 2365-2548,
+    // 2962-3135, 3504-3683); the adjoint form fills λ^1..λ^nt of m_Lam.
+    void Rk2StateSteps();
+    void Rk2IncState(const VecField* vt, float* lt0_out);
+    void Rk2AdjointSteps();
     void HessianMatVecBody(float* Hvtilde, const VecField* vt, bool scale);
     void ProjectField(VecField* b);
     double DivergencePenalty();
@@ -405,6 +420,7 @@ class CLAIRE {
     DeviceArray<float> m_Scalar;         // nl work
     DeviceArray<float> m_AdjCoeff;       // nl: adjoint update coefficient at every point
     DeviceArray<float> m_StateCoeff;     // nl: continuity-equation state coefficient (pdetype 0 only)
+    DeviceArray<float> m_Rk2Work;        // 6·nl, RK2 integrators only: a vector field, rhs0, rhs1 / m̄
     DeviceArray<float> m_VtX;            // ṽ(X_s), 3·nl
     DeviceArray<uint16_t> m_GradM16;     // half copies of m_GradM / m_GradMX (mixed form)
     DeviceArray<uint16_t> m_GradMX16;

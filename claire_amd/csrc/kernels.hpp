@@ -165,4 +165,40 @@ void launch_min(const float* a, int64_t n, double* partial, double* result, hipS
 void launch_max(const float* a, int64_t n, double* partial, double* result, hipStream_t st);
 void launch_sum(const float* a, int64_t n, double* partial, double* result, hipStream_t st);
 
+// ---------------------------------------------------------------------------
+// Explicit RK2 transport stages (rk2.hip); each follows a spectral grad or
+// div of the step (src/CLAIRE.cpp:1995-2105, 2365-2548, 2962-3135, 3504-3683)
+// ---------------------------------------------------------------------------
+struct Vec3Ptr {
+    const float* p[3];
+};
+struct Vec3Out {
+    float* p[3];
+};
+// rhs0 = −g·v, m̄ = m + ht·rhs0
+void launch_rk2_state_stage1(int64_t nl, float ht, const Vec3Ptr& g, const Vec3Ptr& v, const float* m, float* rhs0,
+                             float* mbar, hipStream_t st);
+// m⁺ = m + ht/2·(rhs0 − g·v)
+void launch_rk2_state_stage2(int64_t nl, float hthalf, const Vec3Ptr& g, const Vec3Ptr& v, const float* m,
+                             const float* rhs0, float* mnext, hipStream_t st);
+// rhs0 = −∇m̃·v − ∇m·ṽ, m̃̄ = m̃ + ht·rhs0 (has_gmt = false: m̃ = 0)
+void launch_rk2_inc_stage1(int64_t nl, float ht, bool has_gmt, const Vec3Ptr& gmt, const Vec3Ptr& v,
+                           const Vec3Ptr& gm, const Vec3Ptr& vt, const float* mt, float* rhs0, float* mtbar,
+                           hipStream_t st);
+// out = ±(m̃ + ht/2·(rhs1 + rhs0)), rhs1 = −∇m̃̄·v − ∇m⁺·ṽ (has_mt = false: m̃ = 0)
+void launch_rk2_inc_stage2(int64_t nl, float hthalf, bool has_mt, const Vec3Ptr& gmt, const Vec3Ptr& v,
+                           const Vec3Ptr& gm, const Vec3Ptr& vt, const float* mt, const float* rhs0, float* out,
+                           bool negate, hipStream_t st);
+// v = 0: out = ±m̃ after nt steps m̃ ← m̃ − ht·(∇m·ṽ) from m̃ = 0
+void launch_rk2_inc_static(int64_t nl, int nt, float ht, const Vec3Ptr& gm, const Vec3Ptr& vt, float* out, bool negate,
+                           hipStream_t st);
+// w = λ·v
+void launch_rk2_adj_flux(int64_t nl, const float* lam, const Vec3Ptr& v, const Vec3Out& w, hipStream_t st);
+// w = v·(λ + ht·rhs0)
+void launch_rk2_adj_mid(int64_t nl, float ht, const float* lam, const float* rhs0, const Vec3Ptr& v, const Vec3Out& w,
+                        hipStream_t st);
+// λ⁺ = λ + ht/2·(rhs0 + rhs1); next: w = λ⁺·v
+void launch_rk2_adj_end(int64_t nl, float hthalf, const float* lam, const float* rhs0, const float* rhs1, float* lnext,
+                        bool next, const Vec3Ptr& v, const Vec3Out& w, hipStream_t st);
+
 }  // namespace claire_amd

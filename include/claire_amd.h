@@ -127,6 +127,20 @@ int clr_set_rk_order(clr_ctx* ctx, int order);
 #define CLR_PDE_CONTINUITY 0
 #define CLR_PDE_TRANSPORT 1
 int clr_set_pde_type(clr_ctx* ctx, int type);
+/* Time integrator of every later transport solve, replacing RegOpt
+ * m_PDESolver.type (include/RegOpt.hpp:31-35), set by "-pdesolver rk2|sl"
+ * (src/RegOpt.cpp:653-658, default SL at src/RegOpt.cpp:1083):
+ *   CLR_PDE_SOLVER_RK2  explicit RK2 with spectral derivatives
+ *                       (SolveStateEquationRK2, SolveAdjointEquationRK2,
+ *                       SolveIncStateEquationRK2, SolveIncAdjointEquationGNRK2,
+ *                       src/CLAIRE.cpp:1995-2105, 2365-2548, 2962-3135,
+ *                       3504-3683); conditionally stable;
+ *   CLR_PDE_SOLVER_SL   semi-Lagrangian (default).
+ * The values are the reference enum's; its RK2A (1) has no transport solvers
+ * and is rejected.  With RK2 the mixed precision form has no effect. */
+#define CLR_PDE_SOLVER_RK2 0
+#define CLR_PDE_SOLVER_SL 2
+int clr_set_pde_solver(clr_ctx* ctx, int type);
 /* Operator applied by clr_hessian_matvec, replacing m_KrylovMethod.matvectype
  * (include/RegOpt.hpp:94-98, dispatched in CLAIRE::HessianMatVec,
  * src/CLAIRE.cpp:1251-1277):

@@ -769,11 +769,146 @@ void solve_continuity(const Grid<T>& g, int nt, const T* const v[3], const T* X,
     }
 }
 
+// Time integrator of the transport solves (m_PDESolver.type,
+// include/RegOpt.hpp:31-35): 0 explicit RK2 ("-pdesolver rk2"), 2 SL
+// (default, src/RegOpt.cpp:1083).  Test infrastructure sets it per call.
+enum PDESolverType { kRK2 = 0, kRK2A = 1, kSL = 2 };
+
+inline int& pde_solver() {
+    static int s = kSL;
+    return s;
+}
+
+// −a·b summed as −a1·b1 − a2·b2 − a3·b3.
+template <class T>
+inline T neg_dot3(const std::vector<T>* a, const T* const b[3], int64_t i) {
+    return -a[0][i] * b[0][i] - a[1][i] * b[1][i] - a[2][i] * b[2][i];
+}
+
+// SolveStateEquationRK2 (src/CLAIRE.cpp:1995-2105): for j = 0..nt−1
+//   rhs0 = −∇m^j·v, m̄ = m^j + ht·rhs0, rhs1 = −∇m̄·v,
+//   m^{j+1} = m^j + ht/2·(rhs0 + rhs1);   m^0 is already in place.
+template <class T>
+void solve_state_rk2(const Grid<T>& g, int nt, const T* const v[3], T* m) {
+    const int64_t nl = g.nl;
+    const T ht = T(1) / static_cast<T>(nt);
+    const T hthalf = static_cast<T>(0.5) * ht;
+    std::vector<T> gr[3] = {std::vector<T>(nl), std::vector<T>(nl), std::vector<T>(nl)};
+    std::vector<T> mbar(nl), rhs0(nl);
+    for (int j = 0; j < nt; ++j) {
+        const T* mj = m + static_cast<int64_t>(j) * nl;
+        T* mnext = m + static_cast<int64_t>(j + 1) * nl;
+        grad(g, mj, gr[0].data(), gr[1].data(), gr[2].data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            rhs0[i] = neg_dot3(gr, v, i);
+            mbar[i] = mj[i] + ht * rhs0[i];
+        }
+        grad(g, mbar.data(), gr[0].data(), gr[1].data(), gr[2].data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            const T rhs1 = neg_dot3(gr, v, i);
+            mnext[i] = mj[i] + hthalf * (rhs0[i] + rhs1);
+        }
+    }
+}
+
+// SolveAdjointEquationRK2 (src/CLAIRE.cpp:2365-2548) and
+// SolveIncAdjointEquationGNRK2 (src/CLAIRE.cpp:3504-3683), GN (one slot),
+// nc = 1: λ holds the final condition; for j = 0..nt−1
+//   rhs0 = ∇·(λv), λ̄ = λ + ht·rhs0, rhs1 = ∇·(λ̄v),
+//   b += w_j·∇m^{nt−j}·λ (λ before the update), λ ← λ + ht/2·(rhs0 + rhs1),
+// then b += ht/2·∇m^0·λ (in double, as written); w_0 = ht/2, else ht.
+// b is not projected or scaled here.
+template <class T>
+void adjoint_rk2(const Grid<T>& g, int nt, const T* const v[3], const T* m, T* lambda, T* const b[3]) {
+    const int64_t nl = g.nl;
+    const T ht = T(1) / static_cast<T>(nt);
+    const T hthalf = static_cast<T>(0.5) * ht;
+    std::vector<T> w[3] = {std::vector<T>(nl), std::vector<T>(nl), std::vector<T>(nl)};
+    std::vector<T> rhs0(nl), rhs1(nl);
+    T scale = ht;
+    for (int j = 0; j < nt; ++j) {
+        const T* mj = m + static_cast<int64_t>(nt - j) * nl;
+        if (j == 0) scale *= static_cast<T>(0.5);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            const T l = lambda[i];
+            for (int d = 0; d < 3; ++d) w[d][i] = l * v[d][i];
+        }
+        div(g, w[0].data(), w[1].data(), w[2].data(), rhs0.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            const T lbar = lambda[i] + ht * rhs0[i];
+            for (int d = 0; d < 3; ++d) w[d][i] = v[d][i] * lbar;
+        }
+        div(g, w[0].data(), w[1].data(), w[2].data(), rhs1.data());
+        grad(g, mj, w[0].data(), w[1].data(), w[2].data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            const T l = lambda[i];
+            lambda[i] = l + hthalf * (rhs0[i] + rhs1[i]);
+            for (int d = 0; d < 3; ++d) b[d][i] += scale * w[d][i] * l / T(1);
+        }
+        if (j == 0) scale *= static_cast<T>(2.0);
+    }
+    grad(g, m, w[0].data(), w[1].data(), w[2].data());
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nl; ++i) {
+        const T l = lambda[i];
+        for (int d = 0; d < 3; ++d) b[d][i] += 0.5 * scale * w[d][i] * l / T(1);
+    }
+}
+
+// SolveIncStateEquationRK2 (src/CLAIRE.cpp:2962-3135), GN (one slot),
+// nc = 1: m̃⁰ = 0; for j = 0..nt−1
+//   rhs0 = −∇m̃·v − ∇m^j·ṽ, m̃̄ = m̃ + ht·rhs0,
+//   rhs1 = −∇m̃̄·v − ∇m^{j+1}·ṽ, m̃ ← m̃ + ht/2·(rhs1 + rhs0);
+// for v = 0, nt steps m̃ ← m̃ − ht·(∇m^0·ṽ) (src/CLAIRE.cpp:3022-3052).
+template <class T>
+void solve_inc_state_rk2(const Grid<T>& g, int nt, const T* const v[3], const T* m, const T* const vt[3], T* mt) {
+    const int64_t nl = g.nl;
+    const T ht = T(1) / static_cast<T>(nt);
+    const T hthalf = static_cast<T>(0.5) * ht;
+    std::fill(mt, mt + nl, T(0));
+    std::vector<T> gm[3] = {std::vector<T>(nl), std::vector<T>(nl), std::vector<T>(nl)};
+    if (velocity_is_zero(g, v)) {
+        grad(g, m, gm[0].data(), gm[1].data(), gm[2].data());
+        for (int j = 0; j < nt; ++j)
+            for (int64_t i = 0; i < nl; ++i)
+                mt[i] = mt[i] - ht * (gm[0][i] * vt[0][i] + gm[1][i] * vt[1][i] + gm[2][i] * vt[2][i]);
+        return;
+    }
+    std::vector<T> gt[3] = {std::vector<T>(nl), std::vector<T>(nl), std::vector<T>(nl)};
+    std::vector<T> mtbar(nl), rhs0(nl);
+    for (int j = 0; j < nt; ++j) {
+        grad(g, m + static_cast<int64_t>(j) * nl, gm[0].data(), gm[1].data(), gm[2].data());
+        grad(g, mt, gt[0].data(), gt[1].data(), gt[2].data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            rhs0[i] = neg_dot3(gt, v, i) - gm[0][i] * vt[0][i] - gm[1][i] * vt[1][i] - gm[2][i] * vt[2][i];
+            mtbar[i] = mt[i] + ht * rhs0[i];
+        }
+        grad(g, m + static_cast<int64_t>(j + 1) * nl, gm[0].data(), gm[1].data(), gm[2].data());
+        grad(g, mtbar.data(), gt[0].data(), gt[1].data(), gt[2].data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < nl; ++i) {
+            const T rhs1 = neg_dot3(gt, v, i) - gm[0][i] * vt[0][i] - gm[1][i] * vt[1][i] - gm[2][i] * vt[2][i];
+            mt[i] = mt[i] + hthalf * (rhs1 + rhs0[i]);
+        }
+    }
+}
+
 template <class T>
 void solve_state(const Grid<T>& g, int nt, const T* const v[3], const T* mT, T* m) {
     std::copy(mT, mT + g.nl, m);
     if (velocity_is_zero(g, v)) {
         for (int j = 1; j <= nt; ++j) std::copy(mT, mT + g.nl, m + j * g.nl);
+        return;
+    }
+    // the continuity equation is always solved by SL (src/CLAIRE.cpp:1923-1926)
+    if (pde_type() != kContinuityEq && pde_solver() == kRK2) {
+        solve_state_rk2(g, nt, v, m);
         return;
     }
     const T ht = T(1) / static_cast<T>(nt);
@@ -820,6 +955,9 @@ void solve_adjoint(const Grid<T>& g, int nt, const T* const v[3], const T* m, co
             b[1][i] += l * g2[i];
             b[2][i] += l * g3[i];
         }
+    } else if (pde_solver() == kRK2) {
+        // CLAIREStokes overrides only the SL adjoint solves
+        adjoint_rk2(g, nt, v, m, lambda, b);
     } else {
         const T ht = T(1) / static_cast<T>(nt);
         const bool stokes = reg_model() == kStokes;  // CLAIREStokes: pure transport (src/CLAIREStokes.cpp:169-296)
@@ -869,7 +1007,12 @@ void solve_adjoint(const Grid<T>& g, int nt, const T* const v[3], const T* m, co
 //   m̃^{j+1} = m̃^j(X_s) − ht/2·[(∇m^j)(X_s)·ṽ(X_s)] − ht/2·[∇m^{j+1}·ṽ].
 // There is no v = 0 branch on the SL path (src/CLAIRE.cpp:2866-2950).
 template <class T>
-void solve_inc_state_x(const Grid<T>& g, int nt, const T* X, const T* m, const T* const vt[3], T* mtilde) {
+void solve_inc_state_x(const Grid<T>& g, int nt, const T* const v[3], const T* X, const T* m, const T* const vt[3],
+                       T* mtilde) {
+    if (pde_solver() == kRK2) {
+        solve_inc_state_rk2(g, nt, v, m, vt, mtilde);
+        return;
+    }
     const int64_t nl = g.nl;
     const T ht = T(1) / static_cast<T>(nt);
     const T hthalf = static_cast<T>(0.5) * ht;
@@ -898,8 +1041,8 @@ void solve_inc_state_x(const Grid<T>& g, int nt, const T* X, const T* m, const T
 template <class T>
 void solve_inc_state(const Grid<T>& g, int nt, const T* const v[3], const T* m, const T* const vt[3], T* mtilde) {
     std::vector<T> X(3 * g.nl);
-    trajectory(g, v, T(1) / static_cast<T>(nt), +1, X.data());
-    solve_inc_state_x(g, nt, X.data(), m, vt, mtilde);
+    if (pde_solver() != kRK2) trajectory(g, v, T(1) / static_cast<T>(nt), +1, X.data());
+    solve_inc_state_x(g, nt, v, X.data(), m, vt, mtilde);
 }
 
 // Incremental adjoint equation, GN, SL, with the incremental body force
@@ -922,6 +1065,8 @@ void solve_inc_adjoint_x(const Grid<T>& g, int nt, const T* const v[3], const T*
             bt[1][i] += l * g2[i];
             bt[2][i] += l * g3[i];
         }
+    } else if (pde_solver() == kRK2) {
+        adjoint_rk2(g, nt, v, m, ltilde, bt);
     } else {
         const T ht = T(1) / static_cast<T>(nt);
         const T hthalf = static_cast<T>(0.5) * ht;
@@ -971,7 +1116,7 @@ template <class T>
 void solve_inc_adjoint(const Grid<T>& g, int nt, const T* const v[3], const T* m, const T* mtilde, T* ltilde,
                        T* const bt[3]) {
     std::vector<T> X;
-    if (!velocity_is_zero(g, v)) {
+    if (!velocity_is_zero(g, v) && pde_solver() != kRK2) {
         X.resize(3 * g.nl);
         trajectory(g, v, T(1) / static_cast<T>(nt), -1, X.data());
     }
@@ -1000,7 +1145,7 @@ void hessian_matvec_x(const Grid<T>& g, int nt, int norm, const T* beta, const T
             reg_inverse(g, norm, beta, vt, w, true);
             for (int d = 0; d < 3; ++d) in[d] = w[d];
         }
-        solve_inc_state_x(g, nt, Xs, m, in, mt.data());
+        solve_inc_state_x(g, nt, v, Xs, m, in, mt.data());
         solve_inc_adjoint_x(g, nt, v, Xa, m, mt.data(), lt.data(), bt);
         const T* btc[3] = {bt[0], bt[1], bt[2]};
         reg_inverse(g, norm, beta, btc, hv, sym);
@@ -1012,7 +1157,7 @@ void hessian_matvec_x(const Grid<T>& g, int nt, int norm, const T* beta, const T
         }
         return;
     }
-    solve_inc_state_x(g, nt, Xs, m, vt, mt.data());
+    solve_inc_state_x(g, nt, v, Xs, m, vt, mt.data());
     solve_inc_adjoint_x(g, nt, v, Xa, m, mt.data(), lt.data(), bt);
     reg_apply(g, norm, beta, vt, hv);
     for (int d = 0; d < 3; ++d)

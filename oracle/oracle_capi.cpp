@@ -60,6 +60,17 @@ int oracle_set_pde_type(int t) {
     return 0;
 }
 
+// Time integrator of every later transport solve: 0 RK2, 2 SL
+// (RegOpt m_PDESolver.type).
+int oracle_set_pde_solver(int t) {
+    if (t != 0 && t != 2) {
+        g_err = "pde solver must be 0 (rk2) or 2 (sl)";
+        return 1;
+    }
+    oracle::pde_solver() = t;
+    return 0;
+}
+
 // Hessian operator type of every later matvec (0 default, 1 preconditioned,
 // 2 symmetric preconditioned).
 int oracle_set_matvec_type(int t) {

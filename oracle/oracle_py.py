@@ -96,6 +96,31 @@ class pde_type:
         return False
 
 
+PDE_SOLVERS = {"rk2": 0, "sl": 2}
+
+
+def set_pde_solver(name: str) -> None:
+    """Time integrator of every later transport solve: "sl" (default) or
+    "rk2"."""
+    if lib().oracle_set_pde_solver(PDE_SOLVERS[name]) != 0:
+        raise ValueError(lib().oracle_last_error().decode())
+
+
+class pde_solver:
+    """Context manager that runs a block with the given time integrator."""
+
+    def __init__(self, name: str):
+        self.name = name
+
+    def __enter__(self):
+        set_pde_solver(self.name)
+        return self
+
+    def __exit__(self, *exc):
+        set_pde_solver("sl")
+        return False
+
+
 MODELS = {"compressible": 0, "stokes": 1, "relaxed_stokes": 2}
 
 

@@ -56,3 +56,4 @@ def test_rk_order_and_graph_setters_reject_a_null_context():
     assert lib.clr_set_rk_order(None, 4) == 1
     assert lib.clr_set_matvec_graphs(None, 1) == 1
     assert lib.clr_set_pde_type(None, 0) == 1
+    assert lib.clr_set_pde_solver(None, 0) == 1
