@@ -112,7 +112,7 @@ class GNOptions(ctypes.Structure):
     """clr_gn_options (include/claire_amd.h)."""
     _fields_ = [("maxit", ctypes.c_int), ("gttol", ctypes.c_double), ("gatol", ctypes.c_double),
                 ("krylov_maxit", ctypes.c_int), ("krylov_rtol", ctypes.c_double), ("forcing", ctypes.c_int),
-                ("precond", ctypes.c_int), ("linesearch", ctypes.c_int)]
+                ("precond", ctypes.c_int), ("linesearch", ctypes.c_int), ("monitor", ctypes.c_int)]
 
 
 class GNReport(ctypes.Structure):

@@ -70,6 +70,7 @@ reg::GNOptions gn_options(const clr_gn_options& opt) {
     o.krylov_rtol = opt.krylov_rtol;
     o.forcing = opt.forcing;
     o.precond = opt.precond;
+    o.monitor = opt.monitor;
     o.linesearch = opt.linesearch;
     return o;
 }
@@ -371,7 +372,7 @@ int clr_gn_options_default(clr_gn_options* opt) {
         need(opt, "opt");
         const reg::GNOptions d;
         *opt = clr_gn_options{d.maxit, d.gttol, d.gatol, d.krylov_maxit, d.krylov_rtol, d.forcing, d.precond,
-                              d.linesearch};
+                              d.linesearch, d.monitor};
     });
 }
 
@@ -404,7 +405,8 @@ int clr_solve_gn_batch(clr_ctx* const* ctxs, float* const* v, int count, const c
             CLR_REQUIRE(seen.insert(ctxs[i]).second, claire_amd::kErrInvalidArgument,
                         "a context appears twice in the batch");
         }
-        const reg::GNOptions o = gn_options(*opt);
+        reg::GNOptions o = gn_options(*opt);
+        o.monitor = 0;  // concurrent solves would interleave their tables
         int dev = 0;
         CLR_HIP_CHECK(hipGetDevice(&dev));
         std::atomic<int> next{0};

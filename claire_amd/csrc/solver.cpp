@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <string>
 
 namespace reg {
 
@@ -21,6 +23,34 @@ Optimizer::Optimizer(CLAIRE* problem)
 }
 
 double Optimizer::Dot(const float* a, const float* b) { return m_Prob->Red()->Dot(a, b, m_N); }
+
+namespace {
+
+// The iteration table of the reference: the header printed by
+// CLAIREInterface::Run (src/CLAIREInterface.cpp:722-726, separators of
+// m_LineLength = 101 dashes, src/RegOpt.cpp:1276) and one line per
+// iteration from OptimizationMonitor (src/TaoInterface.cpp:584-588):
+// iteration, J/J0, D/D0, ‖g‖/‖g0‖, ‖g‖ and the step length, each ratio's
+// denominator replaced by 1 when it is not positive.
+void monitor_header() {
+    const std::string line(101, '-');
+    std::printf("%s\n %s  %-20s %-20s %-20s %-20s %-20s\n%s\n", line.c_str(), "iter", "objective (rel)",
+                "mismatch (rel)", "||gradient||_2,rel", "||gradient||_2", "step", line.c_str());
+    std::fflush(stdout);
+}
+
+void monitor_line(int iter, double J, double J0, double D, double D0, double g, double g0, double step) {
+    J0 = J0 > 0.0 ? J0 : 1.0;
+    D0 = D0 > 0.0 ? D0 : 1.0;
+    g0 = g0 > 0.0 ? g0 : 1.0;
+    char msg[256];
+    std::snprintf(msg, sizeof(msg), "  %03d  %-20.12E %-20.12E %-20.12E %-20.12E %.6f", iter, J / J0, D / D0, g / g0,
+                  g, step);
+    std::printf("%-80s\n", msg);
+    std::fflush(stdout);
+}
+
+}  // namespace
 
 // KSPCG on H d = −g, zero initial guess, unpreconditioned residual
 // norm: converged when ‖r‖ <= max(rtol·‖b‖, 1e-16), diverged when
@@ -123,11 +153,15 @@ int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, 
     double kg0 = -1.0;  // ‖g‖ at the first Krylov solve (KrylovMethod.g0norm)
     double step = 1.0;
     int iter = 0;
+    if (opt.monitor) monitor_header();
     for (;;) {
         double* h = (hist && iter < hist_len) ? hist + 8 * iter : nullptr;
         if (h) {
             h[0] = iter; h[1] = J; h[2] = D; h[3] = R; h[4] = gnorm; h[5] = gnorm / g0; h[6] = 0; h[7] = 0;
         }
+        // TaoMonitor runs before the convergence test, the step length being
+        // the last line search's (1 before the first)
+        if (opt.monitor) monitor_line(iter, J, rep.J0, D, rep.D0, gnorm, rep.gnorm0, step);
         // CheckConvergenceGrad (src/TaoInterface.cpp:421-480)
         if (!std::isfinite(J) || !std::isfinite(gnorm)) { rep.reason = kGNNan; break; }
         if (step < 1.0 / 1024.0) { rep.reason = kGNSteptol; break; }

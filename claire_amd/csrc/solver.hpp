@@ -22,6 +22,7 @@ struct GNOptions {
     int forcing = 2;             // 0 none, 1 quadratic, 2 superlinear (SLFS)
     int precond = 1;             // 0 none, 1 inverse regularization (INVREG)
     int linesearch = 1;          // 0 unit step, 1 Armijo
+    int monitor = 0;             // 1: print the iteration table (OptimizationMonitor)
 };
 
 struct GNReport {

@@ -265,6 +265,12 @@ typedef struct clr_gn_options {
     int forcing;         /* 0 none, 1 quadratic, 2 superlinear (default) */
     int precond;         /* 0 none, 1 inverse regularization (default) */
     int linesearch;      /* 0 unit step, 1 Armijo (default) */
+    int monitor;         /* 0 silent (default); 1 print the reference's iteration
+                            table to stdout: the header of CLAIREInterface::Run
+                            (src/CLAIREInterface.cpp:722-726) and one line per
+                            iteration as OptimizationMonitor
+                            (src/TaoInterface.cpp:525-592); batch solves stay
+                            silent */
 } clr_gn_options;
 
 #define CLR_GN_GTTOL 1      /* ‖g‖ < gttol·‖g₀‖ (TAO_CONVERGED_GTTOL) */

@@ -124,3 +124,35 @@ def test_phantom_registration_matches_fp64_golden(name, precision):
     assert abs(mis - ref["mismatch"]) <= 1e-3 * ref["mismatch"]
     assert abs(red - red64) <= 1e-3 * red64 or abs(red - red64) <= 1e-3 * case["gttol"]
     assert abs(rep["J"] - ref["J"]) <= 1e-3 * ref["J"]
+
+
+def test_gn_monitor_prints_the_reference_table(capfd, oracle):
+    """monitor = 1 prints the reference's iteration table (header of
+    src/CLAIREInterface.cpp:722-726, lines of src/TaoInterface.cpp:584-588):
+    one line per iteration with J/J0, D/D0, ‖g‖/‖g0‖, ‖g‖ and the step,
+    agreeing with the returned history; the default stays silent."""
+    shape = (16, 16, 16)
+    print(f"\n[gpu case] GN monitor {shape}", flush=True)
+    mT, mR = _problem(shape, oracle)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA)
+    try:
+        c.set_template_image(mT.astype(np.float32))
+        c.set_reference_image(mR.astype(np.float32))
+        capfd.readouterr()
+        c.solve_gn(maxit=5)
+        assert capfd.readouterr().out == ""
+        v, rep, hist = c.solve_gn(maxit=5, monitor=1)
+        out = capfd.readouterr().out.splitlines()
+    finally:
+        c.close()
+    assert out[0] == "-" * 101 and out[2] == "-" * 101
+    assert out[1].split() == ["iter", "objective", "(rel)", "mismatch", "(rel)", "||gradient||_2,rel",
+                              "||gradient||_2", "step"]
+    rows = [line.split() for line in out[3:]]
+    assert len(rows) == rep["iterations"] + 1
+    for k, r in enumerate(rows):
+        assert len(out[3 + k]) >= 80 and r[0] == f"{k:03d}"
+        it, J, D, R, g, grel, its, step = hist[k]
+        expect = (J / rep["J0"], D / rep["D0"], grel, g, 1.0 if k == 0 else hist[k - 1][7])
+        for got, want in zip(map(float, r[1:]), expect):
+            assert abs(got - want) <= 1e-10 * abs(want) + 5e-7, (k, got, want)
