@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--no-batched", action="store_true",
                    help="skip the 48-registration batched leg")
     p.add_argument("--batch-workers", type=int, default=8)
+    p.add_argument("--no-variants", action="store_true",
+                   help="skip the option variants (matvec types, models, RK2/RK4, continuity) at the same size")
     p.add_argument("--large", action="store_true",
                    help="add the 512^3 leg (40-ellipsoid phantom, nt=8): footprint, matvec/s, 5 GN iterations")
     p.add_argument("--fp64", action="store_true", help="also score GPU and CPU FP32 against the FP64 oracle")
@@ -223,6 +225,91 @@ def batched_leg(args):
         for c in problems:
             c.close()
     return res
+
+
+def variants_leg(c, v_slow, vt, hv, args):
+    """The reference's option variants on the same 256³ FP32 context, each
+    timed over a few steps after its own setup: the matvec operator types,
+    the two other registration models, the RK2 integrator (on a CFL 0.5
+    velocity: it is conditionally stable), the RK4 characteristic (its cost
+    is in the trajectories, so the timed step is the trajectory pair plus the
+    state solve) and the continuity-equation state solve.  Each entry gives
+    ms per step and the dominant kernel class with its algorithmic GB/s
+    (HIP events around every launch)."""
+    import claire_amd
+    from claire_amd import DeviceArray
+
+    steps = max(2, min(args.steps, 8))
+    lib = claire_amd.lib()
+    m = DeviceArray((args.nt + 1,) + c.nx)
+
+    def matvec():
+        c.hessian_matvec_device(hv, vt)
+
+    def state():
+        claire_amd._check(lib.clr_solve_state(c.ctx, claire_amd._vp(m.ptr)))
+
+    def setup_and_state():
+        # a new velocity drops the trajectories; vt holds the benchmark's v
+        claire_amd._check(lib.clr_set_velocity(c.ctx, claire_amd._vp(vt.ptr)))
+        state()
+
+    def timed(name, step, prepare=None):
+        if prepare:
+            prepare()
+        step()
+        c.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        c.sync()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        c.profile_reset()
+        c.profile_enable(True)
+        step()
+        c.profile_enable(False)
+        prof = c.profile()
+        dname, (dms, dcnt, dby) = max(prof.items(), key=lambda kv: kv[1][0])
+        res = {"ms_per_step": round(ms, 3), "per_s": round(1e3 / ms, 2), "dominant": dname,
+               "dominant_GBps": round(dby / (dms * 1e-3) / 1e9, 1), "dominant_share": round(
+                   dms / sum(p[0] for p in prof.values()), 3),
+               "kernels_us": {k: round(1e3 * p[0] / p[1], 1) for k, p in
+                              sorted(prof.items(), key=lambda kv: -kv[1][0])[:6]}}
+        log(f"variant {name}: {ms:.3f} ms, dominant {dname} {res['dominant_GBps']} GB/s")
+        return res
+
+    out = {}
+    try:
+        for kind in ("precond_sym", "precond"):
+            c.set_matvec_type(kind)
+            out[f"matvec_{kind}"] = timed(f"matvec {kind}", matvec, c.prepare_hessian)
+        c.set_matvec_type("default")
+        for model in ("stokes", "relaxed_stokes"):
+            c.set_reg_model(model)
+            out[f"matvec_{model}"] = timed(f"matvec {model}", matvec, c.prepare_hessian)
+        c.set_reg_model("compressible")
+        c.set_pde_type("continuity")
+        out["state_continuity"] = timed("continuity state", state, c.prepare_hessian)
+        c.set_pde_type("transport")
+        out["state_transport"] = timed("transport state", state, c.prepare_hessian)
+        c.set_rk_order(4)
+        out["trajectories_and_state_rk4"] = timed("rk4 trajectories + state", setup_and_state)
+        c.set_rk_order(2)
+        out["trajectories_and_state_rk2"] = timed("rk2 trajectories + state", setup_and_state)
+        c.set_pde_solver("rk2")
+        c.set_velocity(v_slow)
+        out["matvec_pdesolver_rk2"] = timed("matvec -pdesolver rk2", matvec, c.prepare_hessian)
+        out["matvec_pdesolver_rk2"]["velocity"] = "band-limited |k|<=8, CFL 0.5"
+    finally:
+        c.set_pde_solver("sl")
+        c.set_rk_order(2)
+        c.set_pde_type("transport")
+        c.set_reg_model("compressible")
+        c.set_matvec_type("default")
+        claire_amd._check(lib.clr_set_velocity(c.ctx, claire_amd._vp(vt.ptr)))
+        m.free()
+    out["steps"] = steps
+    return out
 
 
 def large_leg(args):
@@ -430,6 +517,13 @@ def main():
                 H64 = oracle_py.hessian_matvec(v, m64, v, nt, regnorm, beta, prec="f64")
                 out["relerr_vs_fp64"] = {"gpu_fp32": rel(Hgpu, H64), "cpu_fp32": rel(Hcpu, H64),
                                          "gpu_mixed": rel(Hmix, H64)}
+    if not args.no_variants:
+        v_slow = synth.bandlimited_velocity(shape, kmax=8, cfl=0.5, nt=nt, seed=5)
+
+        def variants(a):
+            return variants_leg(c, v_slow, vt, hv, a)
+
+        out["variants"] = guarded(variants, args)
     if not args.no_registration:
         out["registration"] = guarded(registration_leg, args)
     if not args.no_batched:

@@ -442,8 +442,32 @@ int SemiLagrangian::ComputeTrajectory(const VecField* v, const std::string& flag
                               m_Opt->stream);
     } else {
         CLR_REQUIRE(m_Opt->rkorder == 2, kErrInvalidArgument, "rk order must be 2 or 4");
+        const int64_t nl = m_Opt->domain.nl;
+        const GatherDims g = gather_dims(m_Opt->domain);
+        m_P[w].resize(3 * static_cast<size_t>(nl));
+        int dpred = -1;
+        {
+            // the Euler predictor's plan: v read (12) + plan written (12) per point
+            KernelScope k(m_Opt->prof, "gather_traj_rk2_plan", 24.0 * nl);
+            dpred = launch_rk2_predictor_plan(m_Opt->domain, vc, sigma, m_Opt->GetTimeStepSize(), m_P[w].get(),
+                                              m_Scratch.get(), m_Opt->stream);
+        }
+        if (dpred >= 0 && dpred <= lds_gather_max_displacement() && vc[1] - vc[0] == nl && vc[2] - vc[1] == nl) {
+            m_W.resize(3 * static_cast<size_t>(nl));
+            {
+                // 3 × (plan 12 + gathered v 4 + v at the point 4 + X component 4) per point
+                KernelScope k(m_Opt->prof, "gather_traj_rk2", 72.0 * nl);
+                launch_lds_traj_rk2_corrector(m_Opt->domain, m_P[w].get(), dpred, vc, sigma,
+                                              m_Opt->GetTimeStepSize(), m_W.get(), m_Opt->stream);
+            }
+            // bytes: X components read (12) + X interleaved (12) + plan (12) per point
+            KernelScope k(m_Opt->prof, "gather_plan", 36.0 * nl);
+            m_D[w] = launch_make_plan_soa(g, m_W.get(), m_X[w].get(), m_P[w].get(), m_Scratch.get(), m_Opt->stream);
+            m_Valid[w] = true;
+            return 0;
+        }
         // bytes: v read (12) + v gathered (12) + X written (12) per point
-        KernelScope k(m_Opt->prof, "gather_traj_rk2", 36.0 * m_Opt->domain.nl);
+        KernelScope k(m_Opt->prof, "gather_traj_rk2", 36.0 * nl);
         launch_trajectory_rk2(m_Opt->domain, vc, sigma, m_Opt->GetTimeStepSize(), m_X[w].get(), m_Opt->stream);
     }
     m_P[w].resize(3 * static_cast<size_t>(m_Opt->domain.nl));
@@ -583,7 +607,7 @@ size_t CLAIRE::DeviceBytes() const {
     const size_t floats = 5 * 3 * nl + 2 * nl + (nt + 1) * nl + nl + 3 * nl + 4 * nl + (nt + 1) * 3 * nl +
                           nt * 3 * nl + nt * nl + nt * nl + nl + 2 * 3 * nl;
     const size_t halves = m_Opt.precision == 1 ? (2 * nt + 1) * 3 * nl : 0;
-    const size_t rk4 = m_Opt.rkorder == 4 ? 3 * nl : 0;  // RK4 stage sum
+    const size_t rk4 = 3 * nl;  // RK4 stage sum, or the RK2 corrector output
     const size_t cont = m_Opt.pdetype == 0 ? nl : 0;      // continuity-equation coefficient
     const size_t rk2 = m_Opt.pdesolver == 0 ? 6 * nl : 0;  // RK2 integrator work
     return (floats + rk4 + cont + rk2) * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
@@ -637,10 +661,10 @@ void CLAIRE::EnsureTrajectories() {
         // incompressible adjoints are pure transport (src/CLAIREStokes.cpp:169-406): c = 1
         launch_fill(m_AdjCoeff.get(), 1.0f, m_Opt.domain.nl, m_Opt.stream);
     } else {
-        // bytes: X 12 + div v gathered 4 + div v 4 + C 4 per point
+        // bytes: plan 12 + div v gathered 4 + div v 4 + C 4 per point
         KernelScope k(m_Opt.prof, "gather_adjoint_coeff", 24.0 * m_Opt.domain.nl);
-        launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Plan("adjoint").X, m_Scalar.get(),
-                             m_Opt.GetTimeStepSize(), 1.0f, m_AdjCoeff.get(), m_Opt.stream);
+        launch_plan_adjoint_coeff(m_SL->Plan("adjoint"), gather_dims(m_Opt.domain), m_Scalar.get(),
+                                  m_Opt.GetTimeStepSize(), 1.0f, m_AdjCoeff.get(), m_Opt.stream);
     }
     if (m_Opt.pdetype == 0) {
         // continuity-equation state update m^{j+1} = c·m^j(X_s) with
@@ -649,8 +673,8 @@ void CLAIRE::EnsureTrajectories() {
         // (SolveContinuityEquationSL, src/CLAIRE.cpp:2813-2835)
         m_StateCoeff.resize(static_cast<size_t>(m_Opt.domain.nl));
         KernelScope k(m_Opt.prof, "gather_state_coeff", 24.0 * m_Opt.domain.nl);
-        launch_adjoint_coeff(gather_dims(m_Opt.domain), m_SL->Plan("state").X, m_Scalar.get(),
-                             m_Opt.GetTimeStepSize(), -1.0f, m_StateCoeff.get(), m_Opt.stream);
+        launch_plan_adjoint_coeff(m_SL->Plan("state"), gather_dims(m_Opt.domain), m_Scalar.get(),
+                                  m_Opt.GetTimeStepSize(), -1.0f, m_StateCoeff.get(), m_Opt.stream);
     }
     m_TrajValid = true;
     ++m_GraphEpoch;
@@ -742,8 +766,8 @@ int CLAIRE::ComputeDetDefGrad(bool inverse, float* jac, double stats[3]) {
     const TrajectoryPlan tp = m_SL->Plan("state");
     {
         KernelScope k(m_Opt.prof, "gather_det_coeff", 24.0 * nl);
-        launch_adjoint_coeff(g, tp.X, divv.get(), m_Opt.GetTimeStepSize(), inverse ? -1.0f : 1.0f, coeff.get(),
-                             m_Opt.stream);
+        launch_plan_adjoint_coeff(tp, g, divv.get(), m_Opt.GetTimeStepSize(), inverse ? -1.0f : 1.0f, coeff.get(),
+                                  m_Opt.stream);
     }
     float* buf[2] = {m_IncState[0].get(), m_IncState[1].get()};
     launch_fill(buf[0], 1.0f, nl, m_Opt.stream);

@@ -284,6 +284,7 @@ class SemiLagrangian {
     DeviceArray<float> m_X[2];
     DeviceArray<uint32_t> m_P[2];
     DeviceArray<float> m_F;  // RK4 stage sum f (m_WorkVecField2 of the reference)
+    DeviceArray<float> m_W;  // RK2 corrector output, one component per field (LDS path)
     DeviceArray<int> m_Scratch;
     bool m_Valid[2] = {false, false};
     int m_D[2] = {0, 0};

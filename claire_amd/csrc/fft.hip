@@ -483,9 +483,13 @@ struct ColGeom {
     int64_t ncols, inner, outer, estride;
 };
 
+// Offset of column cg's first element; column counts and the inner extent
+// fit 32 bits (64-bit division is a long emulated sequence).
 template <int N>
 __device__ __forceinline__ int64_t col_base(const ColGeom& g, int64_t cg) {
-    return (cg / g.inner) * g.outer + (cg % g.inner);
+    const uint32_t c = static_cast<uint32_t>(cg), in = static_cast<uint32_t>(g.inner);
+    const uint32_t q = c / in;
+    return static_cast<int64_t>(q) * g.outer + static_cast<int64_t>(c - q * in);
 }
 
 template <int N, int DIR>
@@ -573,7 +577,8 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_x1_fused(Ptr3 ptr, ColGeom 
     const int64_t cg = static_cast<int64_t>(blockIdx.x) * G::B + c;
     const bool valid = cg < geo.ncols;
     const int64_t base = valid ? cg : 0;
-    const int k2 = static_cast<int>(cg / nh), k3 = static_cast<int>(cg % nh);
+    const int k2 = static_cast<int>(static_cast<uint32_t>(cg) / static_cast<uint32_t>(nh));
+    const int k3 = static_cast<int>(cg) - k2 * nh;
     const int w2 = wavenum(k2, n2), w3 = wavenum(k3, n3);
 
     auto load_fwd = [&](const c32* src, c32 (&v)[V]) {

@@ -179,10 +179,13 @@ __device__ __forceinline__ bool plan_encode(float u, int idx, int n, uint32_t& w
 // Epilogues of the trajectory-based gathers.  Direct kernels use aux(i) (the
 // per-point operand of the update) and store(i, f(X), aux); the LDS kernel
 // reads auxp() and writes apply(f(X), aux) to outp() itself, and kHasAux
-// says whether there is an operand at all.
+// says whether there is an operand at all.  Epilogues with kCoord need the
+// point's grid indices and provide apply_at(f(X), aux, i1, i2, i3) instead
+// (LDS kernel only).
 // ---------------------------------------------------------------------------
 struct EpiPlain {  // out = f(X)
     static constexpr bool kHasAux = false;
+    static constexpr bool kCoord = false;
     float* out;
     int64_t ozstride;
     __device__ void select(int z) { out += z * ozstride; }
@@ -195,6 +198,7 @@ struct EpiPlain {  // out = f(X)
 
 struct EpiIncStep {  // out = sign·(f(X) − s)   (src/CLAIRE.cpp:3228-3273)
     static constexpr bool kHasAux = true;
+    static constexpr bool kCoord = false;
     const float* s;
     float* out;
     float sign;
@@ -208,6 +212,7 @@ struct EpiIncStep {  // out = sign·(f(X) − s)   (src/CLAIRE.cpp:3228-3273)
 
 struct EpiAdjStep {  // out = c·f(X)   (src/CLAIRE.cpp:2668-2673, 3955-3962)
     static constexpr bool kHasAux = true;
+    static constexpr bool kCoord = false;
     const float* c;
     float* out;
     __device__ void select(int) {}
@@ -216,6 +221,56 @@ struct EpiAdjStep {  // out = c·f(X)   (src/CLAIRE.cpp:2668-2673, 3955-3962)
     __device__ float apply(float v, float cv) const { return cv * v; }
     __device__ float aux(int64_t i) const { return c[i]; }
     __device__ void store(int64_t i, float v, float cv) const { out[i] = cv * v; }
+};
+
+// Update coefficient c = 1 + ht/2·(α·dX + α·(1 + ht·α·dX)·d) from the
+// gathered dX = d(X) and d = ∇·v at the point (the expression of
+// k_adjoint_coeff: the SL adjoint step for α = 1 on the adjoint trajectory,
+// det(∇y^{±1}) and the continuity equation for α = ±1 on the state one).
+struct EpiAdjCoeff {
+    static constexpr bool kHasAux = true;
+    static constexpr bool kCoord = false;
+    const float* d;
+    float* out;
+    float ht, alpha;
+    __device__ void select(int) {}
+    __device__ const float* auxp() const { return d; }
+    __device__ float* outp() const { return out; }
+    __device__ float apply(float dx, float dv) const {
+        const float hthalf = 0.5f * ht;
+        const float adx = alpha * dx;
+        return 1.0f + hthalf * (adx + alpha * (1.0f + ht * adx) * dv);
+    }
+    __device__ float aux(int64_t i) const { return d[i]; }
+    __device__ void store(int64_t i, float v, float dv) const { out[i] = apply(v, dv); }
+};
+
+// Corrector of the RK2 characteristic (src/SemiLagrangian.cpp:235-278) for
+// component z = blockIdx.y, gathered along the Euler predictor's plan:
+// X_z = (x_z − σ·ht/2·(v_z(X̃) + v_z(x)))/2π, stored per component (out +
+// z·zstride); aux is v_z at the point.  Same expressions as k_traj_rk2.
+struct EpiTrajRk2 {
+    static constexpr bool kHasAux = true;
+    static constexpr bool kCoord = true;
+    const float* v;
+    float* out;
+    int64_t zstride;
+    float hx[3];
+    float shth;  // σ·ht/2
+    int z;
+    __device__ void select(int zz) {
+        z = zz;
+        v += zz * zstride;
+        out += zz * zstride;
+    }
+    __device__ const float* auxp() const { return v; }
+    __device__ float* outp() const { return out; }
+    __device__ float apply_at(float vx, float vl, int i1, int i2, int i3) const {
+        const int idx = z == 0 ? i1 : (z == 1 ? i2 : i3);
+        const float x = hx[z] * static_cast<float>(idx);
+        const float t = x - shth * (vx + vl);
+        return static_cast<float>(static_cast<double>(t) / (2.0 * 3.14159265358979323846));
+    }
 };
 
 }  // namespace gather

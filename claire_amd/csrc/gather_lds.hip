@@ -160,10 +160,12 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
     uint32_t off[G::PTS];
     int rc4[G::PTS];
     bool valid[G::PTS];
+    int x3s[G::PTS];
 #pragma unroll
     for (int k = 0; k < G::PTS; ++k) {
         const int lx3 = (lane & 31) + 32 * k;
         const int x3 = t3 * G::LT3 + lx3;
+        x3s[k] = x3;
         valid[k] = x2 < g.n2 && x3 < g.n3;
         off[k] = valid[k] ? 4u * static_cast<uint32_t>(x2 * g.n3 + x3) : 0u;  // bytes of a float
         rc4[k] = 4 * ((lx2 + G::HLO) * G::PITCH + lx3 + G::HLO);
@@ -263,8 +265,13 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
         {
             const uint32_t pb = static_cast<uint32_t>(x1) * plane4;
 #pragma unroll
-            for (int k = 0; k < G::PTS; ++k)
-                if (valid[k]) bstore(ro, epi.apply(v[k], caux[k]), off[k], pb);
+            for (int k = 0; k < G::PTS; ++k) {
+                if (!valid[k]) continue;
+                if constexpr (Epi::kCoord)
+                    bstore(ro, epi.apply_at(v[k], caux[k], x1, x2, x3s[k]), off[k], pb);
+                else
+                    bstore(ro, epi.apply(v[k], caux[k]), off[k], pb);
+            }
         }
 
         // land the prefetched plane (x1 + HHI + 1) in the spare ring slot
@@ -560,6 +567,22 @@ void launch_plan_adjoint_step(const TrajectoryPlan& tp, const GatherDims& g, con
     if (!use_lds(tp)) return launch_direct_adjoint_step(g, tp.X, C, lam, out, st);
     CLR_REQUIRE(lam != out, kErrInvalidArgument, "semi-Lagrangian step cannot run in place");
     dispatch(tp.dbound, g, tp.P, lam, 0, 1, EpiAdjStep{C, out}, st);
+}
+
+void launch_plan_adjoint_coeff(const TrajectoryPlan& tp, const GatherDims& g, const float* divv, float ht, float alpha,
+                               float* C, hipStream_t st) {
+    if (!use_lds(tp)) return launch_adjoint_coeff(g, tp.X, divv, ht, alpha, C, st);
+    CLR_REQUIRE(divv != C, kErrInvalidArgument, "coefficient cannot overwrite ∇·v");
+    dispatch(tp.dbound, g, tp.P, divv, 0, 1, EpiAdjCoeff{divv, C, ht, alpha}, st);
+}
+
+void launch_lds_traj_rk2_corrector(const Dims& d, const uint32_t* P, int dbound, const float* const v[3], float sigma,
+                                   float ht, float* W, hipStream_t st) {
+    const GatherDims g = gather_dims(d);
+    const int64_t vz = v[1] - v[0];
+    CLR_REQUIRE(v[2] - v[1] == vz && vz == g.nl, kErrInvalidArgument, "velocity components must be contiguous");
+    EpiTrajRk2 e{v[0], W, g.nl, {d.hx[0], d.hx[1], d.hx[2]}, sigma * (0.5f * ht), 0};
+    dispatch(dbound, g, P, v[0], vz, 3, e, st);
 }
 
 template <class G>

@@ -11,6 +11,8 @@
 //   * direct gathers (one thread per point, stencil rows read from global
 //     memory as unaligned 16-byte loads) for arbitrary query points and for
 //     trajectories whose displacement exceeds the LDS window.
+#include <algorithm>
+
 #include "gather_device.hpp"
 
 namespace claire_amd {
@@ -23,6 +25,19 @@ constexpr int kThreads = 256;
 
 __device__ __forceinline__ int64_t point_index(int nblocks) {
     return static_cast<int64_t>(xcd_remap(blockIdx.x, nblocks)) * kThreads + threadIdx.x;
+}
+
+// Grid indices (i1, i2, i3) of flat point i in 32-bit arithmetic (the
+// gathers require nl < 2^31; 64-bit division is a long emulated sequence).
+__device__ __forceinline__ void grid_index(const GatherDims& g, int64_t i, int idx[3]) {
+    const uint32_t ui = static_cast<uint32_t>(i);
+    const uint32_t plane = static_cast<uint32_t>(g.n2) * static_cast<uint32_t>(g.n3);
+    const uint32_t i1 = ui / plane;
+    const uint32_t r = ui - i1 * plane;
+    const uint32_t i2 = r / static_cast<uint32_t>(g.n3);
+    idx[0] = static_cast<int>(i1);
+    idx[1] = static_cast<int>(i2);
+    idx[2] = static_cast<int>(r - i2 * static_cast<uint32_t>(g.n3));
 }
 
 __device__ __forceinline__ void load_q3(const float* X, int64_t i, float q[3]) {
@@ -77,12 +92,10 @@ __global__ __launch_bounds__(kThreads) void k_traj_rk2(GatherDims g, HH hh, Traj
                                                        float* __restrict__ X, int nblocks) {
     const int64_t i = point_index(nblocks);
     if (i >= g.nl) return;
-    const int64_t plane = static_cast<int64_t>(g.n2) * g.n3;
-    const int i1 = static_cast<int>(i / plane);
-    const int i2 = static_cast<int>((i / g.n3) % g.n2);
-    const int i3 = static_cast<int>(i % g.n3);
-    const float x[3] = {tp.hx[0] * static_cast<float>(i1), tp.hx[1] * static_cast<float>(i2),
-                        tp.hx[2] * static_cast<float>(i3)};
+    int idx[3];
+    grid_index(g, i, idx);
+    const float x[3] = {tp.hx[0] * static_cast<float>(idx[0]), tp.hx[1] * static_cast<float>(idx[1]),
+                        tp.hx[2] * static_cast<float>(idx[2])};
     const double twopi = 2.0 * 3.14159265358979323846;
     float vl[3], q[3];
 #pragma unroll
@@ -119,13 +132,11 @@ struct Rk4Params {
 };
 
 __device__ __forceinline__ void grid_point(const GatherDims& g, const float hx[3], int64_t i, float x[3]) {
-    const int64_t plane = static_cast<int64_t>(g.n2) * g.n3;
-    const int i1 = static_cast<int>(i / plane);
-    const int i2 = static_cast<int>((i / g.n3) % g.n2);
-    const int i3 = static_cast<int>(i % g.n3);
-    x[0] = hx[0] * static_cast<float>(i1);
-    x[1] = hx[1] * static_cast<float>(i2);
-    x[2] = hx[2] * static_cast<float>(i3);
+    int idx[3];
+    grid_index(g, i, idx);
+    x[0] = hx[0] * static_cast<float>(idx[0]);
+    x[1] = hx[1] * static_cast<float>(idx[1]);
+    x[2] = hx[2] * static_cast<float>(idx[2]);
 }
 
 template <int STAGE>
@@ -168,35 +179,8 @@ __global__ __launch_bounds__(kThreads) void k_traj_rk4(GatherDims g, HH hh, Rk4P
     }
 }
 
-// Packed plan of a trajectory (gather_device.hpp) and the smallest D such
-// that every stencil lies within [i − D − 1, i + D + 2] per axis (the window
-// of the LDS-staged gathers).  Points that cannot be encoded (non-finite
-// coordinates) set D to 1 << 20, which routes every gather along this
-// trajectory to the direct kernels on the raw foot points.
-__global__ __launch_bounds__(kThreads) void k_make_plan(GatherDims g, HH hh, const float* __restrict__ X,
-                                                        PlanPt* __restrict__ P, int* __restrict__ dmax, int nblocks) {
-    __shared__ int sh[kThreads];
-    const int64_t i = point_index(nblocks);
-    int need = 0;
-    if (i < g.nl) {
-        const int64_t plane = static_cast<int64_t>(g.n2) * g.n3;
-        const int idx[3] = {static_cast<int>(i / plane), static_cast<int>((i / g.n3) % g.n2),
-                            static_cast<int>(i % g.n3)};
-        const int n[3] = {g.n1, g.n2, g.n3};
-        PlanPt pt;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const float u = index_coord(X[3 * i + a], n[a], hh.h[a]);
-            int lo = 0;
-            if (!plan_encode(u, idx[a], n[a], pt.w[a], lo)) {
-                need = 1 << 20;
-                pt.w[a] = 0u;  // t = 1, o = 0: never read
-                continue;
-            }
-            need = max(need, max(-lo - 1, lo + 1));
-        }
-        P[i] = pt;
-    }
+// Block maximum of `need` into *dmax (one atomic per block).
+__device__ __forceinline__ void block_max_to(int need, int* sh, int* dmax) {
     sh[threadIdx.x] = need;
     __syncthreads();
     for (int s = kThreads / 2; s > 0; s >>= 1) {
@@ -206,7 +190,100 @@ __global__ __launch_bounds__(kThreads) void k_make_plan(GatherDims g, HH hh, con
     if (threadIdx.x == 0) atomicMax(dmax, sh[0]);
 }
 
+// Encodes the three axes of foot point q at grid index idx; returns the
+// displacement bound this point needs (1 << 20 when not encodable).
+__device__ __forceinline__ int encode_point(const GatherDims& g, const HH& hh, const float q[3], const int idx[3],
+                                            PlanPt& pt) {
+    const int n[3] = {g.n1, g.n2, g.n3};
+    int need = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float u = index_coord(q[a], n[a], hh.h[a]);
+        int lo = 0;
+        if (!plan_encode(u, idx[a], n[a], pt.w[a], lo)) {
+            need = 1 << 20;
+            pt.w[a] = 0u;  // t = 1, o = 0: never read
+            continue;
+        }
+        need = max(need, max(-lo - 1, lo + 1));
+    }
+    return need;
+}
+
+// Packed plan of a trajectory (gather_device.hpp) and the smallest D such
+// that every stencil lies within [i − D − 1, i + D + 2] per axis (the window
+// of the LDS-staged gathers).  Points that cannot be encoded (non-finite
+// coordinates) set D to 1 << 20, which routes every gather along this
+// trajectory to the direct kernels on the raw foot points.  A bounded grid
+// strides over the points, so that the blocks' maxima meet in few atomics
+// (one per block: tens of thousands on one address serialize).
+constexpr int kPlanBlocks = 2048;
+
+__global__ __launch_bounds__(kThreads) void k_make_plan(GatherDims g, HH hh, const float* __restrict__ X,
+                                                        PlanPt* __restrict__ P, int* __restrict__ dmax) {
+    __shared__ int sh[kThreads];
+    int need = 0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < g.nl; i += stride) {
+        int idx[3];
+        grid_index(g, i, idx);
+        float q[3];
+        load_q3(X, i, q);
+        PlanPt pt;
+        need = max(need, encode_point(g, hh, q, idx, pt));
+        P[i] = pt;
+    }
+    block_max_to(need, sh, dmax);
+}
+
 __global__ void k_zero_int(int* p) { *p = 0; }
+
+// Plan of the RK2 Euler predictor X̃ = (x − σ·ht·v)/2π, with the same float
+// and double steps as k_traj_rk2 (bounded grid, like k_make_plan).
+__global__ __launch_bounds__(kThreads) void k_predictor_plan(GatherDims g, HH hh, TrajParams tp, P3 v,
+                                                             PlanPt* __restrict__ P, int* __restrict__ dmax) {
+    __shared__ int sh[kThreads];
+    int need = 0;
+    const double twopi = 2.0 * 3.14159265358979323846;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < g.nl; i += stride) {
+        int idx[3];
+        grid_index(g, i, idx);
+        float q[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float x = tp.hx[d] * static_cast<float>(idx[d]);
+            const float t = x - tp.sht * v.in[d][i];
+            q[d] = static_cast<float>(static_cast<double>(t) / twopi);
+        }
+        PlanPt pt;
+        need = max(need, encode_point(g, hh, q, idx, pt));
+        P[i] = pt;
+    }
+    block_max_to(need, sh, dmax);
+}
+
+// Plan of foot points given per component (W + d·nl), which are also
+// written interleaved into X (3 floats per point).
+__global__ __launch_bounds__(kThreads) void k_make_plan_soa(GatherDims g, HH hh, const float* __restrict__ W,
+                                                            float* __restrict__ X, PlanPt* __restrict__ P,
+                                                            int* __restrict__ dmax) {
+    __shared__ int sh[kThreads];
+    int need = 0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < g.nl; i += stride) {
+        int idx[3];
+        grid_index(g, i, idx);
+        const float q[3] = {W[i], W[g.nl + i], W[2 * g.nl + i]};
+        X[3 * i + 0] = q[0];
+        X[3 * i + 1] = q[1];
+        X[3 * i + 2] = q[2];
+        PlanPt pt;
+        need = max(need, encode_point(g, hh, q, idx, pt));
+        P[i] = pt;
+    }
+    block_max_to(need, sh, dmax);
+}
 
 // Update coefficient c = 1 + ht/2·(α·dX + α·(1 + ht·α·dX)·d), d = ∇·v,
 // dX = d(X).  With α = 1 on the adjoint trajectory, the SL adjoint step
@@ -331,10 +408,45 @@ int launch_make_plan(const GatherDims& g, const float* X, uint32_t* P, int* scra
     check_gather_dims(g);
     CLR_REQUIRE(g.n1 <= kPlanMaxN && g.n2 <= kPlanMaxN && g.n3 <= kPlanMaxN, kErrUnsupportedSize,
                 "trajectory plans support at most 512 points per axis");
-    const int nb = nblocks_for(g.nl);
+    const int nb = std::min(nblocks_for(g.nl), kPlanBlocks);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, scratch);
     hipLaunchKernelGGL(k_make_plan, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), X, reinterpret_cast<PlanPt*>(P),
-                       scratch, nb);
+                       scratch);
+    int h = 0;
+    CLR_HIP_CHECK(hipMemcpyAsync(&h, scratch, sizeof(int), hipMemcpyDeviceToHost, st));
+    CLR_HIP_CHECK(hipStreamSynchronize(st));
+    return h;
+}
+
+int launch_rk2_predictor_plan(const Dims& d, const float* const v[3], float sigma, float ht, uint32_t* P,
+                              int* scratch, hipStream_t st) {
+    const GatherDims g = gather_dims(d);
+    check_gather_dims(g);
+    CLR_REQUIRE(g.n1 <= kPlanMaxN && g.n2 <= kPlanMaxN && g.n3 <= kPlanMaxN, kErrUnsupportedSize,
+                "trajectory plans support at most 512 points per axis");
+    TrajParams tp;
+    for (int a = 0; a < 3; ++a) tp.hx[a] = d.hx[a];
+    tp.sht = sigma * ht;
+    tp.shth = sigma * (0.5f * ht);
+    P3 p{{v[0], v[1], v[2]}, {nullptr, nullptr, nullptr}};
+    const int nb = std::min(nblocks_for(g.nl), kPlanBlocks);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, scratch);
+    hipLaunchKernelGGL(k_predictor_plan, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), tp, p,
+                       reinterpret_cast<PlanPt*>(P), scratch);
+    int h = 0;
+    CLR_HIP_CHECK(hipMemcpyAsync(&h, scratch, sizeof(int), hipMemcpyDeviceToHost, st));
+    CLR_HIP_CHECK(hipStreamSynchronize(st));
+    return h;
+}
+
+int launch_make_plan_soa(const GatherDims& g, const float* W, float* X, uint32_t* P, int* scratch, hipStream_t st) {
+    check_gather_dims(g);
+    CLR_REQUIRE(g.n1 <= kPlanMaxN && g.n2 <= kPlanMaxN && g.n3 <= kPlanMaxN, kErrUnsupportedSize,
+                "trajectory plans support at most 512 points per axis");
+    const int nb = std::min(nblocks_for(g.nl), kPlanBlocks);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, scratch);
+    hipLaunchKernelGGL(k_make_plan_soa, dim3(nb), dim3(kThreads), 0, st, g, make_hh(g), W, X,
+                       reinterpret_cast<PlanPt*>(P), scratch);
     int h = 0;
     CLR_HIP_CHECK(hipMemcpyAsync(&h, scratch, sizeof(int), hipMemcpyDeviceToHost, st));
     CLR_HIP_CHECK(hipStreamSynchronize(st));

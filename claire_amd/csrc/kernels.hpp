@@ -84,6 +84,20 @@ void launch_trajectory_rk4(const Dims& d, const float* const v[3], float sigma, 
 // the stream); 1 << 20 when a foot point is not finite.  n <= 512 per axis.
 // `scratch` is one device int.
 int launch_make_plan(const GatherDims& g, const float* X, uint32_t* P, int* scratch, hipStream_t st);
+// RK2 characteristic through the LDS-staged gather, in three passes:
+//   1. the packed plan of the Euler predictor X̃ = (x − σ·ht·v)/2π into P
+//      (launch_rk2_predictor_plan, which returns its displacement bound);
+//   2. v gathered along it with the corrector as epilogue,
+//      X = (x − σ·ht/2·(v(X̃) + v))/2π, one component per field of W (3·nl
+//      floats; launch_lds_traj_rk2_corrector, bound <= 4);
+//   3. X interleaved into X (3 floats per point) and its plan into P
+//      (launch_make_plan_soa, which returns the bound of X).
+// The foot points equal those of launch_trajectory_rk2.
+int launch_rk2_predictor_plan(const Dims& d, const float* const v[3], float sigma, float ht, uint32_t* P,
+                              int* scratch, hipStream_t st);
+void launch_lds_traj_rk2_corrector(const Dims& d, const uint32_t* P, int dbound, const float* const v[3], float sigma,
+                                   float ht, float* W, hipStream_t st);
+int launch_make_plan_soa(const GatherDims& g, const float* W, float* X, uint32_t* P, int* scratch, hipStream_t st);
 // The LDS-staged gathers need D <= lds_gather_max_displacement().
 int lds_gather_max_displacement();
 // Update coefficient per point: C = 1 + ht/2·(α·dX + α·(1 + ht·α·dX)·d)
@@ -113,6 +127,10 @@ void launch_plan_inc_state_step(const TrajectoryPlan& tp, const GatherDims& g, c
                                 float* out, int negate, hipStream_t st);
 void launch_plan_adjoint_step(const TrajectoryPlan& tp, const GatherDims& g, const float* C, const float* lam,
                               float* out, hipStream_t st);
+// launch_adjoint_coeff along a trajectory plan (LDS-staged when its bound
+// allows); divv and C must not alias.
+void launch_plan_adjoint_coeff(const TrajectoryPlan& tp, const GatherDims& g, const float* divv, float ht, float alpha,
+                               float* C, hipStream_t st);
 // Direct versions on the raw foot points X (any displacement):
 void launch_direct_gather(const GatherDims& g, const float* X, const float* const f[], int nf, float* const out[],
                           hipStream_t st);

@@ -74,22 +74,61 @@ __device__ double block_reduce(double v, double* sh) {
 }
 
 template <int OP>
+__device__ __forceinline__ double red_step(double acc, float x, float y) {
+    if constexpr (OP == kRedDot) {
+        return acc + static_cast<double>(x) * static_cast<double>(y);
+    } else if constexpr (OP == kRedAbsMax) {
+        return fmax(acc, fabs(static_cast<double>(x)));
+    } else if constexpr (OP == kRedSqDist) {
+        const double d = static_cast<double>(x - y);
+        return acc + d * d;
+    } else {
+        return red_combine<OP>(acc, static_cast<double>(x));
+    }
+}
+
+// Each thread folds its elements in index order into one FP64 accumulator;
+// the grid and the order are fixed, so the result does not depend on
+// scheduling.  VEC = true reads 16-byte vectors (both operands 16-byte
+// aligned), two of them in flight per thread; the n mod 4 tail elements go
+// to the first threads of block 0.
+template <int OP, bool VEC>
 __global__ __launch_bounds__(kThreads) void k_reduce_partial(const float* __restrict__ a, const float* __restrict__ b,
                                                              int64_t n, double* __restrict__ partial) {
     __shared__ double sh[kThreads];
     double acc = red_init<OP>();
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
-        if constexpr (OP == kRedDot) {
-            acc += static_cast<double>(a[i]) * static_cast<double>(b[i]);
-        } else if constexpr (OP == kRedAbsMax) {
-            acc = fmax(acc, fabs(static_cast<double>(a[i])));
-        } else if constexpr (OP == kRedSqDist) {
-            const double d = static_cast<double>(a[i] - b[i]);
-            acc += d * d;
-        } else {
-            acc = red_combine<OP>(acc, static_cast<double>(a[i]));
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    constexpr bool kB = (OP == kRedDot || OP == kRedSqDist);
+    if constexpr (VEC) {
+        const int64_t n4 = n >> 2;
+        const float4* a4 = reinterpret_cast<const float4*>(a);
+        const float4* b4 = reinterpret_cast<const float4*>(kB ? b : a);
+        int64_t i = first;
+        for (; i + stride < n4; i += 2 * stride) {
+            const float4 x0 = a4[i], x1 = a4[i + stride];
+            const float4 y0 = kB ? b4[i] : x0, y1 = kB ? b4[i + stride] : x1;
+            acc = red_step<OP>(acc, x0.x, y0.x);
+            acc = red_step<OP>(acc, x0.y, y0.y);
+            acc = red_step<OP>(acc, x0.z, y0.z);
+            acc = red_step<OP>(acc, x0.w, y0.w);
+            acc = red_step<OP>(acc, x1.x, y1.x);
+            acc = red_step<OP>(acc, x1.y, y1.y);
+            acc = red_step<OP>(acc, x1.z, y1.z);
+            acc = red_step<OP>(acc, x1.w, y1.w);
         }
+        if (i < n4) {
+            const float4 x0 = a4[i];
+            const float4 y0 = kB ? b4[i] : x0;
+            acc = red_step<OP>(acc, x0.x, y0.x);
+            acc = red_step<OP>(acc, x0.y, y0.y);
+            acc = red_step<OP>(acc, x0.z, y0.z);
+            acc = red_step<OP>(acc, x0.w, y0.w);
+        }
+        const int64_t tail = 4 * n4 + threadIdx.x;
+        if (blockIdx.x == 0 && tail < n) acc = red_step<OP>(acc, a[tail], kB ? b[tail] : 0.0f);
+    } else {
+        for (int64_t i = first; i < n; i += stride) acc = red_step<OP>(acc, a[i], kB ? b[i] : 0.0f);
     }
     const double r = block_reduce<OP>(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
@@ -107,7 +146,12 @@ __global__ __launch_bounds__(kThreads) void k_reduce_final(const double* __restr
 
 template <int OP>
 void reduce(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce_partial<OP>, dim3(kReduceBlocks), dim3(kThreads), 0, st, a, b, n, partial);
+    const uintptr_t align = reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b ? b : a);
+    if ((align & 15u) == 0)
+        hipLaunchKernelGGL((k_reduce_partial<OP, true>), dim3(kReduceBlocks), dim3(kThreads), 0, st, a, b, n, partial);
+    else
+        hipLaunchKernelGGL((k_reduce_partial<OP, false>), dim3(kReduceBlocks), dim3(kThreads), 0, st, a, b, n,
+                           partial);
     hipLaunchKernelGGL(k_reduce_final<OP>, dim3(1), dim3(kThreads), 0, st, partial, kReduceBlocks, result);
 }
 

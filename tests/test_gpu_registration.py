@@ -100,7 +100,14 @@ def test_phantom_registration_matches_fp64_golden(name, precision):
     results (tests/golden/make_registration_golden.py): 128³ with H2SN
     β = 1e-3, and 64³ with RegularizationH2 as written, whose line search
     fails after one Newton step in the oracle as well.  Plain FP32 and the
-    mixed form: stop reason, final mismatch, ‖g‖/‖g₀‖ and J at 3 digits."""
+    mixed form: stop reason, final mismatch, ‖g‖/‖g₀‖ and J at 3 digits.
+
+    In the H2 case the second line search finds no descent in FP64, and in
+    FP32 whether a trial step α < 2⁻¹⁰ passes the Armijo test is decided by
+    round-off in J (the required decrease 1e-4·α·|gᵀd| is below FP32 noise in
+    the state): an FP32 solve then either fails the line search or accepts
+    such a step and stops on the step-length test one iteration later.  Both
+    are the same stall; the end state is held to the same 3 digits."""
     case, ref = _GOLDEN[name]["case"], _GOLDEN[name]["oracle_fp64"]
     print(f"\n[gpu case] phantom registration {name} {precision}", flush=True)
     mT, mR = _GOLDEN_MOD.inputs(case)
@@ -120,7 +127,10 @@ def test_phantom_registration_matches_fp64_golden(name, precision):
           f"|g|/|g0| {red:.6e} J {rep['J']:.6e} ({rep['reason']})")
     print(f"fp64 oracle: its {ref['iterations']} pcg {ref['pcg_iterations']} mismatch {ref['mismatch']:.6e} "
           f"|g|/|g0| {red64:.6e} J {ref['J']:.6e} ({ref['reason']})")
-    assert rep["reason"] == ref["reason"]
+    if ref["reason"] == "linesearch":
+        assert rep["reason"] in ("linesearch", "steptol") and rep["iterations"] <= ref["iterations"] + 1
+    else:
+        assert rep["reason"] == ref["reason"]
     assert abs(mis - ref["mismatch"]) <= 1e-3 * ref["mismatch"]
     assert abs(red - red64) <= 1e-3 * red64 or abs(red - red64) <= 1e-3 * case["gttol"]
     assert abs(rep["J"] - ref["J"]) <= 1e-3 * ref["J"]
