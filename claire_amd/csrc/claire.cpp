@@ -434,7 +434,38 @@ int SemiLagrangian::ComputeTrajectory(const VecField* v, const std::string& flag
     const float* vc[3] = {v->m_X1, v->m_X2, v->m_X3};
     const float sigma = w == 0 ? 1.0f : -1.0f;
     if (m_Opt->rkorder == 4) {
-        m_F.resize(3 * static_cast<size_t>(m_Opt->domain.nl));
+        const int64_t nl = m_Opt->domain.nl;
+        const GatherDims g = gather_dims(m_Opt->domain);
+        const float ht = m_Opt->GetTimeStepSize();
+        m_F.resize(3 * static_cast<size_t>(nl));
+        m_P[w].resize(3 * static_cast<size_t>(nl));
+        const int dmax = lds_gather_max_displacement();
+        int db = -1;
+        {
+            KernelScope k(m_Opt->prof, "gather_traj_rk4_plan", 24.0 * nl);
+            db = launch_rk2_predictor_plan(m_Opt->domain, vc, sigma, 0.5f * ht, m_P[w].get(), m_Scratch.get(),
+                                           m_Opt->stream);
+        }
+        if (vc[1] - vc[0] == nl && vc[2] - vc[1] == nl) {
+            m_W.resize(3 * static_cast<size_t>(nl));
+            for (int stage = 1; stage <= 3 && db >= 0 && db <= dmax; ++stage) {
+                {
+                    // 3 × (plan 12 + gathered v 4 + aux 4 + next point 4 (+ f 4)) per point
+                    KernelScope k(m_Opt->prof, "gather_traj_rk4", (stage < 3 ? 84.0 : 72.0) * nl);
+                    launch_lds_traj_rk4_stage(stage, m_Opt->domain, m_P[w].get(), db, vc, sigma, ht, m_F.get(),
+                                              m_W.get(), m_Opt->stream);
+                }
+                // bytes: point components read (12) + plan (12) (+ X interleaved (12) at the end)
+                KernelScope k(m_Opt->prof, "gather_plan", (stage < 3 ? 24.0 : 36.0) * nl);
+                db = launch_make_plan_soa(g, m_W.get(), stage == 3 ? m_X[w].get() : nullptr, m_P[w].get(),
+                                          m_Scratch.get(), m_Opt->stream);
+                if (stage == 3) {
+                    m_D[w] = db;
+                    m_Valid[w] = true;
+                    return 0;
+                }
+            }
+        }
         // bytes: v read (12), v gathered 3× (36), f and X written, read and
         // rewritten (12 + 12 + 24 + 24 + 12 + 12) per point
         KernelScope k(m_Opt->prof, "gather_traj_rk4", 144.0 * m_Opt->domain.nl);

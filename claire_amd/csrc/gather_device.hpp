@@ -180,12 +180,14 @@ __device__ __forceinline__ bool plan_encode(float u, int idx, int n, uint32_t& w
 // per-point operand of the update) and store(i, f(X), aux); the LDS kernel
 // reads auxp() and writes apply(f(X), aux) to outp() itself, and kHasAux
 // says whether there is an operand at all.  Epilogues with kCoord need the
-// point's grid indices and provide apply_at(f(X), aux, i1, i2, i3) instead
-// (LDS kernel only).
+// point's grid indices and provide apply_at(f(X), aux, i1, i2, i3) instead;
+// kOut2 adds a second output, apply2(f(X), aux) into outp2() (LDS kernel
+// only).
 // ---------------------------------------------------------------------------
 struct EpiPlain {  // out = f(X)
     static constexpr bool kHasAux = false;
     static constexpr bool kCoord = false;
+    static constexpr bool kOut2 = false;
     float* out;
     int64_t ozstride;
     __device__ void select(int z) { out += z * ozstride; }
@@ -199,6 +201,7 @@ struct EpiPlain {  // out = f(X)
 struct EpiIncStep {  // out = sign·(f(X) − s)   (src/CLAIRE.cpp:3228-3273)
     static constexpr bool kHasAux = true;
     static constexpr bool kCoord = false;
+    static constexpr bool kOut2 = false;
     const float* s;
     float* out;
     float sign;
@@ -213,6 +216,7 @@ struct EpiIncStep {  // out = sign·(f(X) − s)   (src/CLAIRE.cpp:3228-3273)
 struct EpiAdjStep {  // out = c·f(X)   (src/CLAIRE.cpp:2668-2673, 3955-3962)
     static constexpr bool kHasAux = true;
     static constexpr bool kCoord = false;
+    static constexpr bool kOut2 = false;
     const float* c;
     float* out;
     __device__ void select(int) {}
@@ -230,6 +234,7 @@ struct EpiAdjStep {  // out = c·f(X)   (src/CLAIRE.cpp:2668-2673, 3955-3962)
 struct EpiAdjCoeff {
     static constexpr bool kHasAux = true;
     static constexpr bool kCoord = false;
+    static constexpr bool kOut2 = false;
     const float* d;
     float* out;
     float ht, alpha;
@@ -252,6 +257,7 @@ struct EpiAdjCoeff {
 struct EpiTrajRk2 {
     static constexpr bool kHasAux = true;
     static constexpr bool kCoord = true;
+    static constexpr bool kOut2 = false;
     const float* v;
     float* out;
     int64_t zstride;
@@ -270,6 +276,52 @@ struct EpiTrajRk2 {
         const float x = hx[z] * static_cast<float>(idx);
         const float t = x - shth * (vx + vl);
         return static_cast<float>(static_cast<double>(t) / (2.0 * 3.14159265358979323846));
+    }
+};
+
+// Stages of the RK4 characteristic (src/SemiLagrangian.cpp:297-459) for
+// component z, gathered along the plan of the previous stage point, with the
+// float/double steps of k_traj_rk4:
+//   STAGE 1 (aux v_z):  f = v + 2·v(X¹) (double), out X² = (x − σ·ht/2·v(X¹))/2π;
+//   STAGE 2 (aux f_z):  f += 2·v(X²) (double),    out X³ = (x − σ·ht·v(X²))/2π;
+//   STAGE 3 (aux f_z):  out X = (x − σ·(ht/6)·(f + v(X³)))/2π (double).
+// out holds the next point per component (out + z·zstride), out2 = f.
+template <int STAGE>
+struct EpiTrajRk4 {
+    static constexpr bool kHasAux = true;
+    static constexpr bool kCoord = true;
+    static constexpr bool kOut2 = STAGE < 3;
+    const float* a;  // v (stage 1) or f
+    float* out;
+    float* f;
+    int64_t zstride;
+    float hx[3];
+    float step;  // σ·ht/2 (stage 1), σ·ht (stage 2)
+    double s6;   // σ·(ht/6) (stage 3)
+    int z;
+    __device__ void select(int zz) {
+        z = zz;
+        a += zz * zstride;
+        out += zz * zstride;
+        f += zz * zstride;
+    }
+    __device__ const float* auxp() const { return a; }
+    __device__ float* outp() const { return out; }
+    __device__ float* outp2() const { return f; }
+    __device__ float apply_at(float vx, float av, int i1, int i2, int i3) const {
+        const int idx = z == 0 ? i1 : (z == 1 ? i2 : i3);
+        const float x = hx[z] * static_cast<float>(idx);
+        constexpr double twopi = 2.0 * 3.14159265358979323846;
+        if constexpr (STAGE < 3) {
+            const float t = x - step * vx;
+            return static_cast<float>(static_cast<double>(t) / twopi);
+        } else {
+            const float fs = av + vx;
+            return static_cast<float>((static_cast<double>(x) - s6 * static_cast<double>(fs)) / twopi);
+        }
+    }
+    __device__ float apply2(float vx, float av) const {
+        return static_cast<float>(static_cast<double>(av) + 2.0 * static_cast<double>(vx));
     }
 };
 

@@ -109,6 +109,14 @@ __device__ __forceinline__ int opaque(int x) {
     return x;
 }
 
+template <class Epi>
+__device__ __forceinline__ float* out2_of(const Epi& e) {
+    if constexpr (Epi::kOut2)
+        return e.outp2();
+    else
+        return e.outp();
+}
+
 template <int D, class Epi>
 __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGeo<D>::WAVES_PER_EU))) void k_gather_lds(
     GatherDims g, const PlanPt* __restrict__ plan, const float* __restrict__ fbase, int64_t fzstride, Epi epi,
@@ -130,6 +138,7 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
     const rsrc_t rp = make_rsrc(plan, 3u * fbytes);
     const rsrc_t ra = make_rsrc(Epi::kHasAux ? epi.auxp() : f, fbytes);
     const rsrc_t ro = make_rsrc(epi.outp(), fbytes);
+    const rsrc_t ro2 = make_rsrc(out2_of(epi), fbytes);
     const uint32_t plane4 = static_cast<uint32_t>(g.n2) * static_cast<uint32_t>(g.n3) * 4u;  // bytes per plane
     const int x1s = ch * chunk;
     const int x1e = min(x1s + chunk, g.n1);
@@ -271,6 +280,7 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
                     bstore(ro, epi.apply_at(v[k], caux[k], x1, x2, x3s[k]), off[k], pb);
                 else
                     bstore(ro, epi.apply(v[k], caux[k]), off[k], pb);
+                if constexpr (Epi::kOut2) bstore(ro2, epi.apply2(v[k], caux[k]), off[k], pb);
             }
         }
 
@@ -583,6 +593,32 @@ void launch_lds_traj_rk2_corrector(const Dims& d, const uint32_t* P, int dbound,
     CLR_REQUIRE(v[2] - v[1] == vz && vz == g.nl, kErrInvalidArgument, "velocity components must be contiguous");
     EpiTrajRk2 e{v[0], W, g.nl, {d.hx[0], d.hx[1], d.hx[2]}, sigma * (0.5f * ht), 0};
     dispatch(dbound, g, P, v[0], vz, 3, e, st);
+}
+
+void launch_lds_traj_rk4_stage(int stage, const Dims& d, const uint32_t* P, int dbound, const float* const v[3],
+                               float sigma, float ht, float* F, float* W, hipStream_t st) {
+    const GatherDims g = gather_dims(d);
+    const int64_t vz = v[1] - v[0];
+    CLR_REQUIRE(v[2] - v[1] == vz && vz == g.nl, kErrInvalidArgument, "velocity components must be contiguous");
+    const float hx[3] = {d.hx[0], d.hx[1], d.hx[2]};
+    const float hthalf = 0.5f * ht;
+    const double s6 = static_cast<double>(sigma) * (static_cast<double>(ht) / 6.0);
+    switch (stage) {
+        case 1: {
+            EpiTrajRk4<1> e{v[0], W, F, g.nl, {hx[0], hx[1], hx[2]}, sigma * hthalf, s6, 0};
+            return dispatch(dbound, g, P, v[0], vz, 3, e, st);
+        }
+        case 2: {
+            EpiTrajRk4<2> e{F, W, F, g.nl, {hx[0], hx[1], hx[2]}, sigma * ht, s6, 0};
+            return dispatch(dbound, g, P, v[0], vz, 3, e, st);
+        }
+        case 3: {
+            EpiTrajRk4<3> e{F, W, F, g.nl, {hx[0], hx[1], hx[2]}, 0.0f, s6, 0};
+            return dispatch(dbound, g, P, v[0], vz, 3, e, st);
+        }
+        default:
+            throw Error(kErrInvalidArgument, "RK4 stage must be 1, 2 or 3");
+    }
 }
 
 template <class G>

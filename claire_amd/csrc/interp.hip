@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void k_predictor_plan(GatherDims g, HH hh
 }
 
 // Plan of foot points given per component (W + d·nl), which are also
-// written interleaved into X (3 floats per point).
+// written interleaved into X (3 floats per point) unless X is null.
 __global__ __launch_bounds__(kThreads) void k_make_plan_soa(GatherDims g, HH hh, const float* __restrict__ W,
                                                             float* __restrict__ X, PlanPt* __restrict__ P,
                                                             int* __restrict__ dmax) {
@@ -275,9 +275,11 @@ __global__ __launch_bounds__(kThreads) void k_make_plan_soa(GatherDims g, HH hh,
         int idx[3];
         grid_index(g, i, idx);
         const float q[3] = {W[i], W[g.nl + i], W[2 * g.nl + i]};
-        X[3 * i + 0] = q[0];
-        X[3 * i + 1] = q[1];
-        X[3 * i + 2] = q[2];
+        if (X) {
+            X[3 * i + 0] = q[0];
+            X[3 * i + 1] = q[1];
+            X[3 * i + 2] = q[2];
+        }
         PlanPt pt;
         need = max(need, encode_point(g, hh, q, idx, pt));
         P[i] = pt;
