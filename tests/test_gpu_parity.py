@@ -162,6 +162,46 @@ def test_hessian_matvec(case, scale, oracle):
     assert np.array_equal(c.hessian_matvec(vt, scale=scale), H)
 
 
+@pytest.mark.parametrize("nt", [1, 2, 3, 8])
+def test_time_step_counts(nt, oracle):
+    """Other time step counts than 4 (the trapezoid weights, the first and
+    last incremental steps, nt = 1 where the incremental state is one
+    right-hand side): state, adjoint body force, gradient and the Hessian
+    matvec in FP32 and in the mixed form, against the oracle run with the
+    same nt.  The mixed form's error is the FP16 rounding of the stored
+    gradient fields (unit round-off 2⁻¹² ≈ 2.4e-4), averaged over the nt + 1
+    time levels: measured 2.2e-4 at nt = 1, 1.6e-4 at 2, 1.3e-4 at 3 and
+    8.1e-5 at 8 on this case."""
+    shape = (32, 16, 64)
+    print(f"\n[gpu case] nt={nt} shape={shape}", flush=True)
+    v = synth.bandlimited_velocity(shape, kmax=4, cfl=1.5, nt=nt, seed=95)
+    mT = synth.gaussian_blobs(shape, count=8, seed=96)
+    mR = synth.gaussian_blobs(shape, count=8, seed=97)
+    vt = synth.bandlimited_velocity(shape, kmax=4, cfl=1.0, nt=nt, seed=98)
+    c = Claire(shape, nt=nt, regnorm="h1sn", beta=BETA)
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        c.set_reference_image(mR)
+        m = c.solve_state()
+        lam, b = c.solve_adjoint()
+        g = c.evaluate_gradient()
+        H = c.hessian_matvec(vt)
+        c.set_precision("mixed")
+        Hm = c.hessian_matvec(vt)
+    finally:
+        c.close()
+    m32, m64 = oracle.solve_state(v, mT, nt, prec="f32"), oracle.solve_state(v, mT, nt)
+    assert_fp32_parity(m, m32, m64)
+    assert_fp32_parity(b, oracle.solve_adjoint(v, m32, mR, nt, prec="f32")[1], oracle.solve_adjoint(v, m64, mR, nt)[1])
+    assert_fp32_parity(g, oracle.gradient(v, mT, mR, nt, "h1sn", BETA, prec="f32"),
+                       oracle.gradient(v, mT, mR, nt, "h1sn", BETA))
+    H64 = oracle.hessian_matvec(v, m64, vt, nt, "h1sn", BETA)
+    assert_fp32_parity(H, oracle.hessian_matvec(v, m32, vt, nt, "h1sn", BETA, prec="f32"), H64)
+    print(f"mixed relerr vs FP64 at nt={nt}: {relerr(Hm, H64):.3e} (FP32 {relerr(H, H64):.3e})")
+    assert relerr(Hm, H64) < 2.4e-4 * (1.0 if nt < 8 else 0.5)
+
+
 @pytest.mark.parametrize("norm", ["h1", "h2", "h2sn"])
 def test_hessian_other_norms(norm, oracle):
     shape = (32, 32, 32)

@@ -166,3 +166,32 @@ def test_rk2_gauss_newton_matches_oracle(oracle):
     assert _close(mis, mis64, mis32)
     assert _close(red, red64, red32, floor=5e-5)
     assert _close(rep["J"], rep64["J"], rep32["J"])
+
+
+@pytest.mark.parametrize("nt", [1, 3])
+def test_rk2_time_step_counts(nt, oracle):
+    """RK2 state, gradient and Hessian matvec with nt other than 4 (the first
+    incremental step without ∇m̃, the trapezoid weights)."""
+    shape = (16, 32, 16)
+    print(f"\n[gpu case] rk2 nt={nt} {shape}", flush=True)
+    v = synth.bandlimited_velocity(shape, kmax=3, cfl=0.5, nt=nt, seed=130)
+    mT = synth.gaussian_blobs(shape, count=6, seed=131)
+    mR = synth.gaussian_blobs(shape, count=6, seed=132)
+    vt = synth.bandlimited_velocity(shape, kmax=3, cfl=0.5, nt=nt, seed=133)
+    c = Claire(shape, nt=nt, regnorm="h1sn", beta=BETA, pde_solver="rk2")
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        c.set_reference_image(mR)
+        m = c.solve_state()
+        g = c.evaluate_gradient()
+        H = c.hessian_matvec(vt)
+    finally:
+        c.close()
+    with oracle.pde_solver("rk2"):
+        m32, m64 = oracle.solve_state(v, mT, nt, prec="f32"), oracle.solve_state(v, mT, nt)
+        assert_fp32_parity(m, m32, m64)
+        assert_fp32_parity(g, oracle.gradient(v, mT, mR, nt, "h1sn", BETA, prec="f32"),
+                           oracle.gradient(v, mT, mR, nt, "h1sn", BETA))
+        assert_fp32_parity(H, oracle.hessian_matvec(v, m32, vt, nt, "h1sn", BETA, prec="f32"),
+                           oracle.hessian_matvec(v, m64, vt, nt, "h1sn", BETA))
