@@ -269,6 +269,33 @@ def test_errors_are_reported():
         c.close()
 
 
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("cfl", [1.0, 3.0, 9.0])
+def test_trajectory_at_any_displacement(cfl, order, oracle):
+    """The characteristic goes through the LDS stage gathers while every stage
+    point stays within the LDS window and through the direct kernels beyond
+    it (CFL 9); the foot points agree with the FP32 oracle either way, and the
+    state solve on them with the FP64 oracle."""
+    shape = (32, 48, 64)
+    print(f"\n[gpu case] trajectory rk{order} cfl={cfl}", flush=True)
+    v = synth.bandlimited_velocity(shape, kmax=4, cfl=cfl, nt=NT, seed=45)
+    mT = synth.gaussian_blobs(shape, count=8, seed=46)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA, rk_order=order)
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        X = {flag: c.trajectory(flag) for flag in (STATE, ADJOINT)}
+        m = c.solve_state()
+    finally:
+        c.close()
+    with oracle.rk_order(order):
+        for flag, sigma in ((STATE, 1), (ADJOINT, -1)):
+            assert np.abs(X[flag] - oracle.trajectory(v, NT, sigma, prec="f32")).max() < 1e-6
+        m64 = oracle.solve_state(v, mT, NT)
+        m32 = oracle.solve_state(v, mT, NT, prec="f32")
+    assert_fp32_parity(m, m32, m64)
+
+
 @pytest.mark.parametrize("cfl", [2.0, 5.0])
 def test_lds_gather_matches_direct_gather(cfl):
     """The LDS-staged gather (along a trajectory) and the direct global gather
