@@ -177,3 +177,64 @@ def test_registration_with_the_symmetric_preconditioned_operator(oracle):
               f"|g|/|g0| {r['gnorm'] / r['gnorm0']:.6e} ({r['reason']})")
     assert rep["reason"] == rep64["reason"]
     assert abs(rep["J"] - rep64["J"]) <= max(1e-3 * rep64["J"], 2.0 * abs(rep32["J"] - rep64["J"]))
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_models_with_rk2_integrator(model, oracle):
+    """-pdesolver rk2 with the other models: CLAIREStokes overrides only the
+    SL adjoint solves, so its RK2 adjoints keep the ∇·v terms; the projection
+    and (relaxed model) the divergence penalty still apply."""
+    shape = (32, 16, 32)
+    print(f"\n[gpu case] model {model} rk2 {shape}", flush=True)
+    v = synth.bandlimited_velocity(shape, kmax=4, cfl=0.5, nt=NT, seed=140)
+    mT = synth.gaussian_blobs(shape, count=8, seed=141)
+    mR = synth.gaussian_blobs(shape, count=8, seed=142)
+    vt = synth.bandlimited_velocity(shape, kmax=4, cfl=0.5, nt=NT, seed=143)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA, model=model, pde_solver="rk2")
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        c.set_reference_image(mR)
+        lam, b = c.solve_adjoint()
+        H = c.hessian_matvec(vt)
+        J, D, R = c.evaluate_objective(v)
+        g = c.evaluate_gradient(v)
+    finally:
+        c.close()
+    with oracle.reg_model(model, BETA), oracle.pde_solver("rk2"):
+        m32, m64 = oracle.solve_state(v, mT, NT, prec="f32"), oracle.solve_state(v, mT, NT)
+        b32, b64 = oracle.solve_adjoint(v, m32, mR, NT, prec="f32")[1], oracle.solve_adjoint(v, m64, mR, NT)[1]
+        H32 = oracle.hessian_matvec(v, m32, vt, NT, "h1sn", BETA, prec="f32")
+        H64 = oracle.hessian_matvec(v, m64, vt, NT, "h1sn", BETA)
+        J64, D64, R64 = oracle.objective(v, mT, mR, NT, "h1sn", BETA)
+        g32 = oracle.gradient(v, mT, mR, NT, "h1sn", BETA, prec="f32")
+        g64 = oracle.gradient(v, mT, mR, NT, "h1sn", BETA)
+    assert_fp32_parity(b, b32, b64)
+    assert_fp32_parity(H, H32, H64)
+    assert abs(J - J64) < 1e-5 * abs(J64) and abs(R - R64) < 1e-5 * abs(R64)
+    assert_fp32_parity(g, g32, g64)
+
+
+def test_continuity_with_rk4_and_rk2(oracle):
+    """The continuity-equation state is an SL solve whatever the integrator
+    (src/CLAIRE.cpp:1923-1926): with -rkorder 4 it follows the RK4
+    characteristic, with -pdesolver rk2 it still uses the SL trajectory."""
+    shape = (32, 32, 16)
+    print(f"\n[gpu case] continuity rk4 / pdesolver rk2 {shape}", flush=True)
+    v = synth.bandlimited_velocity(shape, kmax=4, cfl=1.0, nt=NT, seed=150)
+    mT = synth.gaussian_blobs(shape, count=8, seed=151)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA, pde="continuity", rk_order=4)
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        m4 = c.solve_state()
+        c.set_rk_order(2)
+        c.set_pde_solver("rk2")
+        m2 = c.solve_state()
+    finally:
+        c.close()
+    with oracle.pde_type("continuity"):
+        with oracle.rk_order(4):
+            assert_fp32_parity(m4, oracle.solve_state(v, mT, NT, prec="f32"), oracle.solve_state(v, mT, NT))
+        with oracle.pde_solver("rk2"):
+            assert_fp32_parity(m2, oracle.solve_state(v, mT, NT, prec="f32"), oracle.solve_state(v, mT, NT))
