@@ -302,6 +302,174 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
     if (x1 < x1e) step(x1, qa, aa, qb, ab);
 }
 
+// Pair layout for D <= 2.  Entry c of an LDS row holds the two consecutive
+// values (f[c], f[c+1]) of the tile row, so the four values of a stencil row
+// starting at column c are the 8-byte entries c and c + 2: one ds_read2_b64
+// per stencil row instead of two ds_read2_b32, with no alignment condition
+// on c.  The price is twice the LDS per plane: a 16 × 32 tile with its halo
+// takes 79.5 KB, two blocks (16 waves) per CU.
+struct PairGeo {
+    static constexpr int D = 2;
+    static constexpr int LT3 = 32;
+    static constexpr int WAVES_PER_EU = 4;
+    static constexpr int HLO = D + 1, HHI = D + 2;
+    static constexpr int E2 = kLT2 + HLO + HHI, E3 = LT3 + HLO + HHI;
+    static constexpr int NE3 = E3 - 1;                   // entries per row (the last column only fills entry E3 − 2)
+    static constexpr int PITCH = (2 * NE3 + 31) / 32 * 32;  // dwords per row
+    static constexpr int P = E2 * PITCH;                 // dwords per LDS plane
+    static constexpr int NV = E2 * NE3;                  // entries per plane
+    static constexpr int WIN = HLO + HHI + 1;
+    static constexpr int W = WIN + 1;
+    static constexpr int FILL = (NV + kLThreads - 1) / kLThreads;
+    // every ds_read2_b64 of a stencil (rows 0-3, entries c and c + 2) is
+    // within the instruction's 8-bit offsets (in 8-byte units) of the base
+    static_assert((3 * 4 * PITCH + 16) / 8 <= 255, "row 3 out of ds_read2_b64 offset range");
+};
+
+template <class Epi>
+__global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(PairGeo::WAVES_PER_EU))) void
+k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __restrict__ fbase, int64_t fzstride,
+                  Epi epi, int nblocks, int chunk) {
+    using G = PairGeo;
+    __shared__ __align__(16) float lds[G::W * G::P];
+    const int tid = threadIdx.x;
+    const int nt3 = (g.n3 + G::LT3 - 1) / G::LT3;
+    const int nt2 = (g.n2 + kLT2 - 1) / kLT2;
+    const int L = xcd_remap(blockIdx.x, nblocks);
+    const int t3 = L % nt3;
+    const int t2 = (L / nt3) % nt2;
+    const int ch = L / (nt3 * nt2);
+    const float* f = fbase + static_cast<int64_t>(blockIdx.y) * fzstride;
+    epi.select(blockIdx.y);
+    const uint32_t fbytes = static_cast<uint32_t>(g.nl) * 4u;
+    const rsrc_t rf = make_rsrc(f, fbytes);
+    const rsrc_t rp = make_rsrc(plan, 3u * fbytes);
+    const rsrc_t ra = make_rsrc(Epi::kHasAux ? epi.auxp() : f, fbytes);
+    const rsrc_t ro = make_rsrc(epi.outp(), fbytes);
+    const rsrc_t ro2 = make_rsrc(out2_of(epi), fbytes);
+    const uint32_t plane4 = static_cast<uint32_t>(g.n2) * static_cast<uint32_t>(g.n3) * 4u;
+    const int x1s = ch * chunk;
+    const int x1e = min(x1s + chunk, g.n1);
+    const int r2 = t2 * kLT2 - G::HLO;
+    const int r3 = t3 * G::LT3 - G::HLO;
+
+    // Entries this thread fills: the byte offsets (within a global plane) of
+    // the entry's two values and the entry's dword offset in an LDS plane.
+    // Entries past the plane reload entry NV−1 (never stored).
+    uint32_t foff[G::FILL][2];
+    int loff[G::FILL];
+#pragma unroll
+    for (int k = 0; k < G::FILL; ++k) {
+        const int e = min(tid + kLThreads * k, G::NV - 1);
+        const int e2 = e / G::NE3;
+        const int e3 = e - e2 * G::NE3;
+        const int row = pmod(r2 + e2, g.n2) * g.n3;
+        foff[k][0] = 4u * static_cast<uint32_t>(row + pmod(r3 + e3, g.n3));
+        foff[k][1] = 4u * static_cast<uint32_t>(row + pmod(r3 + e3 + 1, g.n3));
+        loff[k] = e2 * G::PITCH + 2 * e3;
+    }
+
+    const int lane = tid & 63;
+    const int lx2 = 2 * (tid >> 6) + (lane >> 5);
+    const int x2 = t2 * kLT2 + lx2;
+    const int lx3 = lane & 31;
+    const int x3 = t3 * G::LT3 + lx3;
+    const bool valid = x2 < g.n2 && x3 < g.n3;
+    const uint32_t off = valid ? 4u * static_cast<uint32_t>(x2 * g.n3 + x3) : 0u;
+    // byte offset of the point's own cell (its entry) in an LDS plane
+    const int rc = 4 * ((lx2 + G::HLO) * G::PITCH + 2 * (lx3 + G::HLO));
+
+    auto store2 = [&](float* dst, int q, float a, float b) {
+        *reinterpret_cast<float2*>(dst + loff[q]) = make_float2(a, b);
+    };
+    for (int k = 0; k < G::WIN; ++k) {
+        const uint32_t src = static_cast<uint32_t>(pmod(x1s - G::HLO + k, g.n1)) * plane4;
+        float* dst = lds + k * G::P;
+#pragma unroll
+        for (int q = 0; q < G::FILL; ++q)
+            if (tid + kLThreads * q < G::NV) store2(dst, q, bload(rf, foff[q][0], src), bload(rf, foff[q][1], src));
+    }
+    PlanPt qa, qb;
+    float aa, ab;
+    {
+        const uint32_t pb = static_cast<uint32_t>(x1s) * plane4;
+        qa = bload_plan(rp, 3u * off, 3u * pb);
+        aa = Epi::kHasAux ? bload_once(ra, off, pb) : 0.0f;
+    }
+    __syncthreads();
+
+    constexpr int P4 = 4 * G::P;
+    const char* ldsb = reinterpret_cast<const char*>(lds);
+    int cs = 0;
+    auto step = [&](int x1, const PlanPt& cur, float caux, PlanPt& nxt, float& naux) {
+        const int xn = (x1 + 1 < x1e) ? x1 + 1 : x1;
+        float pre[G::FILL][2];
+        {
+            const uint32_t src = static_cast<uint32_t>(pmod(x1 + G::HHI + 1, g.n1)) * plane4;
+#pragma unroll
+            for (int q = 0; q < G::FILL; ++q) {
+                pre[q][0] = bload(rf, foff[q][0], src);
+                pre[q][1] = bload(rf, foff[q][1], src);
+            }
+        }
+        {
+            const uint32_t pb = static_cast<uint32_t>(xn) * plane4;
+            nxt = bload_plan(rp, 3u * off, 3u * pb);
+            naux = Epi::kHasAux ? bload_once(ra, off, pb) : 0.0f;
+        }
+        float w1[4], w2[4], w3[4];
+        lagrange_weights(plan_t(cur.w[0]), w1);
+        lagrange_weights(plan_t(cur.w[1]), w2);
+        lagrange_weights(plan_t(cur.w[2]), w3);
+        // the plan's bound keeps every stencil inside the ring; the clamps
+        // only pin LDS addresses inside the allocation
+        const int lb = min(max(rc + plan_off(cur.w[1]) * (4 * G::PITCH) + 8 * plan_off(cur.w[2]), 0),
+                           4 * (G::P - 3 * G::PITCH - 6));
+        int sw = min(max(cs + G::HLO + plan_off(cur.w[0]), 0), 2 * G::W - 4);
+        sw -= (sw >= G::W) ? G::W : 0;
+        int base[4];
+        base[0] = opaque(sw * P4 + lb);
+#pragma unroll
+        for (int a = 1; a < 4; ++a) base[a] = opaque(base[a - 1] + ((sw + a == G::W) ? P4 - G::W * P4 : P4));
+        float v = 0.f;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            float sa = 0.f;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const char* row = ldsb + base[a] + b * 4 * G::PITCH;
+                const float2 lo = *reinterpret_cast<const float2*>(row);
+                const float2 hi = *reinterpret_cast<const float2*>(row + 16);
+                const float r = w3[0] * lo.x + w3[1] * lo.y + w3[2] * hi.x + w3[3] * hi.y;
+                sa += w2[b] * r;
+            }
+            v += w1[a] * sa;
+        }
+        if (valid) {
+            const uint32_t pb = static_cast<uint32_t>(x1) * plane4;
+            if constexpr (Epi::kCoord)
+                bstore(ro, epi.apply_at(v, caux, x1, x2, x3), off, pb);
+            else
+                bstore(ro, epi.apply(v, caux), off, pb);
+            if constexpr (Epi::kOut2) bstore(ro2, epi.apply2(v, caux), off, pb);
+        }
+        const int spare = (cs == 0) ? G::W - 1 : cs - 1;
+        float* dst = lds + spare * G::P;
+#pragma unroll
+        for (int q = 0; q < G::FILL; ++q)
+            if (tid + kLThreads * q < G::NV) store2(dst, q, pre[q][0], pre[q][1]);
+        cs = (cs + 1 == G::W) ? 0 : cs + 1;
+        __syncthreads();
+    };
+
+    int x1 = x1s;
+    for (; x1 + 1 < x1e; x1 += 2) {
+        step(x1, qa, aa, qb, ab);
+        step(x1 + 1, qb, ab, qa, aa);
+    }
+    if (x1 < x1e) step(x1, qa, aa, qb, ab);
+}
+
 // Incremental state right-hand sides, streamed with 16-byte accesses:
 //   s_j = ht/2·[(∇m^j)(X_s)·ṽ(X_s) + ∇m^{j+1}·ṽ],  j = 0..nt−1
 // (the two decrements of src/CLAIRE.cpp:3253-3273);
@@ -436,11 +604,21 @@ int chunk_override() {
     return v;
 }
 
-template <int D>
+// CLAIRE_AMD_GATHER_PAIR=0 selects the one-value-per-entry layout for single-field D <= 2 gathers
+// (three blocks per CU) instead of the pair layout (two), for comparison.
+bool pair_layout() {
+    static const bool on = [] {
+        const char* e = std::getenv("CLAIRE_AMD_GATHER_PAIR");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
+}
+
+template <int D, bool PAIR>
 int choose_chunk(int n1, int64_t tiles) {
     using G = LdsGeo<D>;
     if (chunk_override() >= 4) return std::min(chunk_override(), n1);
-    const int64_t slots = static_cast<int64_t>(num_cus()) * (D <= 2 ? 3 : 1);
+    const int64_t slots = static_cast<int64_t>(num_cus()) * (PAIR ? 2 : (D <= 2 ? 3 : 1));
     // Chunks longer than 48 planes measured slower at 256³ (fewer, longer
     // blocks start and stream in lockstep); 32-43 planes were best.
     const int cmax = std::min(n1, 48);
@@ -485,25 +663,30 @@ bool tuning_enabled() {
     return on;
 }
 
-template <int D, class Epi>
+template <int D, bool PAIR, class Epi>
 void launch_lds(const GatherDims& g, const uint32_t* P, const float* f, int64_t fzstride, int nz, const Epi& epi,
                 hipStream_t st) {
+    static_assert(!PAIR || D == 2, "the pair layout is for D <= 2");
     const int nt3 = (g.n3 + LdsGeo<D>::LT3 - 1) / LdsGeo<D>::LT3;
     const int nt2 = (g.n2 + kLT2 - 1) / kLT2;
     auto launch = [&](int chunk) {
         const int nch = (g.n1 + chunk - 1) / chunk;
         const int nb = nt3 * nt2 * nch;
-        hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
-                           reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
+        if constexpr (PAIR)
+            hipLaunchKernelGGL((k_gather_lds_pair<Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
+                               reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
+        else
+            hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
+                               reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
     };
     if (chunk_override() >= 4) return launch(std::min(chunk_override(), g.n1));
-    const ShapeKey key{D, g.n1, g.n2, g.n3, nz};
+    const ShapeKey key{PAIR ? 100 + D : D, g.n1, g.n2, g.n3, nz};
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         const auto it = g_tuned.find(key);
         if (it != g_tuned.end()) return launch(it->second);
     }
-    const int guess = choose_chunk<D>(g.n1, static_cast<int64_t>(nt3) * nt2 * nz);
+    const int guess = choose_chunk<D, PAIR>(g.n1, static_cast<int64_t>(nt3) * nt2 * nz);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (!tuning_enabled() || hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
         return launch(guess);
@@ -542,10 +725,15 @@ void dispatch(int dbound, const GatherDims& g, const uint32_t* P, const float* f
     CLR_REQUIRE(g.n1 >= 4 && g.n2 >= 4 && g.n3 >= 4, kErrInvalidArgument, "gather needs n >= 4 per axis");
     CLR_REQUIRE(g.nl < (int64_t(1) << 28), kErrInvalidArgument, "grid too large for 32-bit buffer byte offsets");
     CLR_REQUIRE(dbound >= 0 && dbound <= 4, kErrInvalidArgument, "displacement bound too large for the LDS gather");
-    if (dbound <= 2)
-        launch_lds<2>(g, P, f, fzstride, nz, epi, st);
+    // The pair layout is faster for one field per launch (the solves' steps:
+    // 110 µs against 115 at 256³) and slower for three (313 µs against 303:
+    // three times the blocks at two per CU instead of three).
+    if (dbound <= 2 && nz == 1 && pair_layout())
+        launch_lds<2, true>(g, P, f, fzstride, nz, epi, st);
+    else if (dbound <= 2)
+        launch_lds<2, false>(g, P, f, fzstride, nz, epi, st);
     else
-        launch_lds<4>(g, P, f, fzstride, nz, epi, st);
+        launch_lds<4, false>(g, P, f, fzstride, nz, epi, st);
 }
 
 bool use_lds(const TrajectoryPlan& tp) { return tp.dbound >= 0 && tp.dbound <= lds_gather_max_displacement(); }
