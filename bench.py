@@ -470,7 +470,7 @@ def main():
 
     Hgpu = hv.download()
 
-    # mixed form: the same matvec with FP16 storage of the Newton-invariant
+    # mixed form: the same matvec with 16-bit block-floating-point storage of the Newton-invariant
     # gradient fields (FP32 arithmetic)
     c.set_precision("mixed")
     c.prepare_hessian()
@@ -488,7 +488,8 @@ def main():
     c.set_precision("fp32")
     out["mixed"] = {"value": round(value_m, 3), "unit": "matvec/s", "ms_per_step": round(1e3 * elapsed_m / args.steps, 4),
                     "speedup_vs_fp32": round(value_m / value, 3),
-                    "form": "FP16 storage of grad m^j and (grad m^j)(X_s), FP32 arithmetic, FP64 reductions"}
+                    "form": "16-bit block floating point (int16 mantissas, one power-of-two scale per 256 "
+                            "values) storage of grad m^j and (grad m^j)(X_s), FP32 arithmetic, FP64 reductions"}
 
     if not args.no_cpu_baseline:
         from oracle import oracle_py

@@ -345,8 +345,8 @@ class Claire:
         return bool(reached.value), tuple(st)
 
     def set_precision(self, precision):
-        """"fp32" (the reference's path) or "mixed" (FP16 storage of the
-        Newton-invariant gradient fields in the Hessian matvec)."""
+        """"fp32" (the reference's path) or "mixed" (16-bit block-floating-point
+        storage of the Newton-invariant gradient fields in the Hessian matvec)."""
         _check(lib().clr_set_precision(self.ctx, ctypes.c_int({"fp32": 0, "mixed": 1}[precision])))
 
     def set_matvec_type(self, kind):

@@ -638,10 +638,11 @@ size_t CLAIRE::DeviceBytes() const {
     const size_t floats = 5 * 3 * nl + 2 * nl + (nt + 1) * nl + nl + 3 * nl + 4 * nl + (nt + 1) * 3 * nl +
                           nt * 3 * nl + nt * nl + nt * nl + nl + 2 * 3 * nl;
     const size_t halves = m_Opt.precision == 1 ? (2 * nt + 1) * 3 * nl : 0;
+    const size_t scales = m_Opt.precision == 1 ? (2 * nt + 1) * 3 * static_cast<size_t>(bfp16_blocks(nl)) : 0;
     const size_t rk4 = 3 * nl;  // RK4 stage sum, or the RK2 corrector output
     const size_t cont = m_Opt.pdetype == 0 ? nl : 0;      // continuity-equation coefficient
     const size_t rk2 = m_Opt.pdesolver == 0 ? 6 * nl : 0;  // RK2 integrator work
-    return (floats + rk4 + cont + rk2) * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
+    return (floats + rk4 + cont + rk2 + scales) * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
 }
 
 void CLAIRE::RequireVelocity() const { CLR_REQUIRE(m_HaveVelocity, kErrState, "velocity field not set"); }
@@ -774,10 +775,13 @@ void CLAIRE::EnsureHalfGradients() {
     if (m_HalfValid) return;
     const int64_t nl = m_Opt.domain.nl;
     const int nt = m_Opt.nt;
+    const int64_t nblk = bfp16_blocks(nl);
     m_GradM16.resize(static_cast<size_t>((nt + 1) * 3 * nl));
     m_GradMX16.resize(static_cast<size_t>(nt * 3 * nl));
-    launch_to_half(m_GradM.get(), m_GradM16.get(), (nt + 1) * 3 * nl, m_Opt.stream);
-    launch_to_half(m_GradMX.get(), m_GradMX16.get(), nt * 3 * nl, m_Opt.stream);
+    m_GradMScale.resize(static_cast<size_t>((nt + 1) * 3 * nblk));
+    m_GradMXScale.resize(static_cast<size_t>(nt * 3 * nblk));
+    launch_to_bfp16(m_GradM.get(), nl, (nt + 1) * 3, GradMBfp(), m_Opt.stream);
+    launch_to_bfp16(m_GradMX.get(), nl, nt * 3, GradMXBfp(), m_Opt.stream);
     m_HalfValid = true;
 }
 
@@ -991,10 +995,10 @@ void CLAIRE::IncAdjointCore(const float* lt0, VecField* bt, bool half) {
         }
     }
     if (half) {
-        // bytes: λ^0..λ^nt 4(nt+1) + ∇m (half) 6(nt+1) + b 12
+        // bytes: λ^0..λ^nt 4(nt+1) + ∇m (16-bit) 6(nt+1) + b 12
         KernelScope k(m_Opt.prof, "body_force", (10.0 * (nt + 1) + 12.0) * nl);
-        launch_body_force_half(nl, nt, m_Opt.GetTimeStepSize(), m_Opt.GetLebesgueMeasure(), m_Lam.get(),
-                               m_GradM16.get(), b, m_Opt.stream);
+        launch_body_force_bfp(nl, nt, m_Opt.GetTimeStepSize(), m_Opt.GetLebesgueMeasure(), m_Lam.get(), GradMBfp(),
+                              b, m_Opt.stream);
         return;
     }
     // bytes: λ^0..λ^nt 4(nt+1) + ∇m 12(nt+1) + b 12
@@ -1022,10 +1026,10 @@ void CLAIRE::IncStateCore(const VecField* vt, float* out, bool half) {
     // ṽ(X_s) (src/CLAIRE.cpp:3221)
     m_SL->Gather3(v, vx, "state");
     if (half) {
-        // bytes: ṽX 12 + ṽ 12 + (∇m^j)(X_s) 6nt + ∇m^{j+1} 6nt (half) + s_j 4nt
+        // bytes: ṽX 12 + ṽ 12 + (∇m^j)(X_s) 6nt + ∇m^{j+1} 6nt (16-bit) + s_j 4nt
         KernelScope k(m_Opt.prof, "inc_state_rhs", (24.0 + 16.0 * nt) * nl);
-        launch_inc_state_rhs_half(nl, nt, m_Opt.GetTimeStepSize(), vx, v, m_GradMX16.get(), m_GradM16.get(),
-                                  m_S.get(), first, m_Opt.stream);
+        launch_inc_state_rhs_bfp(nl, nt, m_Opt.GetTimeStepSize(), vx, v, GradMXBfp(), GradMBfp(), m_S.get(), first,
+                                 m_Opt.stream);
     } else {
         // bytes: ṽX 12 + ṽ 12 + (∇m^j)(X_s) 12nt + ∇m^{j+1} 12nt + s_j 4nt
         KernelScope k(m_Opt.prof, "inc_state_rhs", (24.0 + 28.0 * nt) * nl);

@@ -121,8 +121,9 @@ struct RegOpt {
     Profiler* prof = nullptr;
     // 0: plain FP32.  1: mixed form of the Hessian matvec -- the
     // Newton-invariant gradient fields ∇m^j and (∇m^j)(X_s) read by the
-    // streaming kernels are stored as IEEE half (FP32 arithmetic); objective,
-    // gradient and reductions are unchanged.
+    // streaming kernels are stored as 16-bit block floating point (int16
+    // mantissas, one power-of-two scale per 256 values; FP32 arithmetic);
+    // objective, gradient and reductions are unchanged.
     int precision = 0;
     // Replay the Hessian matvec as a captured HIP graph (CLAIRE::HessianMatVec).
     int graphs = 0;
@@ -423,8 +424,12 @@ class CLAIRE {
     DeviceArray<float> m_StateCoeff;     // nl: continuity-equation state coefficient (pdetype 0 only)
     DeviceArray<float> m_Rk2Work;        // 6·nl, RK2 integrators only: a vector field, rhs0, rhs1 / m̄
     DeviceArray<float> m_VtX;            // ṽ(X_s), 3·nl
-    DeviceArray<uint16_t> m_GradM16;     // half copies of m_GradM / m_GradMX (mixed form)
-    DeviceArray<uint16_t> m_GradMX16;
+    // 16-bit block-floating-point copies of m_GradM / m_GradMX (mixed form):
+    // mantissas and per-256-value scales
+    DeviceArray<int16_t> m_GradM16, m_GradMX16;
+    DeviceArray<float> m_GradMScale, m_GradMXScale;
+    claire_amd::Bfp16Fields GradMBfp() { return {m_GradM16.get(), m_GradMScale.get()}; }
+    claire_amd::Bfp16Fields GradMXBfp() { return {m_GradMX16.get(), m_GradMXScale.get()}; }
     bool m_HalfValid = false;
 
     bool m_HaveVelocity = false, m_HaveIncVelocity = false, m_HaveTemplate = false, m_HaveReference = false;

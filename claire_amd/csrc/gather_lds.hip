@@ -474,9 +474,8 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
 //   s_j = ht/2·[(∇m^j)(X_s)·ṽ(X_s) + ∇m^{j+1}·ṽ],  j = 0..nt−1
 // (the two decrements of src/CLAIRE.cpp:3253-3273);
 // writes m̃¹ = −s_0 (λ̃⁰ = s_0 when nt = 1) and s_1..s_{nt−1}.
-// 4 consecutive values as float4: FP32 directly, FP16 converted.
-typedef _Float16 h4 __attribute__((ext_vector_type(4)));
-// 16-byte (FP32) or 8-byte (FP16) streaming loads with the non-temporal
+// 4 consecutive values as float4.
+// 16-byte (FP32) or 8-byte (16-bit) streaming loads with the non-temporal
 // hint: every element of these arrays is read once per launch, and the hint
 // keeps the streams from displacing reusable lines (measured on a read-8,
 // write-1 stream at 256³: 5.84 TB/s with the hint, 5.19 TB/s without).
@@ -486,38 +485,56 @@ __device__ __forceinline__ float4 ld4(const float* a, int64_t i) {
     const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(a) + i);
     return make_float4(v.x, v.y, v.z, v.w);
 }
-__device__ __forceinline__ float4 ld4(const _Float16* a, int64_t i) {
-    const u2v bits = __builtin_nontemporal_load(reinterpret_cast<const u2v*>(a) + i);
-    h4 h;
-    __builtin_memcpy(&h, &bits, sizeof(h));
-    return make_float4(static_cast<float>(h.x), static_cast<float>(h.y), static_cast<float>(h.z),
-                       static_cast<float>(h.w));
-}
 
-template <class G>
+// Sources of the Newton-invariant gradient fields, field k at offset k·nl:
+//   SrcF32  FP32 (the plain form);
+//   SrcBfp  16-bit block floating point (the mixed form): int16 mantissas
+//           with one power-of-two scale per block of 256 consecutive values
+//           of a field (kBfpBlock4 float4 groups, one wave of the streaming
+//           kernels); value = q·scale exactly.
+constexpr int kBfpBlock4 = 64;
+struct SrcF32 {
+    const float* p;
+    __device__ float4 load(int k, int64_t nl, int64_t i) const { return ld4(p + k * nl, i); }
+};
+struct SrcBfp {
+    const int16_t* q;
+    const float* s;
+    int64_t nblk;  // scale blocks per field
+    __device__ float4 load(int k, int64_t nl, int64_t i) const {
+        const u2v bits = __builtin_nontemporal_load(reinterpret_cast<const u2v*>(q + k * nl) + i);
+        const float sc = s[k * nblk + i / kBfpBlock4];
+        const int16_t a = static_cast<int16_t>(bits.x & 0xFFFFu), b = static_cast<int16_t>(bits.x >> 16);
+        const int16_t c = static_cast<int16_t>(bits.y & 0xFFFFu), d = static_cast<int16_t>(bits.y >> 16);
+        return make_float4(static_cast<float>(a) * sc, static_cast<float>(b) * sc, static_cast<float>(c) * sc,
+                           static_cast<float>(d) * sc);
+    }
+};
+
+template <class Src>
 struct RhsP {
     const float* vx[3];
     const float* vt[3];
-    const G* gX;
-    const G* gm;
+    Src gX;
+    Src gm;
     float* s;
     float* mt1;
 };
 
-// G = float (plain) or _Float16 (mixed form: the Newton-invariant gradient
-// fields stored in half precision, accumulated in FP32).
-template <class G>
-__global__ __launch_bounds__(256) void k_inc_state_rhs(int64_t nl4, int nt, float hthalf, RhsP<G> p) {
+// Src = SrcF32 (plain) or SrcBfp (mixed form: the Newton-invariant gradient
+// fields stored as 16-bit block floating point, accumulated in FP32).
+template <class Src>
+__global__ __launch_bounds__(256) void k_inc_state_rhs(int64_t nl4, int nt, float hthalf, RhsP<Src> p) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= nl4) return;
     const int64_t nl = nl4 * 4;
     const float4 x0 = ld4(p.vx[0], i), x1 = ld4(p.vx[1], i), x2 = ld4(p.vx[2], i);
     const float4 v0 = ld4(p.vt[0], i), v1 = ld4(p.vt[1], i), v2 = ld4(p.vt[2], i);
     for (int j = 0; j < nt; ++j) {
-        const G* gx = p.gX + static_cast<int64_t>(3 * j) * nl;
-        const G* gn = p.gm + static_cast<int64_t>(3 * (j + 1)) * nl;
-        const float4 a0 = ld4(gx, i), a1 = ld4(gx + nl, i), a2 = ld4(gx + 2 * nl, i);
-        const float4 c0 = ld4(gn, i), c1 = ld4(gn + nl, i), c2 = ld4(gn + 2 * nl, i);
+        const float4 a0 = p.gX.load(3 * j, nl, i), a1 = p.gX.load(3 * j + 1, nl, i),
+                     a2 = p.gX.load(3 * j + 2, nl, i);
+        const float4 c0 = p.gm.load(3 * (j + 1), nl, i), c1 = p.gm.load(3 * (j + 1) + 1, nl, i),
+                     c2 = p.gm.load(3 * (j + 1) + 2, nl, i);
         float4 s;
         s.x = hthalf * (a0.x * x0.x + a1.x * x1.x + a2.x * x2.x) + hthalf * (c0.x * v0.x + c1.x * v1.x + c2.x * v2.x);
         s.y = hthalf * (a0.y * x0.y + a1.y * x1.y + a2.y * x2.y) + hthalf * (c0.y * v0.y + c1.y * v1.y + c2.y * v2.y);
@@ -536,15 +553,15 @@ __global__ __launch_bounds__(256) void k_inc_state_rhs(int64_t nl4, int nt, floa
 // w_0 = w_nt = ht/2, w_j = ht, accumulated in the reference's order
 // (src/CLAIRE.cpp:2668-2709 and 3940-3999, scaled by hd at :2341/:3471).
 // Streamed with 16-byte accesses; lam holds λ^0..λ^nt.
-template <class G>
+template <class Src>
 struct BodyP {
     const float* lam;
-    const G* gm;
+    Src gm;
     float* b[3];
 };
 
-template <class G>
-__global__ __launch_bounds__(256) void k_body_force(int64_t nl4, int nt, float ht, float hd, BodyP<G> p) {
+template <class Src>
+__global__ __launch_bounds__(256) void k_body_force(int64_t nl4, int nt, float ht, float hd, BodyP<Src> p) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= nl4) return;
     const int64_t nl = nl4 * 4;
@@ -552,10 +569,9 @@ __global__ __launch_bounds__(256) void k_body_force(int64_t nl4, int nt, float h
     for (int j = 0; j <= nt; ++j) {
         const float w = (j == 0 || j == nt) ? 0.5f * ht : ht;
         const float4 l = ld4(p.lam + static_cast<int64_t>(j) * nl, i);
-        const G* gj = p.gm + static_cast<int64_t>(3 * (nt - j)) * nl;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const float4 gd = ld4(gj + d * nl, i);
+            const float4 gd = p.gm.load(3 * (nt - j) + d, nl, i);
             s[d].x += w * gd.x * l.x;
             s[d].y += w * gd.y * l.y;
             s[d].z += w * gd.z * l.z;
@@ -567,18 +583,43 @@ __global__ __launch_bounds__(256) void k_body_force(int64_t nl4, int nt, float h
         reinterpret_cast<float4*>(p.b[d])[i] = make_float4(s[d].x * hd, s[d].y * hd, s[d].z * hd, s[d].w * hd);
 }
 
-// FP32 → FP16 (round to nearest even), 4 values per thread.
-__global__ __launch_bounds__(256) void k_to_half(const float* __restrict__ in, _Float16* __restrict__ out,
-                                                 int64_t n4) {
+// FP32 → 16-bit block floating point, field blockIdx.y, 4 values per thread.
+// A wave (64 threads, 256 values) shares one scale 2^(E−15), E the binary
+// exponent of its largest magnitude (max |v| < 2^E), so every mantissa
+// q = round(v / 2^(E−15)) fits int16 (clamped to ±32767 at the top); the
+// rounding error is at most 2^−16 of the block's largest magnitude.  Blocks
+// whose largest magnitude is below 2^−100 are stored as zero.
+__global__ __launch_bounds__(256) void k_to_bfp16(const float* __restrict__ in, int16_t* __restrict__ q,
+                                                  float* __restrict__ s, int64_t nl4, int64_t nblk) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= n4) return;
-    const float4 a = reinterpret_cast<const float4*>(in)[i];
-    h4 h;
-    h.x = static_cast<_Float16>(a.x);
-    h.y = static_cast<_Float16>(a.y);
-    h.z = static_cast<_Float16>(a.z);
-    h.w = static_cast<_Float16>(a.w);
-    reinterpret_cast<h4*>(out)[i] = h;
+    const int64_t field = blockIdx.y;
+    const int64_t nl = nl4 * 4;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < nl4) a = reinterpret_cast<const float4*>(in + field * nl)[i];
+    float m = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w)));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    float scale = 1.0f, inv = 0.0f;
+    if (m >= 0x1p-100f) {
+        int e = 0;
+        (void)frexpf(m, &e);  // m = f·2^e, f ∈ [0.5, 1)
+        scale = ldexpf(1.0f, e - 15);
+        inv = ldexpf(1.0f, 15 - e);
+    }
+    auto quant = [&](float v) {
+        const float r = rintf(v * inv);
+        return static_cast<int16_t>(fminf(fmaxf(r, -32767.0f), 32767.0f));
+    };
+    const int64_t blk = i / kBfpBlock4;
+    if (i < nl4) {
+        const uint32_t lo = static_cast<uint16_t>(quant(a.x)) | (static_cast<uint32_t>(static_cast<uint16_t>(quant(a.y))) << 16);
+        const uint32_t hi = static_cast<uint16_t>(quant(a.z)) | (static_cast<uint32_t>(static_cast<uint16_t>(quant(a.w))) << 16);
+        u2v bits;
+        bits.x = lo;
+        bits.y = hi;
+        reinterpret_cast<u2v*>(q + field * nl)[i] = bits;
+        if (i % kBfpBlock4 == 0) s[field * nblk + blk] = scale;
+    }
 }
 
 // Planes per block: at most 48, and the chunk count that fills whole rounds
@@ -809,53 +850,61 @@ void launch_lds_traj_rk4_stage(int stage, const Dims& d, const uint32_t* P, int 
     }
 }
 
-template <class G>
-void body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const G* gm, float* const b[3],
+template <class Src>
+void body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const Src& gm, float* const b[3],
                 hipStream_t st) {
     CLR_REQUIRE(nl % 4 == 0, kErrInvalidArgument, "grid size must be a multiple of 4 points");
     const int64_t nl4 = nl / 4;
-    BodyP<G> p{lam, gm, {b[0], b[1], b[2]}};
+    BodyP<Src> p{lam, gm, {b[0], b[1], b[2]}};
     const int nb = static_cast<int>((nl4 + 255) / 256);
-    hipLaunchKernelGGL((k_body_force<G>), dim3(nb), dim3(256), 0, st, nl4, nt, ht, hd, p);
+    hipLaunchKernelGGL((k_body_force<Src>), dim3(nb), dim3(256), 0, st, nl4, nt, ht, hd, p);
 }
 
-template <class G>
-void inc_state_rhs(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3], const G* gX,
-                   const G* gm, float* s, float* mt1, hipStream_t st) {
+template <class Src>
+void inc_state_rhs(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3], const Src& gX,
+                   const Src& gm, float* s, float* mt1, hipStream_t st) {
     CLR_REQUIRE(nl % 4 == 0, kErrInvalidArgument, "grid size must be a multiple of 4 points");
     CLR_REQUIRE(nt >= 1, kErrInvalidArgument, "nt must be >= 1");
     const int64_t nl4 = nl / 4;
-    RhsP<G> p{{vx[0], vx[1], vx[2]}, {vt[0], vt[1], vt[2]}, gX, gm, s, mt1};
+    RhsP<Src> p{{vx[0], vx[1], vx[2]}, {vt[0], vt[1], vt[2]}, gX, gm, s, mt1};
     const int nb = static_cast<int>((nl4 + 255) / 256);
-    hipLaunchKernelGGL((k_inc_state_rhs<G>), dim3(nb), dim3(256), 0, st, nl4, nt, 0.5f * ht, p);
+    hipLaunchKernelGGL((k_inc_state_rhs<Src>), dim3(nb), dim3(256), 0, st, nl4, nt, 0.5f * ht, p);
 }
+
+int64_t bfp16_blocks(int64_t nl) { return (nl / 4 + kBfpBlock4 - 1) / kBfpBlock4; }
+
+namespace {
+
+SrcBfp bfp_src(const Bfp16Fields& f, int64_t nl) { return SrcBfp{f.q, f.s, bfp16_blocks(nl)}; }
+
+}  // namespace
 
 void launch_body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const float* gm, float* const b[3],
                        hipStream_t st) {
-    body_force(nl, nt, ht, hd, lam, gm, b, st);
+    body_force(nl, nt, ht, hd, lam, SrcF32{gm}, b, st);
 }
 
-void launch_body_force_half(int64_t nl, int nt, float ht, float hd, const float* lam, const uint16_t* gm,
-                            float* const b[3], hipStream_t st) {
-    body_force(nl, nt, ht, hd, lam, reinterpret_cast<const _Float16*>(gm), b, st);
+void launch_body_force_bfp(int64_t nl, int nt, float ht, float hd, const float* lam, const Bfp16Fields& gm,
+                           float* const b[3], hipStream_t st) {
+    body_force(nl, nt, ht, hd, lam, bfp_src(gm, nl), b, st);
 }
 
 void launch_inc_state_rhs(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
                           const float* gX, const float* gm, float* s, float* mt1, hipStream_t st) {
-    inc_state_rhs(nl, nt, ht, vx, vt, gX, gm, s, mt1, st);
+    inc_state_rhs(nl, nt, ht, vx, vt, SrcF32{gX}, SrcF32{gm}, s, mt1, st);
 }
 
-void launch_inc_state_rhs_half(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
-                               const uint16_t* gX, const uint16_t* gm, float* s, float* mt1, hipStream_t st) {
-    inc_state_rhs(nl, nt, ht, vx, vt, reinterpret_cast<const _Float16*>(gX), reinterpret_cast<const _Float16*>(gm),
-                  s, mt1, st);
+void launch_inc_state_rhs_bfp(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
+                              const Bfp16Fields& gX, const Bfp16Fields& gm, float* s, float* mt1, hipStream_t st) {
+    inc_state_rhs(nl, nt, ht, vx, vt, bfp_src(gX, nl), bfp_src(gm, nl), s, mt1, st);
 }
 
-void launch_to_half(const float* in, uint16_t* out, int64_t n, hipStream_t st) {
-    CLR_REQUIRE(n % 4 == 0, kErrInvalidArgument, "length must be a multiple of 4");
-    const int64_t n4 = n / 4;
-    const int nb = static_cast<int>((n4 + 255) / 256);
-    hipLaunchKernelGGL(k_to_half, dim3(nb), dim3(256), 0, st, in, reinterpret_cast<_Float16*>(out), n4);
+void launch_to_bfp16(const float* in, int64_t nl, int nfields, const Bfp16Fields& out, hipStream_t st) {
+    CLR_REQUIRE(nl % 4 == 0, kErrInvalidArgument, "field length must be a multiple of 4");
+    CLR_REQUIRE(nfields >= 1 && nfields <= 65535, kErrInvalidArgument, "1 to 65535 fields");
+    const int64_t nl4 = nl / 4;
+    const int nb = static_cast<int>((nl4 + 255) / 256);
+    hipLaunchKernelGGL(k_to_bfp16, dim3(nb, nfields), dim3(256), 0, st, in, out.q, out.s, nl4, bfp16_blocks(nl));
 }
 
 }  // namespace claire_amd

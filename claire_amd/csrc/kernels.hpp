@@ -150,10 +150,18 @@ void launch_direct_adjoint_step(const GatherDims& g, const float* X, const float
 // λ^0..λ^nt, gm holds ∇m^0..∇m^nt (3 fields each).
 void launch_body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const float* gm, float* const b[3],
                        hipStream_t st);
-// Mixed form: the same with the gradient fields stored as IEEE half
-// (uint16_t bit patterns), accumulated in FP32.
-void launch_body_force_half(int64_t nl, int nt, float ht, float hd, const float* lam, const uint16_t* gm,
-                            float* const b[3], hipStream_t st);
+// Fields stored as 16-bit block floating point (the mixed form): int16
+// mantissas q (field k at q + k·nl) and one power-of-two scale per block of
+// 256 consecutive values of a field (field k's scales at s + k·bfp16_blocks(nl)).
+struct Bfp16Fields {
+    int16_t* q;
+    float* s;
+};
+int64_t bfp16_blocks(int64_t nl);
+// The same with the gradient fields in 16-bit block floating point,
+// accumulated in FP32.
+void launch_body_force_bfp(int64_t nl, int nt, float ht, float hd, const float* lam, const Bfp16Fields& gm,
+                           float* const b[3], hipStream_t st);
 // v = 0 body force: b_d = hd·λ·∂_d m
 void launch_body_force_static(int64_t nl, float hd, const float* lam, const float* gm, float* const b[3],
                               hipStream_t st);
@@ -162,10 +170,13 @@ void launch_body_force_static(int64_t nl, float hd, const float* lam, const floa
 void launch_inc_state_rhs(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
                           const float* gX, const float* gm, float* s, float* mt1, hipStream_t st);
 
-void launch_inc_state_rhs_half(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
-                               const uint16_t* gX, const uint16_t* gm, float* s, float* mt1, hipStream_t st);
-// FP32 → IEEE half, round to nearest even (n a multiple of 4).
-void launch_to_half(const float* in, uint16_t* out, int64_t n, hipStream_t st);
+void launch_inc_state_rhs_bfp(int64_t nl, int nt, float ht, const float* const vx[3], const float* const vt[3],
+                              const Bfp16Fields& gX, const Bfp16Fields& gm, float* s, float* mt1, hipStream_t st);
+// nfields consecutive FP32 fields of nl values (nl a multiple of 4) into 16-bit
+// block floating point: per block of 256 values, scale 2^(E−15) with
+// max |v| < 2^E, mantissas rounded to nearest (error <= 2^−16 of the block's
+// largest magnitude).
+void launch_to_bfp16(const float* in, int64_t nl, int nfields, const Bfp16Fields& out, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Element-wise kernels and reductions (pointwise.hip)

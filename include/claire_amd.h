@@ -102,7 +102,9 @@ int clr_bin_write(const char* path, int64_t n, const float* data);
 /* Arithmetic form of the Hessian matvec.  CLR_PRECISION_FP32 (default) is
  * the reference's single-precision path.  CLR_PRECISION_MIXED stores the
  * Newton-invariant gradient fields ∇m^j and (∇m^j)(X_s) that the streaming
- * kernels read as IEEE half (FP32 arithmetic, FP64 reductions unchanged);
+ * kernels read as 16-bit block floating point (int16 mantissas, one
+ * power-of-two scale per 256 values; FP32 arithmetic, FP64 reductions
+ * unchanged);
  * it has no counterpart in the reference, whose precision is fixed by the
  * PETSc build (ScalarType = PetscReal, include/CLAIREUtils.hpp:58-67). */
 #define CLR_PRECISION_FP32 0

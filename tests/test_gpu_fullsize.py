@@ -108,7 +108,8 @@ def test_hessian_accuracy_vs_fp64_128(norm, oracle):
 
 @pytest.mark.parametrize("norm", ["h1sn", "h2"])
 def test_mixed_hessian_accuracy_vs_fp64_128(norm, oracle):
-    """Mixed form (FP16 storage of the Newton-invariant gradient fields)
+    """Mixed form (16-bit block-floating-point storage of the Newton-invariant
+    gradient fields)
     scored against the FP64 oracle at 128³, next to the plain FP32 path."""
     shape = (128, 128, 128)
     print(f"\n[gpu case] mixed accuracy 128^3 {norm}", flush=True)
