@@ -48,7 +48,8 @@ typedef struct clr_ctx clr_ctx;
 /* Last error message of the calling thread. */
 const char* clr_last_error(void);
 
-/* --- device memory helpers (plumbing for bindings without a GPU array type) */
+/* --- device memory helpers (plumbing for bindings without a GPU array type;
+ * no reference counterpart: the reference's Vecs live in host memory) */
 int clr_device_malloc(void** ptr, size_t bytes);
 int clr_device_free(void* ptr);
 int clr_memcpy_htod(void* dst, const void* src, size_t bytes);
@@ -61,7 +62,8 @@ int clr_device_synchronize(void);
  * beta[0..2] (RegOpt m_RegNorm.beta), on `stream` (a hipStream_t or NULL).
  * Replaces `new CLAIRE(opt)` + RegOpt::DoSetup/InitializeFFT
  * (src/RegOpt.cpp:906-1030; hx = 2π/nx src/RegOpt.cpp:935).
- * Supported grid sizes: powers of two from 8 to 512 per axis. */
+ * Supported grid sizes: powers of two from 8 to 512 per axis, and 48, 96,
+ * 150, 192. */
 int clr_create(clr_ctx** ctx, const int nx[3], int nt, int regnorm, const double beta[3], void* stream);
 int clr_destroy(clr_ctx* ctx);
 int clr_synchronize(clr_ctx* ctx);
@@ -192,15 +194,15 @@ int clr_apply_projection(clr_ctx* ctx, float* v);
  * reports the graphs currently held. */
 int clr_set_matvec_graphs(clr_ctx* ctx, int on);
 int clr_matvec_graph_count(clr_ctx* ctx, int* count);
-/* Device memory held by the context (bytes). */
+/* Device memory held by the context (bytes); no reference counterpart. */
 int clr_device_bytes(clr_ctx* ctx, size_t* bytes);
 
-/* CLAIRE::SetControlVariable (include/CLAIRE.hpp; src/CLAIREBase.cpp) --
+/* CLAIREBase::SetControlVariable (src/CLAIREBase.cpp:393) --
  * sets v (3·nl) and invalidates trajectories and Newton-invariant caches. */
 int clr_set_velocity(clr_ctx* ctx, const float* v);
-/* CLAIRE::SetIncControlVariable */
+/* CLAIREBase::SetIncControlVariable (src/CLAIREBase.cpp:488) */
 int clr_set_inc_velocity(clr_ctx* ctx, const float* vtilde);
-/* CLAIREBase::SetTemplateImage / SetReferenceImage */
+/* CLAIREBase::SetTemplateImage / SetReferenceImage (src/CLAIREBase.cpp:304, :277) */
 int clr_set_template_image(clr_ctx* ctx, const float* mT);
 int clr_set_reference_image(clr_ctx* ctx, const float* mR);
 
@@ -305,11 +307,15 @@ int clr_solve_gn(clr_ctx* ctx, float* v, const clr_gn_options* opt, clr_gn_repor
 int clr_solve_gn_batch(clr_ctx* const* ctxs, float* const* v, int count, const clr_gn_options* opt,
                        clr_gn_report* reports, int* status, int workers);
 
-/* Regularization::EvaluateGradient == HessianMatVec (3·nl → 3·nl). */
+/* Regularization::EvaluateGradient == HessianMatVec (3·nl → 3·nl):
+ * src/RegularizationH1.cpp:164,277, RegularizationH1SN.cpp:158,264,
+ * RegularizationH2.cpp:178,285, RegularizationH2SN.cpp:182,293. */
 int clr_reg_apply(clr_ctx* ctx, const float* v, float* out);
-/* Regularization::EvaluateFunctional */
+/* Regularization::EvaluateFunctional: src/RegularizationH1.cpp:64,
+ * RegularizationH1SN.cpp:64, RegularizationH2.cpp:64, RegularizationH2SN.cpp:66. */
 int clr_reg_functional(clr_ctx* ctx, const float* v, double* R);
-/* Regularization::GetExtremeEigValsInvOp */
+/* Regularization::GetExtremeEigValsInvOp: src/RegularizationH1.cpp:427,
+ * RegularizationH1SN.cpp:404, RegularizationH2.cpp:426, RegularizationH2SN.cpp:435. */
 int clr_reg_eigvals_inv(clr_ctx* ctx, double* emin, double* emax);
 
 /* SemiLagrangian::ComputeTrajectory (RK2, src/SemiLagrangian.cpp:192-288)
@@ -333,11 +339,14 @@ int clr_div(clr_ctx* ctx, const float* v, float* d);
 int clr_fft_r2c(clr_ctx* ctx, const float* in, float* out_complex);
 int clr_fft_c2r(clr_ctx* ctx, const float* in_complex, float* out);
 
-/* FP64 inner product / 2-norm of n FP32 values (VecTDot / VecNorm). */
+/* FP64 inner product / 2-norm of n FP32 values: PETSc's VecTDot / VecNorm
+ * as the reference calls them (e.g. src/CLAIRE.cpp:286,334). */
 int clr_dot(clr_ctx* ctx, const float* a, const float* b, int64_t n, double* out);
 int clr_norm2(clr_ctx* ctx, const float* a, int64_t n, double* out);
 
-/* Kernel timing with HIP events on the context stream.  While enabled every
+/* Kernel timing with HIP events on the context stream, in place of the
+ * reference's host timers (RegOpt::StartTimer/StopTimer/ResetTimers,
+ * src/RegOpt.cpp:2585,2610,2519).  While enabled every
  * kernel launch is bracketed by events; clr_profile_get returns, for entry
  * i (0-based, sorted by name), the kernel class name, accumulated time in
  * ms, launch count and the algorithmic bytes of those launches.  Returns
