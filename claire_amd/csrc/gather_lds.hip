@@ -655,11 +655,12 @@ bool pair_layout() {
     return on;
 }
 
-template <int D, bool PAIR>
-int choose_chunk(int n1, int64_t tiles) {
+// per_cu: resident blocks per CU (the LDS of one block decides it).
+template <int D>
+int choose_chunk(int n1, int64_t tiles, int per_cu) {
     using G = LdsGeo<D>;
     if (chunk_override() >= 4) return std::min(chunk_override(), n1);
-    const int64_t slots = static_cast<int64_t>(num_cus()) * (PAIR ? 2 : (D <= 2 ? 3 : 1));
+    const int64_t slots = static_cast<int64_t>(num_cus()) * per_cu;
     // Chunks longer than 48 planes measured slower at 256³ (fewer, longer
     // blocks start and stream in lockstep); 32-43 planes were best.
     const int cmax = std::min(n1, 48);
@@ -704,36 +705,23 @@ bool tuning_enabled() {
     return on;
 }
 
-template <int D, bool PAIR, class Epi>
-void launch_lds(const GatherDims& g, const uint32_t* P, const float* f, int64_t fzstride, int nz, const Epi& epi,
-                hipStream_t st) {
-    static_assert(!PAIR || D == 2, "the pair layout is for D <= 2");
-    const int nt3 = (g.n3 + LdsGeo<D>::LT3 - 1) / LdsGeo<D>::LT3;
-    const int nt2 = (g.n2 + kLT2 - 1) / kLT2;
-    auto launch = [&](int chunk) {
-        const int nch = (g.n1 + chunk - 1) / chunk;
-        const int nb = nt3 * nt2 * nch;
-        if constexpr (PAIR)
-            hipLaunchKernelGGL((k_gather_lds_pair<Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
-                               reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
-        else
-            hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
-                               reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
-    };
-    if (chunk_override() >= 4) return launch(std::min(chunk_override(), g.n1));
-    const ShapeKey key{PAIR ? 100 + D : D, g.n1, g.n2, g.n3, nz};
+// Launches through `launch(chunk)` with the tuned chunk length of `key`,
+// tuning it on the first launch of that shape (see ShapeKey).
+template <class Launch, class Heuristic>
+void tuned_launch(const ShapeKey& key, int n1, hipStream_t st, Launch&& launch, Heuristic&& heuristic) {
+    if (chunk_override() >= 4) return launch(std::min(chunk_override(), n1));
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         const auto it = g_tuned.find(key);
         if (it != g_tuned.end()) return launch(it->second);
     }
-    const int guess = choose_chunk<D, PAIR>(g.n1, static_cast<int64_t>(nt3) * nt2 * nz);
+    const int guess = heuristic();
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (!tuning_enabled() || hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
         return launch(guess);
     std::vector<int> cand{guess};
     for (int c : {16, 24, 32, 40, 48})
-        if (c <= g.n1 && std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+        if (c <= n1 && std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
     launch(guess);  // warm-up; its output is the result
     int best = guess;
     if (cand.size() > 1) {
@@ -758,6 +746,28 @@ void launch_lds(const GatherDims& g, const uint32_t* P, const float* f, int64_t 
     }
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tuned[key] = best;
+}
+
+template <int D, bool PAIR, class Epi>
+void launch_lds(const GatherDims& g, const uint32_t* P, const float* f, int64_t fzstride, int nz, const Epi& epi,
+                hipStream_t st) {
+    static_assert(!PAIR || D == 2, "the pair layout is for D <= 2");
+    const int nt3 = (g.n3 + LdsGeo<D>::LT3 - 1) / LdsGeo<D>::LT3;
+    const int nt2 = (g.n2 + kLT2 - 1) / kLT2;
+    auto launch = [&](int chunk) {
+        const int nch = (g.n1 + chunk - 1) / chunk;
+        const int nb = nt3 * nt2 * nch;
+        if constexpr (PAIR)
+            hipLaunchKernelGGL((k_gather_lds_pair<Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
+                               reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
+        else
+            hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
+                               reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
+    };
+    const ShapeKey key{PAIR ? 100 + D : D, g.n1, g.n2, g.n3, nz};
+    tuned_launch(key, g.n1, st, launch, [&] {
+        return choose_chunk<D>(g.n1, static_cast<int64_t>(nt3) * nt2 * nz, PAIR ? 2 : (D <= 2 ? 3 : 1));
+    });
 }
 
 template <class Epi>
