@@ -459,13 +459,32 @@ def main():
               "frac": round(g[2] / (g[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "avg_us": round(1e3 * g[0] / g[1], 2),
               "bytes_per_point": 20}
 
+    # The whole matvec against HBM: the compulsory traffic of the reference's
+    # structure (every array read and written once per operation, no caches:
+    # incremental state 36 + 152·nt B/pt, incremental adjoint 100 + 92·nt,
+    # the hd scaling 24, the regularization 24, the final AXPY 36, the
+    # flat/component copies 48), and the algorithmic bytes of this path's
+    # launches (the per-kernel sums above, with the Newton-invariant caches).
+    ref_bytes = (268.0 + 244.0 * nt) * n ** 3
+    path_bytes = sum(p[2] for p in prof.values()) / args.steps
+    matvec_roofline = {
+        "reference_structure_bytes": ref_bytes,
+        "reference_structure_bound_per_s": round(HBM_PEAK_GBS * 1e9 / ref_bytes, 1),
+        "reference_structure_GBps": round(ref_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "path_bytes": path_bytes,
+        "path_GBps": round(path_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "path_frac": round(path_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"matvec: {ref_bytes / 1e9:.1f} GB reference-structure traffic (bound "
+        f"{matvec_roofline['reference_structure_bound_per_s']} matvec/s at 8 TB/s), "
+        f"{path_bytes / 1e9:.1f} GB moved by this path at {matvec_roofline['path_GBps']} GB/s")
+
     out = {"metric": METRIC, "value": round(value, 3), "unit": "matvec/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"GN-SL Hessian matvec, {n}^3 FP32, nt={nt}, H1SN beta=1e-2, band-limited v "
                                   f"|k|<=8 CFL 2, 24 Gaussian blobs",
                       "grid": list(shape), "nt": nt, "regnorm": regnorm, "beta": list(beta[:2]), "precision": "fp32"},
-           "roofline": roofline, "interp": interp, "kernels": kernels,
+           "roofline": roofline, "matvec_roofline": matvec_roofline, "interp": interp, "kernels": kernels,
            "device_bytes": c.device_bytes()}
 
     Hgpu = hv.download()
