@@ -112,7 +112,8 @@ class GNOptions(ctypes.Structure):
     """clr_gn_options (include/claire_amd.h)."""
     _fields_ = [("maxit", ctypes.c_int), ("gttol", ctypes.c_double), ("gatol", ctypes.c_double),
                 ("krylov_maxit", ctypes.c_int), ("krylov_rtol", ctypes.c_double), ("forcing", ctypes.c_int),
-                ("precond", ctypes.c_int), ("linesearch", ctypes.c_int), ("monitor", ctypes.c_int)]
+                ("precond", ctypes.c_int), ("linesearch", ctypes.c_int), ("monitor", ctypes.c_int),
+                ("ls_sigma", ctypes.c_double)]
 
 
 class GNReport(ctypes.Structure):
@@ -396,7 +397,8 @@ class Claire:
     def solve_gn(self, v0=None, hist_len=128, **options):
         """Gauss–Newton–Krylov registration (clr_solve_gn) from v0 (zero by
         default).  Options override the reference defaults (maxit, gttol,
-        gatol, krylov_maxit, krylov_rtol, forcing, precond, linesearch).
+        gatol, krylov_maxit, krylov_rtol, forcing, precond, linesearch,
+        monitor, ls_sigma).
         Returns (v, report dict, history rows (iter, J, D, R, ‖g‖, ‖g‖/‖g₀‖,
         pcg_its, step))."""
         opt = GNOptions()

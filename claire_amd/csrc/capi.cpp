@@ -72,6 +72,9 @@ reg::GNOptions gn_options(const clr_gn_options& opt) {
     o.precond = opt.precond;
     o.monitor = opt.monitor;
     o.linesearch = opt.linesearch;
+    CLR_REQUIRE(opt.ls_sigma > 0.0 && opt.ls_sigma < 1.0, claire_amd::kErrInvalidArgument,
+                "line-search sigma must be in (0, 1)");
+    o.ls_sigma = opt.ls_sigma;
     return o;
 }
 
@@ -371,8 +374,8 @@ int clr_gn_options_default(clr_gn_options* opt) {
     return guard([&] {
         need(opt, "opt");
         const reg::GNOptions d;
-        *opt = clr_gn_options{d.maxit, d.gttol, d.gatol, d.krylov_maxit, d.krylov_rtol, d.forcing, d.precond,
-                              d.linesearch, d.monitor};
+        *opt = clr_gn_options{d.maxit,   d.gttol,      d.gatol,   d.krylov_maxit, d.krylov_rtol,
+                              d.forcing, d.precond,    d.linesearch, d.monitor,   d.ls_sigma};
     });
 }
 

@@ -64,8 +64,10 @@ void Profiler::begin(const char* name, double bytes) {
 void Profiler::end() { CLR_HIP_CHECK(hipEventRecord(pending_.back().b, stream_)); }
 
 void Profiler::collect() {
-    CLR_HIP_CHECK(hipStreamSynchronize(stream_));
     for (auto& p : pending_) {
+        // each pair waits on its own end event: the pairs may sit on
+        // different streams when the problem moved between streams
+        CLR_HIP_CHECK(hipEventSynchronize(p.b));
         float ms = 0.f;
         CLR_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
         Entry& e = totals_[p.name];
@@ -345,8 +347,8 @@ int Regularization::EvaluateFunctional(double* R, const VecField* v) {
     if (m_Opt->beta[0] == 0.0f) return 0;
     const int64_t nl = m_Opt->domain.nl;
     const double hd = m_Opt->GetLebesgueMeasure();
-    VecField work(m_Opt);
-    float* w[3] = {work.m_X1, work.m_X2, work.m_X3};
+    if (!m_Work) m_Work.reset(new VecField(m_Opt));
+    float* w[3] = {m_Work->m_X1, m_Work->m_X2, m_Work->m_X3};
     const int norm = Norm();
     double acc = 0.0;
     double dots[Reducer::kSlots];
@@ -408,10 +410,10 @@ int Regularization::GetExtremeEigValsInvOp(ScalarType& emin, ScalarType& emax) {
             emin = 1.0f / regop;
             emax = 1.0f / b2;
             break;
-        default:  // src/RegularizationH2SN.cpp:447-455
+        default:  // src/RegularizationH2SN.cpp:447-455 ("by definition; it's 1/0")
             regop = b1 * (regop * regop);
             emin = 1.0f / regop;
-            emax = 1.0f / b1;
+            emax = 1.0f;
             break;
     }
     return 0;
@@ -448,12 +450,16 @@ int SemiLagrangian::ComputeTrajectory(const VecField* v, const std::string& flag
         }
         if (vc[1] - vc[0] == nl && vc[2] - vc[1] == nl) {
             m_W.resize(3 * static_cast<size_t>(nl));
+            m_F2.resize(3 * static_cast<size_t>(nl));
+            // stage sums: stage 1 writes f into m_F, stage 2 reads m_F and
+            // writes m_F2, stage 3 reads m_F2
+            float* const fbuf[4] = {nullptr, m_F.get(), m_F2.get(), nullptr};
             for (int stage = 1; stage <= 3 && db >= 0 && db <= dmax; ++stage) {
                 {
                     // 3 × (plan 12 + gathered v 4 + aux 4 + next point 4 (+ f 4)) per point
                     KernelScope k(m_Opt->prof, "gather_traj_rk4", (stage < 3 ? 84.0 : 72.0) * nl);
-                    launch_lds_traj_rk4_stage(stage, m_Opt->domain, m_P[w].get(), db, vc, sigma, ht, m_F.get(),
-                                              m_W.get(), m_Opt->stream);
+                    launch_lds_traj_rk4_stage(stage, m_Opt->domain, m_P[w].get(), db, vc, sigma, ht,
+                                              fbuf[stage - 1], fbuf[stage], m_W.get(), m_Opt->stream);
                 }
                 // bytes: point components read (12) + plan (12) (+ X interleaved (12) at the end)
                 KernelScope k(m_Opt->prof, "gather_plan", (stage < 3 ? 24.0 : 36.0) * nl);
@@ -552,10 +558,11 @@ void SemiLagrangian::Gather3(const float* const in[3], float* const out[3], cons
 int SemiLagrangian::Interpolate(float* xo, const float* xi, const std::string& flag) {
     const GatherDims g = gather_dims(m_Opt->domain);
     if (xo == xi) {
-        DeviceArray<float> tmp(static_cast<size_t>(g.nl));
-        Gather(tmp.get(), xi, flag);
-        launch_copy(tmp.get(), xo, g.nl, m_Opt->stream);
-        CLR_HIP_CHECK(hipStreamSynchronize(m_Opt->stream));
+        // in place: gather into the kept work buffer, then copy back
+        // (stream-ordered, so the buffer is free for the next call)
+        if (m_Tmp.size() < static_cast<size_t>(g.nl)) m_Tmp.resize(static_cast<size_t>(g.nl));
+        Gather(m_Tmp.get(), xi, flag);
+        launch_copy(m_Tmp.get(), xo, g.nl, m_Opt->stream);
         return 0;
     }
     Gather(xo, xi, flag);
@@ -565,11 +572,11 @@ int SemiLagrangian::Interpolate(float* xo, const float* xi, const std::string& f
 int SemiLagrangian::Interpolate(VecField* vo, const VecField* vi, const std::string& flag) {
     const float* in[3] = {vi->m_X1, vi->m_X2, vi->m_X3};
     if (vo->m_X1 == vi->m_X1) {
-        VecField tmp(m_Opt);
-        float* out[3] = {tmp.m_X1, tmp.m_X2, tmp.m_X3};
+        const int64_t nl = m_Opt->domain.nl;
+        if (m_Tmp.size() < static_cast<size_t>(3 * nl)) m_Tmp.resize(static_cast<size_t>(3 * nl));
+        float* out[3] = {m_Tmp.get(), m_Tmp.get() + nl, m_Tmp.get() + 2 * nl};
         Gather3(in, out, flag);
-        vo->Copy(&tmp);
-        CLR_HIP_CHECK(hipStreamSynchronize(m_Opt->stream));
+        vo->SetComponents(m_Tmp.get());
         return 0;
     }
     float* out[3] = {vo->m_X1, vo->m_X2, vo->m_X3};
@@ -639,7 +646,8 @@ size_t CLAIRE::DeviceBytes() const {
                           nt * 3 * nl + nt * nl + nt * nl + nl + 2 * 3 * nl;
     const size_t halves = m_Opt.precision == 1 ? (2 * nt + 1) * 3 * nl : 0;
     const size_t scales = m_Opt.precision == 1 ? (2 * nt + 1) * 3 * static_cast<size_t>(bfp16_blocks(nl)) : 0;
-    const size_t rk4 = 3 * nl;  // RK4 stage sum, or the RK2 corrector output
+    // the corrector / stage-point output, plus the two RK4 stage sums
+    const size_t rk4 = 3 * nl + (m_Opt.rkorder == 4 ? 6 * nl : 0);
     const size_t cont = m_Opt.pdetype == 0 ? nl : 0;      // continuity-equation coefficient
     const size_t rk2 = m_Opt.pdesolver == 0 ? 6 * nl : 0;  // RK2 integrator work
     return (floats + rk4 + cont + rk2 + scales) * 4 + halves * 2 + 3 * static_cast<size_t>(m_Opt.domain.nlc) * 8;
@@ -795,20 +803,22 @@ int CLAIRE::ComputeDetDefGrad(bool inverse, float* jac, double stats[3]) {
     const int64_t nl = m_Opt.domain.nl;
     const int nt = m_Opt.nt;
     const GatherDims g = gather_dims(m_Opt.domain);
-    DeviceArray<float> divv(static_cast<size_t>(nl)), coeff(static_cast<size_t>(nl));
+    // work space of the problem, free outside a solve: ∇·v in m_Scalar, the
+    // coefficient in the first slot of the inc-state decrements m_S
+    float* divv = m_Scalar.get();
+    float* coeff = m_S.get();
     const float* vc[3] = {m_VelocityField->m_X1, m_VelocityField->m_X2, m_VelocityField->m_X3};
-    m_Spectral->Div(divv.get(), vc);
+    m_Spectral->Div(divv, vc);
     const TrajectoryPlan tp = m_SL->Plan("state");
     {
         KernelScope k(m_Opt.prof, "gather_det_coeff", 24.0 * nl);
-        launch_plan_adjoint_coeff(tp, g, divv.get(), m_Opt.GetTimeStepSize(), inverse ? -1.0f : 1.0f, coeff.get(),
-                                  m_Opt.stream);
+        launch_plan_adjoint_coeff(tp, g, divv, m_Opt.GetTimeStepSize(), inverse ? -1.0f : 1.0f, coeff, m_Opt.stream);
     }
     float* buf[2] = {m_IncState[0].get(), m_IncState[1].get()};
     launch_fill(buf[0], 1.0f, nl, m_Opt.stream);
     for (int j = 0; j < nt; ++j) {
         KernelScope k(m_Opt.prof, "det_step", 24.0 * nl);
-        launch_plan_adjoint_step(tp, g, coeff.get(), buf[j % 2], buf[(j + 1) % 2], m_Opt.stream);
+        launch_plan_adjoint_step(tp, g, coeff, buf[j % 2], buf[(j + 1) % 2], m_Opt.stream);
     }
     const float* J = buf[nt % 2];
     stats[0] = m_Red->Min(J, nl);

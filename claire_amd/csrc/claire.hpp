@@ -76,6 +76,9 @@ class Profiler {
     ~Profiler();
     bool enabled() const { return enabled_; }
     void enable(bool on);
+    // Stream of later begin/end events (the problem's stream follows
+    // CLAIRE::SetStream; events already recorded keep theirs).
+    void set_stream(hipStream_t s) { stream_ = s; }
     void begin(const char* name, double bytes);
     void end();
     // Synchronize, fold the recorded events into the totals.
@@ -235,6 +238,9 @@ class Regularization {
     const RegOpt* m_Opt;
     SpectralOps* m_Spec;
     Reducer* m_Red;
+    // work field of EvaluateFunctional, allocated on its first call and kept
+    // (no allocation inside the Newton loop)
+    std::unique_ptr<VecField> m_Work;
 };
 
 class RegularizationH1 : public Regularization {
@@ -284,8 +290,10 @@ class SemiLagrangian {
     const RegOpt* m_Opt;
     DeviceArray<float> m_X[2];
     DeviceArray<uint32_t> m_P[2];
-    DeviceArray<float> m_F;  // RK4 stage sum f (m_WorkVecField2 of the reference)
+    DeviceArray<float> m_F;   // RK4 stage sum f (m_WorkVecField2 of the reference)
+    DeviceArray<float> m_F2;  // RK4 stage sum after stage 2 (LDS path; kept apart from m_F)
     DeviceArray<float> m_W;  // RK2 corrector output, one component per field (LDS path)
+    DeviceArray<float> m_Tmp;  // in-place interpolation (the reference's ghost copy), kept between calls
     DeviceArray<int> m_Scratch;
     bool m_Valid[2] = {false, false};
     int m_D[2] = {0, 0};
@@ -348,8 +356,12 @@ class CLAIRE {
     // The model's ApplyProjection on a flat vector field, in place (a no-op
     // for the compressible model, src/CLAIRE.cpp:4044-4048).
     int ApplyProjection(float* v);
-    // Stream of every later launch (work already queued stays where it is).
-    void SetStream(hipStream_t s) { m_Opt.stream = s; }
+    // Stream of every later launch and of the profiler's events (work
+    // already queued stays where it is).
+    void SetStream(hipStream_t s) {
+        m_Opt.stream = s;
+        if (m_Opt.prof) m_Opt.prof->set_stream(s);
+    }
     // Capture the Hessian matvec into HIP graphs and replay them (on/off).
     int SetMatVecGraphs(bool on);
     int MatVecGraphCount() const { return static_cast<int>(m_Graphs.size()); }

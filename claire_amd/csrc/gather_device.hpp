@@ -283,9 +283,10 @@ struct EpiTrajRk2 {
 // component z, gathered along the plan of the previous stage point, with the
 // float/double steps of k_traj_rk4:
 //   STAGE 1 (aux v_z):  f = v + 2·v(X¹) (double), out X² = (x − σ·ht/2·v(X¹))/2π;
-//   STAGE 2 (aux f_z):  f += 2·v(X²) (double),    out X³ = (x − σ·ht·v(X²))/2π;
-//   STAGE 3 (aux f_z):  out X = (x − σ·(ht/6)·(f + v(X³)))/2π (double).
-// out holds the next point per component (out + z·zstride), out2 = f.
+//   STAGE 2 (aux f_z):  f' = f + 2·v(X²) (double), out X³ = (x − σ·ht·v(X²))/2π;
+//   STAGE 3 (aux f'_z): out X = (x − σ·(ht/6)·(f' + v(X³)))/2π (double).
+// out holds the next point per component (out + z·zstride), out2 = the new
+// stage sum, in a buffer other than aux (stage 3 has no out2).
 template <int STAGE>
 struct EpiTrajRk4 {
     static constexpr bool kHasAux = true;

@@ -686,7 +686,9 @@ int choose_chunk(int n1, int64_t tiles, int per_cu) {
 // lengths on the launch stream and keeps the fastest for every later launch
 // of that shape.  The chunk length only changes how the planes are split
 // among blocks, not the arithmetic of any point, so every candidate writes
-// the same output.  Tuning is skipped while the stream is being captured into
+// the same output; repeating a launch is harmless because no gather writes a
+// buffer it reads (the launchers check it: the steps refuse to run in place,
+// and the RK4 stage sums ping-pong between two buffers).  Tuning is skipped while the stream is being captured into
 // a graph; CLAIRE_AMD_GATHER_TUNE=0 turns it off (the heuristic is used).
 struct ShapeKey {
     int d, n1, n2, n3, nz;
@@ -835,24 +837,28 @@ void launch_lds_traj_rk2_corrector(const Dims& d, const uint32_t* P, int dbound,
 }
 
 void launch_lds_traj_rk4_stage(int stage, const Dims& d, const uint32_t* P, int dbound, const float* const v[3],
-                               float sigma, float ht, float* F, float* W, hipStream_t st) {
+                               float sigma, float ht, const float* Fin, float* Fout, float* W, hipStream_t st) {
     const GatherDims g = gather_dims(d);
     const int64_t vz = v[1] - v[0];
     CLR_REQUIRE(v[2] - v[1] == vz && vz == g.nl, kErrInvalidArgument, "velocity components must be contiguous");
+    // the tuner may repeat a launch: no stage may overwrite what it reads
+    CLR_REQUIRE(stage == 1 || Fin != nullptr, kErrInvalidArgument, "RK4 stage needs the previous stage sum");
+    CLR_REQUIRE(stage == 3 || (Fout != nullptr && Fout != Fin), kErrInvalidArgument,
+                "RK4 stage sum must go to a buffer distinct from its input");
     const float hx[3] = {d.hx[0], d.hx[1], d.hx[2]};
     const float hthalf = 0.5f * ht;
     const double s6 = static_cast<double>(sigma) * (static_cast<double>(ht) / 6.0);
     switch (stage) {
         case 1: {
-            EpiTrajRk4<1> e{v[0], W, F, g.nl, {hx[0], hx[1], hx[2]}, sigma * hthalf, s6, 0};
+            EpiTrajRk4<1> e{v[0], W, Fout, g.nl, {hx[0], hx[1], hx[2]}, sigma * hthalf, s6, 0};
             return dispatch(dbound, g, P, v[0], vz, 3, e, st);
         }
         case 2: {
-            EpiTrajRk4<2> e{F, W, F, g.nl, {hx[0], hx[1], hx[2]}, sigma * ht, s6, 0};
+            EpiTrajRk4<2> e{Fin, W, Fout, g.nl, {hx[0], hx[1], hx[2]}, sigma * ht, s6, 0};
             return dispatch(dbound, g, P, v[0], vz, 3, e, st);
         }
         case 3: {
-            EpiTrajRk4<3> e{F, W, F, g.nl, {hx[0], hx[1], hx[2]}, 0.0f, s6, 0};
+            EpiTrajRk4<3> e{Fin, W, W, g.nl, {hx[0], hx[1], hx[2]}, 0.0f, s6, 0};  // no second output
             return dispatch(dbound, g, P, v[0], vz, 3, e, st);
         }
         default:

@@ -100,11 +100,14 @@ void launch_lds_traj_rk2_corrector(const Dims& d, const uint32_t* P, int dbound,
 int launch_make_plan_soa(const GatherDims& g, const float* W, float* X, uint32_t* P, int* scratch, hipStream_t st);
 // RK4 characteristic the same way: the first stage point (x − σ·ht/2·v)/2π
 // is launch_rk2_predictor_plan with ht/2; each stage gathers v along the
-// previous point's plan (bound <= 4), accumulates f in F and writes the next
-// point per component into W (the final X at stage 3), whose plan
-// launch_make_plan_soa builds (X null for the intermediate points).
+// previous point's plan (bound <= 4), accumulates the stage sum f and writes
+// the next point per component into W (the final X at stage 3), whose plan
+// launch_make_plan_soa builds (X null for the intermediate points).  Stage 1
+// writes f into Fout (Fin unused), stage 2 reads Fin and writes Fout (two
+// distinct buffers: no launch updates an operand in place, so a gather may
+// be repeated while its chunk length is tuned), stage 3 reads Fin.
 void launch_lds_traj_rk4_stage(int stage, const Dims& d, const uint32_t* P, int dbound, const float* const v[3],
-                               float sigma, float ht, float* F, float* W, hipStream_t st);
+                               float sigma, float ht, const float* Fin, float* Fout, float* W, hipStream_t st);
 // The LDS-staged gathers need D <= lds_gather_max_displacement().
 int lds_gather_max_displacement();
 // Update coefficient per point: C = 1 + ht/2·(α·dX + α·(1 + ht·α·dX)·d)

@@ -162,14 +162,18 @@ int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, 
         // TaoMonitor runs before the convergence test, the step length being
         // the last line search's (1 before the first)
         if (opt.monitor) monitor_line(iter, J, rep.J0, D, rep.D0, gnorm, rep.gnorm0, step);
-        // CheckConvergenceGrad (src/TaoInterface.cpp:421-480)
-        if (!std::isfinite(J) || !std::isfinite(gnorm)) { rep.reason = kGNNan; break; }
-        if (step < 1.0 / 1024.0) { rep.reason = kGNSteptol; break; }
-        if (gnorm < opt.gttol * g0) { rep.reason = kGNGttol; break; }
-        if (gnorm < opt.gatol) { rep.reason = kGNGatol; break; }
-        if (iter > opt.maxit - 1) { rep.reason = kGNMaxits; break; }
+        // CheckConvergenceGrad (src/TaoInterface.cpp:421-480): a non-finite J
+        // or ‖g‖ first, then the step length; of the three tolerance tests
+        // every one that holds sets the reason in turn, so the last wins
+        // (GATOL over GTTOL, MAXITS over both).  TAO's iteration limit is
+        // maxit − 1 (src/Optimizer.cpp:431).
+        rep.reason = convergence_reason(J, gnorm, step, g0, iter, opt.gttol, opt.gatol, opt.maxit);
+        if (rep.reason != kGNContinue) break;
 
-        // Eisenstat–Walker forcing (src/KrylovInterface.cpp:120-160)
+        // Eisenstat–Walker forcing (src/KrylovInterface.cpp:120-160).  The
+        // ratio is ‖g‖/‖g_first‖ for every operator type: with PRECONDMATVEC
+        // and PRECONDMATVECSYM the reference takes ‖b‖ before PreKrylovSolve
+        // transforms b (src/KrylovInterface.cpp:104-108, :114), and ‖b‖ = ‖g‖.
         if (kg0 < 0.0) kg0 = gnorm;
         double rtol = opt.krylov_rtol;
         if (opt.forcing != 0 && kg0 > 0.0) {
@@ -199,7 +203,7 @@ int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, 
                 launch_waxpy(static_cast<float>(alpha), d, v, vt, m_N, Stream());
                 m_Prob->EvaluateObjective(&Jt, &Dt, &Rt, vt);
                 rep.objective_evals++;
-                if (!opt.linesearch || (std::isfinite(Jt) && Jt <= J + 1e-4 * alpha * gd)) {
+                if (!opt.linesearch || (std::isfinite(Jt) && Jt <= J + opt.ls_sigma * alpha * gd)) {
                     ok = true;
                     break;
                 }

@@ -9,6 +9,8 @@
 // products are FP64 reductions; the host only sees scalars.
 #pragma once
 
+#include <cmath>
+
 #include "claire.hpp"
 
 namespace reg {
@@ -23,6 +25,7 @@ struct GNOptions {
     int precond = 1;             // 0 none, 1 inverse regularization (INVREG)
     int linesearch = 1;          // 0 unit step, 1 Armijo
     int monitor = 0;             // 1: print the iteration table (OptimizationMonitor)
+    double ls_sigma = 1e-4;      // Armijo sufficient decrease (PETSc armijo sigma)
 };
 
 struct GNReport {
@@ -33,6 +36,24 @@ struct GNReport {
 // Stop reasons (TAO_CONVERGED_* > 0, TAO_DIVERGED_* < 0).
 enum GNReason { kGNContinue = 0, kGNGttol = 1, kGNGatol = 2, kGNSteptol = 3, kGNMaxits = -1, kGNNan = -2,
                 kGNLineSearch = -3 };
+
+// The gradient stopping test CheckConvergenceGrad (src/TaoInterface.cpp:
+// 390-512) with the default miniter = 0 (src/RegOpt.cpp:1158): a non-finite
+// J or ‖g‖ stops first (TAO_DIVERGED_NAN), then a line-search step below
+// 2^-10 (TAO_CONVERGED_STEPTOL); then ‖g‖ < gttol·‖g₀‖ (GTTOL), ‖g‖ < gatol
+// (GATOL) and iter > maxit − 1 (MAXITS, TAO's limit being maxit − 1,
+// src/Optimizer.cpp:431) each set the reason in that order, so the last that
+// holds wins.  g0 is ‖g₀‖, or 1 when ‖g₀‖ = 0.
+inline int convergence_reason(double J, double gnorm, double step, double g0, int iter, double gttol, double gatol,
+                              int maxit) {
+    if (!std::isfinite(J) || !std::isfinite(gnorm)) return kGNNan;
+    if (step < 1.0 / 1024.0) return kGNSteptol;
+    int reason = kGNContinue;
+    if (gnorm < gttol * g0) reason = kGNGttol;
+    if (gnorm < gatol) reason = kGNGatol;
+    if (iter > maxit - 1) reason = kGNMaxits;
+    return reason;
+}
 
 class Optimizer {
   public:

@@ -275,6 +275,10 @@ typedef struct clr_gn_options {
                             iteration as OptimizationMonitor
                             (src/TaoInterface.cpp:525-592); batch solves stay
                             silent */
+    double ls_sigma;     /* sufficient-decrease constant of the Armijo line
+                            search, J(v + αd) <= J + σ·α·gᵀd (PETSc's
+                            TAOLINESEARCHARMIJO, selected at src/Optimizer.cpp:390,
+                            option -tao_ls_armijo_sigma, default 1e-4); in (0, 1) */
 } clr_gn_options;
 
 #define CLR_GN_GTTOL 1      /* ‖g‖ < gttol·‖g₀‖ (TAO_CONVERGED_GTTOL) */

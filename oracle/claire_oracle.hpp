@@ -531,7 +531,9 @@ void reg_eigvals_inv(const Grid<T>& g, int norm, const T* beta, T* emin, T* emax
         case kH1:   regop = beta[0] * (-regop + beta[1]); *emin = T(1) / regop; *emax = T(1) / beta[1]; break;
         case kH2:   regop = beta[0] * (regop * regop) + beta[1]; *emin = T(1) / regop; *emax = T(1) / beta[1]; break;
         case kH1SN: regop = -beta[0] * regop + beta[1]; *emin = T(1) / regop; *emax = T(1) / beta[1]; break;
-        case kH2SN: regop = beta[0] * (regop * regop); *emin = T(1) / regop; *emax = T(1) / beta[0]; break;
+        // emax = 1 for H2SN: the operator has no constant term ("by definition;
+        // it's 1/0", src/RegularizationH2SN.cpp:453)
+        case kH2SN: regop = beta[0] * (regop * regop); *emin = T(1) / regop; *emax = T(1); break;
         default: throw std::invalid_argument("unknown regularization norm");
     }
 }
@@ -1309,6 +1311,24 @@ struct GNReport {
 enum GNReason { kGNContinue = 0, kGNGttol = 1, kGNGatol = 2, kGNSteptol = 3, kGNMaxits = -1, kGNNan = -2,
                 kGNLineSearch = -3 };
 
+// CheckConvergenceGrad (src/TaoInterface.cpp:421-480), miniter = 0
+// (src/RegOpt.cpp:1158): TAO_DIVERGED_NAN (:424-436) and
+// TAO_CONVERGED_STEPTOL (:445-453) return at once; TAO_CONVERGED_GTTOL
+// (:456-459), TAO_CONVERGED_GATOL (:467-470) and TAO_DIVERGED_MAXITS
+// (:478-481) are set one after the other, the last holding one standing
+// (then :491-494 stops).
+// TAO's iteration limit is maxiter − 1 (src/Optimizer.cpp:431).
+inline int convergence_reason(double J, double gnorm, double step, double g0, int iter, double gttol, double gatol,
+                              int maxit) {
+    if (std::isnan(J) || std::isinf(J) || std::isnan(gnorm) || std::isinf(gnorm)) return kGNNan;
+    if (step < 1.0 / 1024.0) return kGNSteptol;
+    int r = kGNContinue;
+    if (gnorm < gttol * g0) r = kGNGttol;
+    if (gnorm < gatol) r = kGNGatol;
+    if (iter > maxit - 1) r = kGNMaxits;
+    return r;
+}
+
 template <class T>
 double dot3(const Grid<T>& g, const T* const a[3], const T* const b[3]) {
     double s = 0.0;
@@ -1355,12 +1375,10 @@ void gn_solve(const Grid<T>& g, int nt, int norm, const T* beta, T* const v[3], 
         if (h) {
             h[0] = iter; h[1] = J; h[2] = D; h[3] = R; h[4] = gnorm; h[5] = gnorm / g0; h[6] = 0; h[7] = 0;
         }
-        // CheckConvergenceGrad
-        if (!std::isfinite(J) || !std::isfinite(gnorm)) { rep.reason = kGNNan; break; }
-        if (step < 1.0 / 1024.0) { rep.reason = kGNSteptol; break; }
-        if (gnorm < o.gttol * g0) { rep.reason = kGNGttol; break; }
-        if (gnorm < o.gatol) { rep.reason = kGNGatol; break; }
-        if (iter > o.maxit - 1) { rep.reason = kGNMaxits; break; }
+        // CheckConvergenceGrad (src/TaoInterface.cpp:421-480): NaN, then the
+        // step; of GTTOL, GATOL and MAXITS the last that holds wins
+        rep.reason = convergence_reason(J, gnorm, step, g0, iter, o.gttol, o.gatol, o.maxit);
+        if (rep.reason != kGNContinue) break;
 
         // Newton system H d = −g at the current iterate
         solve_state(g, nt, V.data(), mT, m.data());

@@ -171,16 +171,57 @@ def test_reg_functional_parseval(oracle, norm):
     assert abs(oracle.reg_functional(v, norm, beta) - expect) < 1e-11 * abs(expect)
 
 
-def test_reg_eigvals_inv_reference_formulas(oracle):
-    n = (16, 32, 8)
-    b0, b1 = 1e-2, 1e-4
-    lap = -(8.0 ** 2 + 16.0 ** 2 + 4.0 ** 2)
-    expect = {
-        "h1": (1 / (b0 * (-lap + b1)), 1 / b1),   # src/RegularizationH1.cpp:444-448
-        "h2": (1 / (b0 * lap * lap + b1), 1 / b1),  # src/RegularizationH2.cpp:443-446
-        "h1sn": (1 / (-b0 * lap + b1), 1 / b1),   # src/RegularizationH1SN.cpp:420-424
-        "h2sn": (1 / (b0 * lap * lap), 1 / b0),   # src/RegularizationH2SN.cpp:451-455
-    }
-    for norm, (lo, hi) in expect.items():
-        got = oracle.reg_eigvals_inv(n, norm, (b0, b1, 0.0))
-        assert abs(got[0] - lo) < 1e-12 * lo and abs(got[1] - hi) < 1e-12 * hi
+def _inverse_eigenvalue(oracle, shape, norm, beta, w):
+    """Eigenvalue of the oracle's ApplyInverse on the real Fourier mode
+    cos(w·x) of a grid of the given shape, read off by applying the operator
+    to the mode (brute force, no formula)."""
+    x = np.meshgrid(*[2 * np.pi * np.arange(n) / n for n in shape], indexing="ij")
+    mode = np.cos(w[0] * x[0] + w[1] * x[1] + w[2] * x[2])
+    v = np.stack([mode, 0.5 * mode, -2.0 * mode])
+    out = oracle.reg_inverse(v, norm, beta)
+    lam = float(np.sum(out * v) / np.sum(v * v))
+    # it is an eigenpair; the residual is the round-off mean of the sampled
+    # mode, amplified by the constant mode's eigenvalue (up to 1/(β0·β1))
+    assert relerr(out, lam * v) < 1e-8
+    return lam
+
+
+@pytest.mark.parametrize("norm", ["h1", "h2", "h1sn", "h2sn"])
+def test_reg_eigvals_inv_brute_force(oracle, norm):
+    """GetExtremeEigValsInvOp (src/RegularizationH1.cpp:427-453,
+    RegularizationH2.cpp:426-450, RegularizationH1SN.cpp:404-430,
+    RegularizationH2SN.cpp:435-460) against eigenvalues of the inverse
+    operator found by applying it to Fourier modes.
+
+    emin is the inverse's eigenvalue at the corner wave vector (N1/2, N2/2,
+    N3/2).  On the N-grid that is the Nyquist mode, which the operator zeroes,
+    so it is measured on a grid twice as fine, where the same wave vector is
+    an ordinary mode.  The reference's formulas equal the operator's
+    eigenvalues only for some weights: H1 for any β; H2 and H1SN when the
+    formula's β1 term matches the operator's (H2 at β0 = 1, H1SN at
+    β1 = 0; H2 keeps β1 > 0, its inverse has no constant-mode fallback);
+    H2SN for any β.  emax is the
+    eigenvalue of the constant mode where the formula agrees with it (H1 and
+    H2 at β0 = 1, H1SN at β1 = β0, H2SN at β = 1); for other weights the
+    formula is the reference's quirk (1/β1, or 1 for H2SN) and is kept as
+    written, unpinned."""
+    n = (4, 6, 8)
+    fine = tuple(2 * k for k in n)
+    corner = tuple(k // 2 for k in n)
+    emin_beta = {"h1": (1e-2, 1e-4), "h2": (1.0, 1e-3), "h1sn": (1e-2, 0.0), "h2sn": (1e-3, 0.0)}[norm]
+    lo, _ = oracle.reg_eigvals_inv(n, norm, emin_beta + (0.0,))
+    lam = _inverse_eigenvalue(oracle, fine, norm, emin_beta + (0.0,), corner)
+    assert abs(lo - lam) <= 1e-12 * lam, (lo, lam)
+    # the corner bounds every mode of the N-grid from below (the inverse
+    # decreases with |w|): brute force over all wave vectors of the N-grid
+    for w in itertools.product(*[range(k // 2) for k in n]):
+        if any(w):
+            assert _inverse_eigenvalue(oracle, n, norm, emin_beta + (0.0,), w) >= lo
+    emax_beta = {"h1": (1.0, 1e-3), "h2": (1.0, 1e-3), "h1sn": (2e-2, 2e-2), "h2sn": (1.0, 0.0)}[norm]
+    _, hi = oracle.reg_eigvals_inv(n, norm, emax_beta + (0.0,))
+    lam0 = _inverse_eigenvalue(oracle, n, norm, emax_beta + (0.0,), (0, 0, 0))
+    assert abs(hi - lam0) <= 1e-12 * lam0, (hi, lam0)
+    # a plausible slip -- 1/β0 for H2SN's emax, as an earlier restatement had
+    # -- is caught at other weights: the reference's value there is 1
+    if norm == "h2sn":
+        assert oracle.reg_eigvals_inv(n, norm, (1e-3, 0.0, 0.0))[1] == 1.0
