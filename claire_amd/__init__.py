@@ -160,6 +160,24 @@ def solve_gn_batch(problems, v0s=None, workers=4, **options):
     return out
 
 
+def grad_noalloc(f, stream=None):
+    """Spectral gradient of f (shape (n1, n2, n3)) through clr_grad_noalloc,
+    the context-free entry point that allocates no device memory; returns
+    the (3, n1, n2, n3) FP32 gradient.  The device buffers of the argument
+    and result are allocated here, by the caller, as the entry point
+    requires."""
+    f = np.ascontiguousarray(f, dtype=np.float32)
+    nx = (ctypes.c_int * 3)(*f.shape)
+    d = DeviceArray.from_host(f)
+    out = DeviceArray((3,) + f.shape)
+    try:
+        _check(lib().clr_grad_noalloc(nx, _vp(d.ptr), _vp(out.ptr), _vp(stream or 0)))
+        return out.download()
+    finally:
+        d.free()
+        out.free()
+
+
 class DeviceArray:
     """Device allocation owned by Python (clr_device_malloc/free)."""
 

@@ -578,6 +578,16 @@ int clr_div(clr_ctx* ctx, const float* v, float* d) {
     });
 }
 
+int clr_grad_noalloc(const int nx[3], const float* f, float* g, void* stream) {
+    return guard([&] {
+        need(nx, "nx");
+        need(f, "f");
+        need(g, "g");
+        claire_amd::grad_noalloc(nx, f, g, static_cast<hipStream_t>(stream));
+        check_hip_launch();
+    });
+}
+
 int clr_fft_r2c(clr_ctx* ctx, const float* in, float* out_complex) {
     return guard([&] {
         need(in, "in");

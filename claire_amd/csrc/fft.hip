@@ -803,22 +803,42 @@ bool fft_size_supported(int n) {
            n == 150 || n == 192;
 }
 
-FFTPlan3D::FFTPlan3D(const Dims& d) : dims(d) {
-    for (int a = 0; a < 3; ++a) {
+void fft_twiddles(int n, c32* h) {
+    for (int k = 0; k < n; ++k) {
+        const double ang = -2.0 * 3.14159265358979323846 * static_cast<double>(k) / static_cast<double>(n);
+        h[k] = {static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang))};
+    }
+}
+
+namespace {
+void require_fft_sizes(const Dims& d) {
+    for (int a = 0; a < 3; ++a)
         CLR_REQUIRE(fft_size_supported(d.n[a]), kErrUnsupportedSize,
                     "grid size " + std::to_string(d.n[a]) +
                         " not supported (powers of two 8..512, and 48, 96, 150, 192)");
+}
+}  // namespace
+
+FFTPlan3D::FFTPlan3D(const Dims& d) : dims(d) {
+    require_fft_sizes(d);
+    for (int a = 0; a < 3; ++a) {
         std::vector<c32> h(d.n[a]);
-        for (int k = 0; k < d.n[a]; ++k) {
-            const double ang = -2.0 * 3.14159265358979323846 * static_cast<double>(k) / static_cast<double>(d.n[a]);
-            h[k] = {static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang))};
-        }
+        fft_twiddles(d.n[a], h.data());
         CLR_HIP_CHECK(hipMalloc(&tw[a], sizeof(c32) * d.n[a]));
         CLR_HIP_CHECK(hipMemcpy(tw[a], h.data(), sizeof(c32) * d.n[a], hipMemcpyHostToDevice));
     }
 }
 
+FFTPlan3D::FFTPlan3D(const Dims& d, c32* const tw_ext[3]) : dims(d), owns_tw(false) {
+    require_fft_sizes(d);
+    for (int a = 0; a < 3; ++a) {
+        CLR_REQUIRE(tw_ext[a] != nullptr, kErrInvalidArgument, "missing twiddle table");
+        tw[a] = tw_ext[a];
+    }
+}
+
 FFTPlan3D::~FFTPlan3D() {
+    if (!owns_tw) return;
     for (int a = 0; a < 3; ++a)
         if (tw[a]) (void)hipFree(tw[a]);
 }

@@ -20,6 +20,10 @@ bool fft_size_supported(int n);
 class FFTPlan3D {
   public:
     explicit FFTPlan3D(const Dims& d);
+    // A plan over twiddle tables the caller keeps on the device (tw_ext[a]
+    // holds n[a] values e^{-2πik/n[a]}, as fft_twiddles writes them); the
+    // plan neither allocates nor frees device memory.
+    FFTPlan3D(const Dims& d, c32* const tw_ext[3]);
     ~FFTPlan3D();
     FFTPlan3D(const FFTPlan3D&) = delete;
     FFTPlan3D& operator=(const FFTPlan3D&) = delete;
@@ -53,7 +57,19 @@ class FFTPlan3D {
 
   private:
     c32* tw[3] = {nullptr, nullptr, nullptr};
+    bool owns_tw = true;
 };
+
+// The twiddle table of an n-point transform: h[k] = e^{-2πik/n}, k < n,
+// evaluated in double and rounded to FP32.
+void fft_twiddles(int n, c32* h);
+
+// Spectral gradient without device allocation (noalloc.cpp): g = ∇f with
+// the launches of SpectralOps::Grad, its spectra and twiddles in a
+// file-scope device array of kNoAllocScratchBytes per device.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+size_t noalloc_grad_bytes(const int nx[3]);
+void grad_noalloc(const int nx[3], const float* f, float* g, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Semi-Lagrangian gathers (interp.hip, gather_lds.hip).  X: normalized query

@@ -340,6 +340,18 @@ int clr_interp_points(clr_ctx* ctx, const float* f, const float* X, float* out);
  * unnormalized and leaves its input intact. */
 int clr_grad(clr_ctx* ctx, const float* f, float* g);
 int clr_div(clr_ctx* ctx, const float* v, float* d);
+/* clr_grad without a context and without device allocation: g = ∇f on an
+ * nx grid (g: [g1|g2|g3], 3·nl floats), launched on `stream` (null: the
+ * default stream) of the current device, with results bit for bit equal to
+ * clr_grad's on the same grid.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The three half spectra
+ * live in a 32 MiB device array per device that every call shares (grids
+ * with 3·nx0·nx1·(nx2/2+1)·8 bytes of spectra up to 32 MiB − 12 KiB, e.g.
+ * 128³ or 128×150×128; larger grids return CLR_ERR_UNSUPPORTED_SIZE), so
+ * calls that overlap on one device race.  The first call on a device looks
+ * the array up and writes its twiddle tables synchronously. */
+int clr_grad_noalloc(const int nx[3], const float* f, float* g, void* stream);
 int clr_fft_r2c(clr_ctx* ctx, const float* in, float* out_complex);
 int clr_fft_c2r(clr_ctx* ctx, const float* in_complex, float* out);
 
