@@ -113,7 +113,7 @@ class GNOptions(ctypes.Structure):
     _fields_ = [("maxit", ctypes.c_int), ("gttol", ctypes.c_double), ("gatol", ctypes.c_double),
                 ("krylov_maxit", ctypes.c_int), ("krylov_rtol", ctypes.c_double), ("forcing", ctypes.c_int),
                 ("precond", ctypes.c_int), ("linesearch", ctypes.c_int), ("monitor", ctypes.c_int),
-                ("ls_sigma", ctypes.c_double)]
+                ("ls_sigma", ctypes.c_double), ("ls_stepinit", ctypes.c_double), ("krylov_block", ctypes.c_int)]
 
 
 class GNReport(ctypes.Structure):
@@ -285,6 +285,10 @@ class Claire:
         d = self._in("v", v, (3,) + self.nx)
         _check(lib().clr_set_velocity(self.ctx, _vp(d.ptr)))
 
+    def set_velocity_device(self, d):
+        """set_velocity from a DeviceArray already in HBM."""
+        _check(lib().clr_set_velocity(self.ctx, _vp(d.ptr)))
+
     def set_inc_velocity(self, vt):
         d = self._in("vt", vt, (3,) + self.nx)
         _check(lib().clr_set_inc_velocity(self.ctx, _vp(d.ptr)))
@@ -416,7 +420,7 @@ class Claire:
         """Gauss–Newton–Krylov registration (clr_solve_gn) from v0 (zero by
         default).  Options override the reference defaults (maxit, gttol,
         gatol, krylov_maxit, krylov_rtol, forcing, precond, linesearch,
-        monitor, ls_sigma).
+        monitor, ls_sigma, ls_stepinit, krylov_block).
         Returns (v, report dict, history rows (iter, J, D, R, ‖g‖, ‖g‖/‖g₀‖,
         pcg_its, step))."""
         opt = GNOptions()
@@ -516,6 +520,12 @@ class Claire:
         db = DeviceArray.from_host(b)
         r = ctypes.c_double()
         _check(lib().clr_dot(self.ctx, _vp(da.ptr), _vp(db.ptr), ctypes.c_int64(da.nbytes // 4), ctypes.byref(r)))
+        return r.value
+
+    def norm2(self, a):
+        da = DeviceArray.from_host(a)
+        r = ctypes.c_double()
+        _check(lib().clr_norm2(self.ctx, _vp(da.ptr), ctypes.c_int64(da.nbytes // 4), ctypes.byref(r)))
         return r.value
 
     # -- kernel timing ---------------------------------------------------------

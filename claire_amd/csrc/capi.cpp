@@ -1,5 +1,6 @@
 // capi.cpp -- the extern "C" boundary (include/claire_amd.h).
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstring>
 #include <memory>
@@ -75,6 +76,11 @@ reg::GNOptions gn_options(const clr_gn_options& opt) {
     CLR_REQUIRE(opt.ls_sigma > 0.0 && opt.ls_sigma < 1.0, claire_amd::kErrInvalidArgument,
                 "line-search sigma must be in (0, 1)");
     o.ls_sigma = opt.ls_sigma;
+    CLR_REQUIRE(opt.ls_stepinit > 0.0 && std::isfinite(opt.ls_stepinit), claire_amd::kErrInvalidArgument,
+                "initial line-search step must be positive");
+    o.ls_stepinit = opt.ls_stepinit;
+    CLR_REQUIRE(opt.krylov_block >= 1, claire_amd::kErrInvalidArgument, "krylov_block must be >= 1");
+    o.krylov_block = opt.krylov_block;
     return o;
 }
 
@@ -375,7 +381,8 @@ int clr_gn_options_default(clr_gn_options* opt) {
         need(opt, "opt");
         const reg::GNOptions d;
         *opt = clr_gn_options{d.maxit,   d.gttol,      d.gatol,   d.krylov_maxit, d.krylov_rtol,
-                              d.forcing, d.precond,    d.linesearch, d.monitor,   d.ls_sigma};
+                              d.forcing, d.precond,    d.linesearch, d.monitor,   d.ls_sigma,
+                              d.ls_stepinit, d.krylov_block};
     });
 }
 
@@ -617,6 +624,7 @@ int clr_dot(clr_ctx* ctx, const float* a, const float* b, int64_t n, double* out
         need(a, "a");
         need(b, "b");
         need(out, "out");
+        CLR_REQUIRE(n >= 0, claire_amd::kErrInvalidArgument, "negative length");
         *out = C(ctx).Red()->Dot(a, b, n);
     });
 }
@@ -625,8 +633,9 @@ int clr_norm2(clr_ctx* ctx, const float* a, int64_t n, double* out) {
     return guard([&] {
         need(a, "a");
         need(out, "out");
-        const double s = C(ctx).Red()->Dot(a, a, n);
-        *out = s > 0 ? std::sqrt(s) : 0.0;
+        CLR_REQUIRE(n >= 0, claire_amd::kErrInvalidArgument, "negative length");
+        // a sum of squares: non-negative, or NaN when an entry is not finite
+        *out = std::sqrt(C(ctx).Red()->Dot(a, a, n));
     });
 }
 

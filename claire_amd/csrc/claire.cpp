@@ -1133,6 +1133,20 @@ int CLAIRE::HessianMatVec(float* Hvtilde, const float* vtilde, bool scale) {
 }
 
 void CLAIRE::CaptureMatVec(float* Hvtilde, const VecField* vt, bool scale, hipGraphExec_t* exec) {
+    CaptureGraph([&] { HessianMatVecBody(Hvtilde, vt, scale); }, exec);
+}
+
+void CLAIRE::HessianMatVecInline(float* Hvtilde, const float* vtilde, bool scale) {
+    CLR_REQUIRE(vtilde != nullptr && Hvtilde != nullptr && Hvtilde != vtilde, kErrInvalidArgument,
+                "HessianMatVec needs distinct input and output vectors");
+    RequireVelocity();
+    if (!m_StateValid) SolveStateEquation();
+    EnsureMatVecCaches();
+    VecField vt(&m_Opt, const_cast<float*>(vtilde));
+    HessianMatVecBody(Hvtilde, &vt, scale);
+}
+
+void CLAIRE::CaptureGraph(const std::function<void()>& body, hipGraphExec_t* exec) {
     if (!m_CaptureStream) CLR_HIP_CHECK(hipStreamCreateWithFlags(&m_CaptureStream, hipStreamNonBlocking));
     const hipStream_t saved = m_Opt.stream;
     Profiler* const saved_prof = m_Opt.prof;
@@ -1141,7 +1155,7 @@ void CLAIRE::CaptureMatVec(float* Hvtilde, const VecField* vt, bool scale, hipGr
     hipGraph_t graph = nullptr;
     CLR_HIP_CHECK(hipStreamBeginCapture(m_CaptureStream, hipStreamCaptureModeThreadLocal));
     try {
-        HessianMatVecBody(Hvtilde, vt, scale);
+        body();
     } catch (...) {
         m_Opt.stream = saved;
         m_Opt.prof = saved_prof;

@@ -14,6 +14,8 @@
 // claire_amd::Error internally and are converted at the C-ABI boundary.
 #pragma once
 
+#include <functional>
+
 #include <map>
 #include <memory>
 #include <string>
@@ -167,6 +169,9 @@ class Reducer {
     static constexpr int kSlots = 16;
     void DotTo(const float* a, const float* b, IntType n, int slot);
     void Fetch(double* out, int nslots);
+    // Device address of the result slots (read by kernels that consume the
+    // products without a host round trip).
+    const double* Slots() const { return m_Result.get(); }
 
   private:
     double fetch();
@@ -364,6 +369,19 @@ class CLAIRE {
     }
     // Capture the Hessian matvec into HIP graphs and replay them (on/off).
     int SetMatVecGraphs(bool on);
+    bool MatVecGraphs() const { return m_Opt.graphs != 0; }
+    // What a captured graph of this problem depends on (trajectories and
+    // their displacement bound, the velocity-is-zero branch, precision,
+    // operator type): graphs of an older epoch must not be replayed.
+    uint64_t GraphEpoch() const { return m_GraphEpoch; }
+    // The Hessian matvec without graph replay, for callers that capture a
+    // larger body around it (the Newton-invariant caches must be current:
+    // PrepareHessian first).
+    void HessianMatVecInline(float* Hvtilde, const float* vtilde, bool scale);
+    // Captures body() on the problem's private capture stream (body launches
+    // on Opt().stream, which points there during the capture) and
+    // instantiates it into *exec.
+    void CaptureGraph(const std::function<void()>& body, hipGraphExec_t* exec);
     int MatVecGraphCount() const { return static_cast<int>(m_Graphs.size()); }
     // det(∇y) (inverse = false) or det(∇y⁻¹) by SL transport
     // (DeformationFields::ComputeDetDefGrad); jac may be null; stats receive

@@ -213,6 +213,32 @@ void launch_sub(const float* x, const float* y, float* z, int64_t n, hipStream_t
 // FP64 reductions.  `partial` must hold kReduceBlocks doubles and `result`
 // one double (device memory).
 constexpr int kReduceBlocks = 1024;
+
+// Device-resident scalars of a conjugate-gradient solve (reg::Optimizer::Pcg),
+// so that a run of iterations needs no host round trip: the step lengths are
+// formed on the device from the reduction slots, in double, exactly as the
+// host would form them, and every update is skipped once the solve has
+// stopped.  Slots: 0 = pᵀw, 1 = rᵀr, 2 = rᵀz.
+struct PcgState {
+    double rz;     // rᵀz of the current residual
+    double tol;    // stop when ‖r‖ <= tol
+    double bnorm;  // ‖b‖ (divergence test ‖r‖ >= 1e6·‖b‖)
+    double rnorm;  // ‖r‖ after the last iteration
+    float a, na, beta;  // float(α), float(−α), float(β) for the updates
+    int k;         // iterations taken
+    int kmax;      // iteration limit
+    int done;      // 0 while running; 1 once stopped
+};
+// s.rz = slots[2], s.k = 0, s.done = 0, tolerances as given
+void launch_pcg_init(PcgState* s, const double* slots, double tol, double bnorm, int kmax, hipStream_t st);
+// k += 1; α = rz/pᵀw, or stop (before the updates) when pᵀw is not positive
+void launch_pcg_alpha(PcgState* s, const double* slots, hipStream_t st);
+// ‖r‖ test and the limit, else β = rᵀz_new/rᵀz, rz = rᵀz_new
+void launch_pcg_beta(PcgState* s, const double* slots, hipStream_t st);
+// y += (sign > 0 ? α : −α)·x unless stopped
+void launch_axpy_pcg(const PcgState* s, int sign, const float* x, float* y, int64_t n, hipStream_t st);
+// p = β·p + z unless stopped
+void launch_xpay_pcg(const PcgState* s, float* p, const float* z, int64_t n, hipStream_t st);
 void launch_dot(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st);
 void launch_absmax(const float* a, int64_t n, double* partial, double* result, hipStream_t st);
 void launch_sqdist(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st);

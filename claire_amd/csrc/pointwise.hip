@@ -46,6 +46,59 @@ __global__ __launch_bounds__(kThreads) void k_sub(const float* __restrict__ x, c
     if (i < n) z[i] = x[i] - y[i];
 }
 
+// Conjugate-gradient scalars on the device (PcgState).  One thread; the
+// arithmetic is the host's: double division and square root, then the
+// rounding to float that the host's update launches apply.
+__global__ void k_pcg_init(PcgState* s, const double* __restrict__ slots, double tol, double bnorm, int kmax) {
+    s->rz = slots[2];
+    s->tol = tol;
+    s->bnorm = bnorm;
+    s->rnorm = bnorm;
+    s->a = s->na = s->beta = 0.0f;
+    s->k = 0;
+    s->kmax = kmax;
+    s->done = 0;
+}
+__global__ void k_pcg_alpha(PcgState* s, const double* __restrict__ slots) {
+    if (s->done) return;
+    s->k += 1;
+    const double pw = slots[0];
+    if (!(pw > 0.0)) {  // indefinite operator or breakdown: stop before the update
+        s->done = 1;
+        return;
+    }
+    const double a = s->rz / pw;
+    s->a = static_cast<float>(a);
+    s->na = static_cast<float>(-a);
+}
+__global__ void k_pcg_beta(PcgState* s, const double* __restrict__ slots) {
+    if (s->done) return;
+    const double rnorm = sqrt(slots[1]);
+    s->rnorm = rnorm;
+    if (rnorm <= s->tol || !(rnorm < 1e6 * s->bnorm) || s->k >= s->kmax) {
+        s->done = 1;
+        return;
+    }
+    const double rz_new = slots[2];
+    s->beta = static_cast<float>(rz_new / s->rz);
+    s->rz = rz_new;
+}
+__global__ __launch_bounds__(kThreads) void k_axpy_pcg(const PcgState* __restrict__ s, int sign,
+                                                       const float* __restrict__ x, float* __restrict__ y,
+                                                       int64_t n) {
+    if (s->done) return;
+    const float a = sign > 0 ? s->a : s->na;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + y[i];
+}
+__global__ __launch_bounds__(kThreads) void k_xpay_pcg(const PcgState* __restrict__ s, float* __restrict__ p,
+                                                       const float* __restrict__ z, int64_t n) {
+    if (s->done) return;
+    const float b = s->beta;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) p[i] = b * p[i] + z[i];
+}
+
 enum RedOp { kRedDot = 0, kRedAbsMax = 1, kRedSqDist = 2, kRedMin = 3, kRedMax = 4, kRedSum = 5 };
 
 template <int OP>
@@ -174,6 +227,21 @@ void launch_waxpy(float alpha, const float* x, const float* y, float* w, int64_t
 }
 void launch_sub(const float* x, const float* y, float* z, int64_t n, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_sub, dim3(blocks_for(n)), dim3(kThreads), 0, st, x, y, z, n);
+}
+void launch_pcg_init(PcgState* s, const double* slots, double tol, double bnorm, int kmax, hipStream_t st) {
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(1), 0, st, s, slots, tol, bnorm, kmax);
+}
+void launch_pcg_alpha(PcgState* s, const double* slots, hipStream_t st) {
+    hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(1), 0, st, s, slots);
+}
+void launch_pcg_beta(PcgState* s, const double* slots, hipStream_t st) {
+    hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(1), 0, st, s, slots);
+}
+void launch_axpy_pcg(const PcgState* s, int sign, const float* x, float* y, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_axpy_pcg, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, sign, x, y, n);
+}
+void launch_xpay_pcg(const PcgState* s, float* p, const float* z, int64_t n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_xpay_pcg, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, p, z, n);
 }
 void launch_dot(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st) {
     reduce<kRedDot>(a, b, n, partial, result, st);

@@ -20,6 +20,14 @@ Optimizer::Optimizer(CLAIRE* problem)
     m_P.resize(n);
     m_W.resize(n);
     m_Vt.resize(n);
+    m_State.resize(1);
+}
+
+Optimizer::~Optimizer() { DropIterationGraph(); }
+
+void Optimizer::DropIterationGraph() {
+    if (m_IterGraph) (void)hipGraphExecDestroy(m_IterGraph);
+    m_IterGraph = nullptr;
 }
 
 double Optimizer::Dot(const float* a, const float* b) { return m_Prob->Red()->Dot(a, b, m_N); }
@@ -58,15 +66,67 @@ void monitor_line(int iter, double J, double J0, double D, double D0, double g, 
 // (KSPSetTolerances, src/Optimizer.cpp:298-307); stops on a non-positive
 // curvature pᵀHp (indefinite operator).  Returns the iteration count; the
 // solution is in m_D.
-int Optimizer::Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& rep) {
+void Optimizer::Precond(const float* in, float* out, const GNOptions& opt) {
+    // the symmetrically preconditioned operator is solved without a further
+    // preconditioner (the reference pairs it with its two-level scheme)
+    if (opt.precond && m_Prob->Opt().matvectype != 2)
+        m_Prob->ApplyInvRegularizationOperator(out, in, false);
+    else
+        launch_copy(in, out, m_N, Stream());
+}
+
+// The body of one KSPCG iteration.  Step lengths live in m_State and are
+// formed on the device from FP64 reduction slots with the host's arithmetic,
+// so a run of iterations equals the host-driven loop bit for bit; once the
+// solve has stopped, the updates are skipped (the matvec and preconditioner
+// of a queued iteration still run, on vectors that no longer change the
+// result).  z = M⁻¹r is applied before the convergence test so that ‖r‖²
+// and rᵀz are reduced together.
+void Optimizer::PcgIteration(bool inline_matvec, const GNOptions& opt) {
     float* d = m_D.get();
     float* r = m_R.get();
     float* z = m_Z.get();
     float* p = m_P.get();
     float* w = m_W.get();
+    PcgState* s = m_State.get();
+    Reducer* red = m_Prob->Red();
+    const double* slots = red->Slots();
+    if (inline_matvec)
+        m_Prob->HessianMatVecInline(w, p, true);
+    else
+        m_Prob->HessianMatVec(w, p, true);
+    red->DotTo(p, w, m_N, 0);
+    launch_pcg_alpha(s, slots, Stream());
+    launch_axpy_pcg(s, +1, p, d, m_N, Stream());
+    launch_axpy_pcg(s, -1, w, r, m_N, Stream());
+    Precond(r, z, opt);
+    red->DotTo(r, r, m_N, 1);
+    red->DotTo(r, z, m_N, 2);
+    launch_pcg_beta(s, slots, Stream());
+    launch_xpay_pcg(s, p, z, m_N, Stream());
+}
+
+// KSPCG on H d = −g, zero initial guess, unpreconditioned residual
+// norm: converged when ‖r‖ <= max(rtol·‖b‖, 1e-16), diverged when
+// ‖r‖ >= 1e6·‖b‖, at most krylov_maxit − 1 iterations
+// (KSPSetTolerances, src/Optimizer.cpp:298-307); stops on a non-positive
+// curvature pᵀHp (indefinite operator).  Returns the iteration count; the
+// solution is in m_D.
+//
+// Iterations are queued in runs between host checks of the device state.
+// A run's length is the remainder predicted from the residual's decay over
+// the previous run (at most krylov_block), so few queued iterations outlive
+// the stopping test.  With matvec graphs on, one iteration is captured per
+// graph epoch and the runs replay it.
+int Optimizer::Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& rep) {
+    float* d = m_D.get();
+    float* r = m_R.get();
+    float* z = m_Z.get();
+    float* p = m_P.get();
     const int matvec = m_Prob->Opt().matvectype;
     CLR_REQUIRE(matvec != 1, kErrInvalidArgument,
                 "the preconditioned Hessian (PRECONDMATVEC) is not symmetric; this driver's Krylov method is CG");
+    CLR_REQUIRE(opt.krylov_block >= 1, kErrInvalidArgument, "krylov_block must be >= 1");
     const bool sym = matvec == 2;
     launch_fill(d, 0.0f, m_N, Stream());
     launch_copy(g, r, m_N, Stream());
@@ -81,47 +141,54 @@ int Optimizer::Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& 
     auto post = [&]() {
         if (sym) m_Prob->ApplyInvRegularizationOperator(d, d, true);
     };
-    if (!(bnorm > tol)) {
+    if (!(bnorm > tol) || kmax == 0) {
         post();
         return 0;
     }
-    // the symmetrically preconditioned operator is solved without a further
-    // preconditioner (the reference pairs it with its two-level scheme)
-    auto precond = [&](const float* in, float* out) {
-        if (opt.precond && !sym)
-            m_Prob->ApplyInvRegularizationOperator(out, in, false);
-        else
-            launch_copy(in, out, m_N, Stream());
-    };
-    precond(r, z);
-    double rz = Dot(r, z);
-    int its = 0;
-    for (int k = 1; k <= kmax; ++k) {
-        if (k == 1) launch_copy(z, p, m_N, Stream());
-        m_Prob->HessianMatVec(w, p, true);
-        rep.matvecs++;
-        its = k;
-        const double pw = Dot(p, w);
-        if (!(pw > 0.0)) break;
-        const double a = rz / pw;
-        launch_axpy(static_cast<float>(a), p, d, m_N, Stream());
-        launch_axpy(static_cast<float>(-a), w, r, m_N, Stream());
-        // z = M⁻¹r is applied before the convergence test so that ‖r‖² and
-        // rᵀz come back with one synchronization (on the last iteration z
-        // is not used)
-        precond(r, z);
-        m_Prob->Red()->DotTo(r, r, m_N, 0);
-        m_Prob->Red()->DotTo(r, z, m_N, 1);
-        double rr_rz[2];
-        m_Prob->Red()->Fetch(rr_rz, 2);
-        const double rnorm = std::sqrt(rr_rz[0]);
-        if (rnorm <= tol || !(rnorm < 1e6 * bnorm)) break;
-        const double rz_new = rr_rz[1];
-        const double beta = rz_new / rz;
-        rz = rz_new;
-        // p = z + β·p
-        launch_waxpy(static_cast<float>(beta), p, z, p, m_N, Stream());
+    Precond(r, z, opt);
+    m_Prob->Red()->DotTo(r, z, m_N, 2);
+    launch_pcg_init(m_State.get(), m_Prob->Red()->Slots(), tol, bnorm, kmax, Stream());
+    launch_copy(z, p, m_N, Stream());
+
+    const bool graphs = m_Prob->MatVecGraphs() && !(m_Prob->Opt().prof && m_Prob->Opt().prof->enabled());
+    const int precond_key = opt.precond ? 1 : 0;
+    if (m_IterGraph && (m_IterEpoch != m_Prob->GraphEpoch() || m_IterPrecond != precond_key)) DropIterationGraph();
+    int its = 0, run = 1, runs = 0;
+    double rprev = bnorm;
+    PcgState hs{};
+    for (;;) {
+        run = std::min(run, kmax - its);
+        if (graphs && runs > 0 && run >= 3 && !m_IterGraph) {
+            // the first run was eager (lazy allocations, gather tuning); the
+            // caches are current, so the matvec is captured inline.  A solve
+            // predicted to end within two more iterations pays no capture.
+            m_Prob->PrepareHessian();
+            m_Prob->CaptureGraph([&] { PcgIteration(true, opt); }, &m_IterGraph);
+            m_IterEpoch = m_Prob->GraphEpoch();
+            m_IterPrecond = precond_key;
+        }
+        for (int q = 0; q < run; ++q) {
+            if (graphs && m_IterGraph)
+                CLR_HIP_CHECK(hipGraphLaunch(m_IterGraph, Stream()));
+            else
+                PcgIteration(false, opt);
+        }
+        ++runs;
+        CLR_HIP_CHECK(hipMemcpyAsync(&hs, m_State.get(), sizeof(PcgState), hipMemcpyDeviceToHost, Stream()));
+        CLR_HIP_CHECK(hipStreamSynchronize(Stream()));
+        its = hs.k;
+        if (hs.done || its >= kmax) break;
+        // iterations still needed if the residual keeps its decay rate
+        int next = opt.krylov_block;
+        if (hs.rnorm > 0.0 && hs.rnorm < rprev && run > 0) {
+            const double rate = std::pow(hs.rnorm / rprev, 1.0 / run);
+            const double need = std::log(tol / hs.rnorm) / std::log(rate);
+            if (std::isfinite(need)) next = static_cast<int>(std::ceil(std::max(1.0, need)));
+        }
+        run = std::max(1, std::min(next, opt.krylov_block));
+        rprev = hs.rnorm;
     }
+    rep.matvecs += its;
     post();
     return its;
 }
@@ -197,7 +264,7 @@ int Optimizer::Run(float* v, const GNOptions& opt, GNReport& rep, double* hist, 
         double alpha = 1.0, Jt = 0.0, Dt = 0.0, Rt = 0.0;
         bool ok = false;
         for (;;) {
-            alpha = 1.0;
+            alpha = opt.linesearch ? opt.ls_stepinit : 1.0;
             const int trials = opt.linesearch ? 30 : 1;
             for (int k = 0; k < trials; ++k) {
                 launch_waxpy(static_cast<float>(alpha), d, v, vt, m_N, Stream());

@@ -26,6 +26,11 @@ struct GNOptions {
     int linesearch = 1;          // 0 unit step, 1 Armijo
     int monitor = 0;             // 1: print the iteration table (OptimizationMonitor)
     double ls_sigma = 1e-4;      // Armijo sufficient decrease (PETSc armijo sigma)
+    double ls_stepinit = 1.0;    // first trial step (PETSc -tao_ls_stepinit)
+    // CG iterations queued between two host checks of the device-resident
+    // Krylov scalars (at most; the run is shortened to the predicted
+    // remainder of the solve).  1 checks after every iteration.
+    int krylov_block = 8;
 };
 
 struct GNReport {
@@ -63,11 +68,24 @@ class Optimizer {
     // ‖g‖/‖g₀‖, PCG iterations and step length of the step taken from it.
     int Run(float* v, const GNOptions& opt, GNReport& rep, double* history, int history_len);
 
+    ~Optimizer();
+
   private:
     double Dot(const float* a, const float* b);
     int Pcg(const float* g, double rtol, const GNOptions& opt, GNReport& rep);
+    // One CG iteration (matvec, step length, updates, preconditioner, the
+    // convergence test and the next direction) on device-resident scalars.
+    void PcgIteration(bool inline_matvec, const GNOptions& opt);
+    void Precond(const float* in, float* out, const GNOptions& opt);
+    void DropIterationGraph();
     CLAIRE* m_Prob;
     IntType m_N;
+    DeviceArray<claire_amd::PcgState> m_State;
+    // A captured CG iteration, replayed while the problem's graph epoch and
+    // the operator options stay the same.
+    hipGraphExec_t m_IterGraph = nullptr;
+    uint64_t m_IterEpoch = 0;
+    int m_IterPrecond = -1;
     // the problem's stream at the time of the call (a batch driver may move
     // a problem to another stream between solves)
     hipStream_t Stream() const { return m_Prob->Opt().stream; }

@@ -279,6 +279,17 @@ typedef struct clr_gn_options {
                             search, J(v + αd) <= J + σ·α·gᵀd (PETSc's
                             TAOLINESEARCHARMIJO, selected at src/Optimizer.cpp:390,
                             option -tao_ls_armijo_sigma, default 1e-4); in (0, 1) */
+    double ls_stepinit;  /* first trial step of every line search (PETSc's
+                            TaoLineSearch initial step, option
+                            -tao_ls_stepinit, default 1); halved on each
+                            rejected trial; > 0 */
+    int krylov_block;    /* most CG iterations queued between two host checks
+                            of the Krylov scalars, which stay on the device
+                            (8); the queue is cut to the remainder predicted
+                            from the residual's decay, and iterations queued
+                            past the stopping test change nothing.  1 checks
+                            after every iteration.  No counterpart in the
+                            reference (PETSc's KSPCG checks on the host). */
 } clr_gn_options;
 
 #define CLR_GN_GTTOL 1      /* ‖g‖ < gttol·‖g₀‖ (TAO_CONVERGED_GTTOL) */

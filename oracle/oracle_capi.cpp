@@ -117,9 +117,11 @@ void oracle_set_num_threads(int n) {
             std::vector<T> Xs(3 * g.nl), Xa(3 * g.nl);                                                              \
             oracle::trajectory(g, v, ht, +1, Xs.data());                                                            \
             oracle::trajectory(g, v, ht, -1, Xa.data());                                                            \
-            const auto t0 = std::chrono::steady_clock::now();                                                       \
-            for (int r = 0; r < reps; ++r) oracle::hessian_matvec_x(g, nt, norm, b, v, Xs.data(), Xa.data(), m, w, h, true); \
-            *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();               \
+            for (int r = 0; r < reps; ++r) {                                                                        \
+                const auto t0 = std::chrono::steady_clock::now();                                                   \
+                oracle::hessian_matvec_x(g, nt, norm, b, v, Xs.data(), Xa.data(), m, w, h, true);                   \
+                seconds[r] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();         \
+            }                                                                                                       \
         });                                                                                                        \
     }                                                                                                              \
                                                                                       \

@@ -351,14 +351,16 @@ def objective(v, mT, mR, nt, regnorm, beta, prec="f64"):
 
 
 def hessian_bench(v, m, vt, nt, regnorm, beta, reps=1, prec="f32"):
-    """Time `reps` Hessian matvecs of the CPU reference path; returns (H, seconds)."""
+    """Time `reps` Hessian matvecs of the CPU reference path (trajectories
+    computed before the timed region); returns (H, list of per-matvec
+    seconds)."""
     v, m, vt = _c(v, prec), _c(m, prec), _c(vt, prec)
     h = np.empty_like(vt)
-    sec = ctypes.c_double()
+    sec = (ctypes.c_double * max(1, int(reps)))()
     _call("oracle_hessian_bench", prec, _nx(v.shape[1:]), ctypes.c_int(nt), ctypes.c_int(REG[regnorm]), _beta(beta),
           _p(v[0]), _p(v[1]), _p(v[2]), _p(m), _p(vt[0]), _p(vt[1]), _p(vt[2]), _p(h[0]), _p(h[1]), _p(h[2]),
-          ctypes.c_int(reps), ctypes.byref(sec))
-    return h, sec.value
+          ctypes.c_int(reps), sec)
+    return h, [float(s) for s in sec[: max(0, int(reps))]]
 
 
 GN_REASONS = {0: "continue", 1: "gttol", 2: "gatol", 3: "steptol", -1: "maxits", -2: "nan", -3: "linesearch"}

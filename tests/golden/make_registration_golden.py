@@ -6,7 +6,8 @@ of rerunning the oracle.  Inputs are regenerated from the seeds recorded with
 each case (claire_amd.synth, numpy's PCG64 generator) and the reference image
 is the phantom transported by the FP64 oracle's state solve.
 
-Usage: python tests/golden/make_registration_golden.py [--threads N]
+Usage: python tests/golden/make_registration_golden.py [--threads N] [--cases NAME ...]
+(--cases recomputes only the named cases and keeps the others' stored results)
 """
 import argparse
 import json
@@ -31,6 +32,9 @@ CASES = [
      "velocity_seed": 4, "count": 30, "maxit": 50, "gttol": 5e-2},
     {"name": "phantom64_h2", "n": 64, "regnorm": "h2", "beta": 1e-3, "nt": 4, "phantom_seed": 3,
      "velocity_seed": 4, "count": 30, "maxit": 50, "gttol": 5e-2},
+    # the full-registration configuration as written: 128³ phantom, RegularizationH2 β = 1e-3, nt = 4
+    {"name": "phantom128_h2", "n": 128, "regnorm": "h2", "beta": 1e-3, "nt": 4, "phantom_seed": 3,
+     "velocity_seed": 4, "count": 30, "maxit": 50, "gttol": 5e-2},
 ]
 
 
@@ -49,11 +53,18 @@ def mismatch(m1, mT, mR):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--cases", nargs="*", default=None)
     args = ap.parse_args()
     if args.threads:
         oracle.set_num_threads(args.threads)
+    path = os.path.join(HERE, "registration_phantom.json")
     out = {}
+    if args.cases is not None and os.path.exists(path):
+        with open(path) as f:
+            out = json.load(f)
     for case in CASES:
+        if args.cases is not None and case["name"] not in args.cases:
+            continue
         t0 = time.time()
         mT, mR = inputs(case)
         shape = mT.shape
@@ -66,8 +77,8 @@ def main():
         out[case["name"]] = {"case": case, "oracle_fp64": res}
         print(f"{case['name']}: {rep['iterations']} its ({rep['reason']}), {rep['matvecs']} matvecs, "
               f"mismatch {res['mismatch']:.6e}, {time.time() - t0:.0f}s", flush=True)
-    with open(os.path.join(HERE, "registration_phantom.json"), "w") as f:
-        json.dump(out, f, indent=1)
+    with open(path, "w") as f:
+        json.dump({k: out[k] for k in sorted(out)}, f, indent=1)
 
 
 if __name__ == "__main__":
