@@ -50,8 +50,9 @@ def parse():
     p.add_argument("--batch-workers", type=int, default=8)
     p.add_argument("--no-variants", action="store_true",
                    help="skip the option variants (matvec types, models, RK2/RK4, continuity) at the same size")
-    p.add_argument("--large", action="store_true",
-                   help="add the 512^3 leg (40-ellipsoid phantom, nt=8): footprint, matvec/s, 5 GN iterations")
+    p.add_argument("--no-large", action="store_true",
+                   help="skip the 512^3 leg (40-ellipsoid phantom, nt=8): footprint, matvec/s, 5 GN iterations, "
+                        "error vs FP64 on the 128^3 subsample")
     p.add_argument("--no-fp64", action="store_true",
                    help="skip scoring the GPU (plain and mixed) and CPU FP32 matvecs against the FP64 oracle")
     p.add_argument("--calib-copy", action="store_true",
@@ -241,10 +242,60 @@ def batched_leg(args):
                 "gpu_grad_reduction": repg["gnorm"] / repg["gnorm0"],
                 "velocity_relerr_gpu_vs_cpu": float(np.linalg.norm(vg - v32) / np.linalg.norm(v32))}
             log(f"batched: CPU FP32 port on pair 0 (64^3): {dt:.2f}s")
+            res["pairs_vs_fp64"] = golden_pairs_check(nt, beta)
     finally:
         for c in problems:
             c.close()
     return res
+
+
+def golden_pairs_check(nt, beta):
+    """Per-pair error of the batch's registrations against the stored FP64
+    end states (tests/golden/batch_pairs.json: pairs 0, 32 and 40, i.e. one
+    of each size) and the spread of the reference FP32 path on the same
+    pair.  The golden's reference image is the FP64 state solve rounded to
+    FP32 (recomputed here by the oracle, as the checker): these end states
+    move by ~1% with the rounding of the reference image, so the GPU solve
+    is rerun on exactly that image."""
+    from claire_amd import Claire, synth
+    from oracle import oracle_py
+
+    path = os.path.join(ROOT, "tests", "golden", "batch_pairs.json")
+    if not os.path.exists(path):
+        return None
+    with open(path, encoding="utf-8") as f:
+        pairs = json.load(f)["pairs"]
+    out = []
+    for key in sorted(pairs, key=int):
+        g = pairs[key]
+        i, n = int(key), g["n"]
+        shape = (n, n, n)
+        mT = synth.ellipsoid_phantom(shape, count=30, seed=1000 + i)
+        v = synth.bandlimited_velocity(shape, kmax=8, cfl=2.0, nt=nt, seed=2000 + i)
+        mR = oracle_py.solve_state(v.astype(np.float64), mT.astype(np.float64), nt)[nt].astype(np.float32)
+        c = Claire(shape, nt=nt, regnorm="h1", beta=beta, graphs=True)
+        try:
+            c.set_template_image(mT)
+            c.set_reference_image(mR)
+            vs, rep, _ = c.solve_gn()
+            c.set_velocity(vs)
+            m1 = c.solve_state()[nt]
+        finally:
+            c.close()
+        mis = float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT))
+        red = rep["gnorm"] / rep["gnorm0"]
+        ref = g["oracle_fp64"]
+        out.append({"pair": i, "grid": n, "gpu_iterations": rep["iterations"], "fp64_iterations": ref["iterations"],
+                    "gpu_mismatch": mis, "fp64_mismatch": ref["mismatch"],
+                    "mismatch_relerr": abs(mis - ref["mismatch"]) / ref["mismatch"],
+                    "gpu_grad_reduction": red, "fp64_grad_reduction": ref["grad_reduction"],
+                    "grad_reduction_relerr": abs(red - ref["grad_reduction"]) / ref["grad_reduction"],
+                    "fp32_path_mismatch_relerr_max": max(abs(s["mismatch"] - ref["mismatch"]) / ref["mismatch"]
+                                                         for s in g["fp32_spread"]),
+                    "fp32_path_grad_reduction_relerr_max": max(
+                        abs(s["grad_reduction"] - ref["grad_reduction"]) / ref["grad_reduction"]
+                        for s in g["fp32_spread"])})
+    return out
 
 
 def variants_leg(c, v_slow, vt, hv, args):
@@ -375,6 +426,49 @@ def large_leg(args):
         res["device_bytes_problem"] = c.device_bytes()
         vt.free()
         hv.free()
+    finally:
+        c.close()
+    if not args.no_cpu_baseline:
+        # error against FP64 on the 128^3 subsample of the same problem (every
+        # fourth point of the template and the velocity, a trigonometric
+        # polynomial of degree 16 < 64), next to the reference FP32 path's
+        from oracle import oracle_py
+
+        oracle_py.set_num_threads(cpu_threads())
+        ms, vs = np.ascontiguousarray(mT[::4, ::4, ::4]), np.ascontiguousarray(v[:, ::4, ::4, ::4])
+        cs = Claire(ms.shape, nt=nt, regnorm="h2", beta=(1e-3, 0.0, 0.0))
+        try:
+            cs.set_velocity(vs)
+            cs.set_template_image(ms)
+            Hs = cs.hessian_matvec(vs).astype(np.float64)
+            cs.set_precision("mixed")
+            Hm = cs.hessian_matvec(vs).astype(np.float64)
+        finally:
+            cs.close()
+        b = (1e-3, 0.0, 0.0)
+        H64 = oracle_py.hessian_matvec(vs, oracle_py.solve_state(vs, ms, nt), vs, nt, "h2", b)
+        H32 = oracle_py.hessian_matvec(vs, oracle_py.solve_state(vs, ms, nt, prec="f32"), vs, nt, "h2", b, prec="f32")
+
+        def rel(a, r):
+            return float(np.linalg.norm(a - r) / np.linalg.norm(r))
+
+        res["relerr_vs_fp64_subsample_128"] = {"gpu_fp32": rel(Hs, H64), "gpu_mixed": rel(Hm, H64),
+                                               "cpu_fp32": rel(H32.astype(np.float64), H64)}
+        log(f"large: 128^3 subsample relerr vs FP64 gpu {rel(Hs, H64):.3e} mixed {rel(Hm, H64):.3e} "
+            f"cpu {rel(H32.astype(np.float64), H64):.3e}")
+    c = Claire(shape, nt=nt, regnorm="h2", beta=(1e-3, 0.0, 0.0))
+    try:
+        # RegularizationH2 as written: its line search stalls after one step
+        # (its gradient omits the hd of its functional), as in the FP64 oracle
+        c.set_template_image(mT)
+        c.set_reference_image(mR)
+        t0 = time.perf_counter()
+        _, rep, _ = c.solve_gn(maxit=6)
+        c.sync()
+        res["gn_h2"] = {"regnorm": "h2", "beta": 1e-3, "seconds": round(time.perf_counter() - t0, 2),
+                        "iterations": rep["iterations"], "matvecs": rep["matvecs"], "reason": rep["reason"],
+                        "grad_reduction": rep["gnorm"] / rep["gnorm0"]}
+        log(f"large: GN with H2 {rep['iterations']} its ({rep['reason']})")
     finally:
         c.close()
     c = Claire(shape, nt=nt, regnorm="h2sn", beta=(1e-3, 0.0, 0.0))
@@ -604,7 +698,7 @@ def main():
         out["registration"] = guarded(registration_leg, args)
     if not args.no_batched:
         out["batched"] = guarded(batched_leg, args)
-    if args.large:
+    if not args.no_large:
         out["large"] = guarded(large_leg, args)
     if args.calib_copy:
         big = DeviceArray((64 * 1024 * 1024,))
