@@ -27,6 +27,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "gather_device.hpp"
@@ -389,34 +390,55 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
         for (int q = 0; q < G::FILL; ++q)
             if (tid + kLThreads * q < G::NV) store2(dst, q, bload(rf, foff[q][0], src), bload(rf, foff[q][1], src));
     }
-    PlanPt qa, qb;
-    float aa, ab;
-    {
-        const uint32_t pb = static_cast<uint32_t>(x1s) * plane4;
-        qa = bload_plan(rp, 3u * off, 3u * pb);
-        aa = Epi::kHasAux ? bload_once(ra, off, pb) : 0.0f;
-    }
+    // Loads run two planes ahead of their use: the plan words (and the
+    // epilogue operand) of plane x1 + 2 and the input plane x1 + HHI + 2
+    // are issued while plane x1 is evaluated, so each load has two steps of
+    // evaluation to arrive.  Q[k & 3] / A[k & 3] hold the plan words and
+    // operand of step k; PRE[k & 1] the input plane that step k stores into
+    // the ring's spare slot.
+    PlanPt Q[4];
+    float A[4];
+    float PRE[2][G::FILL][2];
+    auto load_point = [&](int x, PlanPt& q, float& a) {
+        const int xc = min(x, x1e - 1);  // past the chunk end: a plane of this chunk, never used
+        const uint32_t pb = static_cast<uint32_t>(xc) * plane4;
+        q = bload_plan(rp, 3u * off, 3u * pb);
+        a = Epi::kHasAux ? bload_once(ra, off, pb) : 0.0f;
+    };
+    auto load_plane = [&](int x, float (&pre)[G::FILL][2]) {
+        const uint32_t src = static_cast<uint32_t>(pmod(x, g.n1)) * plane4;
+#pragma unroll
+        for (int q = 0; q < G::FILL; ++q) {
+            pre[q][0] = bload(rf, foff[q][0], src);
+            pre[q][1] = bload(rf, foff[q][1], src);
+        }
+    };
+    load_point(x1s, Q[0], A[0]);
+    load_point(x1s + 1, Q[1], A[1]);
+    load_plane(x1s + G::HHI + 1, PRE[0]);
     __syncthreads();
 
     constexpr int P4 = 4 * G::P;
     const char* ldsb = reinterpret_cast<const char*>(lds);
     int cs = 0;
-    auto step = [&](int x1, const PlanPt& cur, float caux, PlanPt& nxt, float& naux) {
-        const int xn = (x1 + 1 < x1e) ? x1 + 1 : x1;
-        float pre[G::FILL][2];
-        {
-            const uint32_t src = static_cast<uint32_t>(pmod(x1 + G::HHI + 1, g.n1)) * plane4;
-#pragma unroll
-            for (int q = 0; q < G::FILL; ++q) {
-                pre[q][0] = bload(rf, foff[q][0], src);
-                pre[q][1] = bload(rf, foff[q][1], src);
-            }
-        }
-        {
-            const uint32_t pb = static_cast<uint32_t>(xn) * plane4;
-            nxt = bload_plan(rp, 3u * off, 3u * pb);
-            naux = Epi::kHasAux ? bload_once(ra, off, pb) : 0.0f;
-        }
+    // Stencil rows kept in registers across output planes.  A point's
+    // stencil spans input planes x1 + o1 … x1 + o1 + 3; when the next plane's
+    // point has the same offsets (o1, o2, o3) -- about 95% of points on a
+    // CFL-2 trajectory -- its planes 0..2 are this point's planes 1..3, so
+    // only its plane 3 is read from LDS.  S[k & 3] holds the rows of the
+    // plane that step k reads as its plane 3; step k's plane a lives in
+    // S[(k + a) & 3], which is the set step k + 1 uses for plane a − 1.
+    // Lanes whose offsets changed (and every lane on a chunk's first plane)
+    // read all four planes.  The values and their order of summation are
+    // those of the LDS reads, so results are unchanged bit for bit.
+    float2 S[4][4][2];
+    uint32_t prev_offs = 0xFFFFFFFFu;
+    auto step = [&](auto kconst, int x1, bool first) {
+        constexpr int K = decltype(kconst)::value;
+        const PlanPt& cur = Q[K & 3];
+        const float caux = A[K & 3];
+        load_point(x1 + 2, Q[(K + 2) & 3], A[(K + 2) & 3]);
+        load_plane(x1 + G::HHI + 2, PRE[(K + 1) & 1]);
         float w1[4], w2[4], w3[4];
         lagrange_weights(plan_t(cur.w[0]), w1);
         lagrange_weights(plan_t(cur.w[1]), w2);
@@ -431,15 +453,32 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
         base[0] = opaque(sw * P4 + lb);
 #pragma unroll
         for (int a = 1; a < 4; ++a) base[a] = opaque(base[a - 1] + ((sw + a == G::W) ? P4 - G::W * P4 : P4));
+        // the three 9-bit offsets of the point
+        const uint32_t offs = (cur.w[0] >> 23) | ((cur.w[1] >> 23) << 9) | ((cur.w[2] >> 23) << 18);
+        const bool reuse = !first && offs == prev_offs;
+        prev_offs = offs;
+        auto read_plane = [&](int a, float2 (&R)[4][2]) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const char* row = ldsb + base[a] + b * 4 * G::PITCH;
+                R[b][0] = *reinterpret_cast<const float2*>(row);
+                R[b][1] = *reinterpret_cast<const float2*>(row + 16);
+            }
+        };
+        if (!reuse) {
+            read_plane(0, S[K & 3]);
+            read_plane(1, S[(K + 1) & 3]);
+            read_plane(2, S[(K + 2) & 3]);
+        }
+        read_plane(3, S[(K + 3) & 3]);
         float v = 0.f;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             float sa = 0.f;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const char* row = ldsb + base[a] + b * 4 * G::PITCH;
-                const float2 lo = *reinterpret_cast<const float2*>(row);
-                const float2 hi = *reinterpret_cast<const float2*>(row + 16);
+                const float2 lo = S[(K + a) & 3][b][0];
+                const float2 hi = S[(K + a) & 3][b][1];
                 const float r = w3[0] * lo.x + w3[1] * lo.y + w3[2] * hi.x + w3[3] * hi.y;
                 sa += w2[b] * r;
             }
@@ -453,21 +492,34 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
                 bstore(ro, epi.apply(v, caux), off, pb);
             if constexpr (Epi::kOut2) bstore(ro2, epi.apply2(v, caux), off, pb);
         }
+        // land input plane x1 + HHI + 1 (loaded one step earlier) in the spare slot
         const int spare = (cs == 0) ? G::W - 1 : cs - 1;
         float* dst = lds + spare * G::P;
 #pragma unroll
         for (int q = 0; q < G::FILL; ++q)
-            if (tid + kLThreads * q < G::NV) store2(dst, q, pre[q][0], pre[q][1]);
+            if (tid + kLThreads * q < G::NV) store2(dst, q, PRE[K & 1][q][0], PRE[K & 1][q][1]);
         cs = (cs + 1 == G::W) ? 0 : cs + 1;
         __syncthreads();
     };
+    using K0 = std::integral_constant<int, 0>;
+    using K1 = std::integral_constant<int, 1>;
+    using K2 = std::integral_constant<int, 2>;
+    using K3 = std::integral_constant<int, 3>;
 
+    // four planes per trip (the register sets rotate once)
     int x1 = x1s;
-    for (; x1 + 1 < x1e; x1 += 2) {
-        step(x1, qa, aa, qb, ab);
-        step(x1 + 1, qb, ab, qa, aa);
+    bool first = true;
+    for (; x1 + 3 < x1e; x1 += 4) {
+        step(K0{}, x1, first);
+        step(K1{}, x1 + 1, false);
+        step(K2{}, x1 + 2, false);
+        step(K3{}, x1 + 3, false);
+        first = false;
     }
-    if (x1 < x1e) step(x1, qa, aa, qb, ab);
+    // the last 0-3 planes continue the rotation
+    if (x1 < x1e) step(K0{}, x1, first);
+    if (x1 + 1 < x1e) step(K1{}, x1 + 1, false);
+    if (x1 + 2 < x1e) step(K2{}, x1 + 2, false);
 }
 
 // Incremental state right-hand sides, streamed with 16-byte accesses:
