@@ -432,7 +432,7 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
     // read all four planes.  The values and their order of summation are
     // those of the LDS reads, so results are unchanged bit for bit.
     float2 S[4][4][2];
-    uint32_t prev_offs = 0xFFFFFFFFu;
+    int prev_o1 = 0, prev_lb = -1;
     auto step = [&](auto kconst, int x1, bool first) {
         constexpr int K = decltype(kconst)::value;
         const PlanPt& cur = Q[K & 3];
@@ -453,10 +453,12 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
         base[0] = opaque(sw * P4 + lb);
 #pragma unroll
         for (int a = 1; a < 4; ++a) base[a] = opaque(base[a - 1] + ((sw + a == G::W) ? P4 - G::W * P4 : P4));
-        // the three 9-bit offsets of the point
-        const uint32_t offs = (cur.w[0] >> 23) | ((cur.w[1] >> 23) << 9) | ((cur.w[2] >> 23) << 18);
-        const bool reuse = !first && offs == prev_offs;
-        prev_offs = offs;
+        // same offsets as the previous plane's point: the same x1 offset and
+        // the same in-plane stencil address (which fixes the x2 and x3 offsets)
+        const int o1 = plan_off(cur.w[0]);
+        const bool reuse = !first && o1 == prev_o1 && lb == prev_lb;
+        prev_o1 = o1;
+        prev_lb = lb;
         auto read_plane = [&](int a, float2 (&R)[4][2]) {
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
