@@ -51,7 +51,7 @@ def parse():
     p.add_argument("--no-variants", action="store_true",
                    help="skip the option variants (matvec types, models, RK2/RK4, continuity) at the same size")
     p.add_argument("--no-large", action="store_true",
-                   help="skip the 512^3 leg (40-ellipsoid phantom, nt=8): footprint, matvec/s, 5 GN iterations, "
+                   help="skip the 512^3 leg (40-ellipsoid phantom, nt=8): footprint, matvec/s, 6 GN iterations, "
                         "error vs FP64 on the 128^3 subsample")
     p.add_argument("--no-fp64", action="store_true",
                    help="skip scoring the GPU (plain and mixed) and CPU FP32 matvecs against the FP64 oracle")
