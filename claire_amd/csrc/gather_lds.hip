@@ -307,8 +307,16 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
 // values (f[c], f[c+1]) of the tile row, so the four values of a stencil row
 // starting at column c are the 8-byte entries c and c + 2: one ds_read2_b64
 // per stencil row instead of two ds_read2_b32, with no alignment condition
-// on c.  The price is twice the LDS per plane: a 16 × 32 tile with its halo
-// takes 79.5 KB, two blocks (16 waves) per CU.
+// on c.  The price is twice the LDS per plane.  A block keeps a ring of
+// W = WIN + 2·PERIOD − 1 planes and synchronizes once per PERIOD = 2 output
+// planes (the planes of two consecutive stencil windows stay resident while
+// the next two are landed in slots neither window reads): 11 planes of a
+// 16 × 32 tile with its halo at a row pitch of 80 dwords take 79.1 KB, two
+// blocks (16 waves) per CU.  The pitch of 96 dwords that a one-plane period
+// affords has a third of the bank conflicts (8 k against 22 k cycles per CU
+// and launch at 256³), but the halved barriers gain more (adjoint step
+// 107.2 → 105.1 µs; at one plane per barrier the pitch of 80 alone costs
+// 6 µs).
 struct PairGeo {
     static constexpr int D = 2;
     static constexpr int LT3 = 32;
@@ -316,15 +324,20 @@ struct PairGeo {
     static constexpr int HLO = D + 1, HHI = D + 2;
     static constexpr int E2 = kLT2 + HLO + HHI, E3 = LT3 + HLO + HHI;
     static constexpr int NE3 = E3 - 1;                   // entries per row (the last column only fills entry E3 − 2)
-    static constexpr int PITCH = (2 * NE3 + 31) / 32 * 32;  // dwords per row
+    static constexpr int PITCH = (2 * NE3 + 15) / 16 * 16;  // dwords per row
     static constexpr int P = E2 * PITCH;                 // dwords per LDS plane
     static constexpr int NV = E2 * NE3;                  // entries per plane
-    static constexpr int WIN = HLO + HHI + 1;
-    static constexpr int W = WIN + 1;
+    static constexpr int WIN = HLO + HHI + 1;            // planes of one output plane's stencils
+    static constexpr int PERIOD = 2;                     // output planes per barrier
+    static constexpr int W = WIN + 2 * PERIOD - 1;       // ring slots
     static constexpr int FILL = (NV + kLThreads - 1) / kLThreads;
     // every ds_read2_b64 of a stencil (rows 0-3, entries c and c + 2) is
     // within the instruction's 8-bit offsets (in 8-byte units) of the base
     static_assert((3 * 4 * PITCH + 16) / 8 <= 255, "row 3 out of ds_read2_b64 offset range");
+    // the kernel's steps are unrolled by four and barriers close periods
+    static_assert(4 % PERIOD == 0, "the period must divide the unroll of four");
+    // two blocks (16 waves) per CU in the 160 KB of LDS
+    static_assert(2 * W * P * 4 <= 160 * 1024, "ring too large for two blocks per CU");
 };
 
 template <class Epi>
@@ -383,7 +396,9 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
     auto store2 = [&](float* dst, int q, float a, float b) {
         *reinterpret_cast<float2*>(dst + loff[q]) = make_float2(a, b);
     };
-    for (int k = 0; k < G::WIN; ++k) {
+    // prologue: input planes x1s − HLO … x1s + HHI + PERIOD − 1 (the stencils
+    // of the first PERIOD output planes) into ring slots 0 … WIN + PERIOD − 2
+    for (int k = 0; k < G::WIN + G::PERIOD - 1; ++k) {
         const uint32_t src = static_cast<uint32_t>(pmod(x1s - G::HLO + k, g.n1)) * plane4;
         float* dst = lds + k * G::P;
 #pragma unroll
@@ -391,11 +406,11 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
             if (tid + kLThreads * q < G::NV) store2(dst, q, bload(rf, foff[q][0], src), bload(rf, foff[q][1], src));
     }
     // Loads run two planes ahead of their use: the plan words (and the
-    // epilogue operand) of plane x1 + 2 and the input plane x1 + HHI + 2
-    // are issued while plane x1 is evaluated, so each load has two steps of
-    // evaluation to arrive.  Q[k & 3] / A[k & 3] hold the plan words and
-    // operand of step k; PRE[k & 1] the input plane that step k stores into
-    // the ring's spare slot.
+    // epilogue operand) of plane x1 + 2 and the input plane
+    // x1 + HHI + PERIOD + 1 are issued while plane x1 is evaluated, so each
+    // load has two steps of evaluation to arrive.  Q[k & 3] / A[k & 3] hold
+    // the plan words and operand of step k; PRE[k & 1] the input plane that
+    // step k stores into a free ring slot.
     PlanPt Q[4];
     float A[4];
     float PRE[2][G::FILL][2];
@@ -415,7 +430,7 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
     };
     load_point(x1s, Q[0], A[0]);
     load_point(x1s + 1, Q[1], A[1]);
-    load_plane(x1s + G::HHI + 1, PRE[0]);
+    load_plane(x1s + G::HHI + G::PERIOD, PRE[0]);
     __syncthreads();
 
     constexpr int P4 = 4 * G::P;
@@ -438,7 +453,7 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
         const PlanPt& cur = Q[K & 3];
         const float caux = A[K & 3];
         load_point(x1 + 2, Q[(K + 2) & 3], A[(K + 2) & 3]);
-        load_plane(x1 + G::HHI + 2, PRE[(K + 1) & 1]);
+        load_plane(x1 + G::HHI + G::PERIOD + 1, PRE[(K + 1) & 1]);
         float w1[4], w2[4], w3[4];
         lagrange_weights(plan_t(cur.w[0]), w1);
         lagrange_weights(plan_t(cur.w[1]), w2);
@@ -494,14 +509,17 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
                 bstore(ro, epi.apply(v, caux), off, pb);
             if constexpr (Epi::kOut2) bstore(ro2, epi.apply2(v, caux), off, pb);
         }
-        // land input plane x1 + HHI + 1 (loaded one step earlier) in the spare slot
-        const int spare = (cs == 0) ? G::W - 1 : cs - 1;
+        // land input plane x1 + HHI + PERIOD (loaded one step earlier) in
+        // the slot of plane x1 + HHI + PERIOD − W, which no step between
+        // this period's barriers reads
+        int spare = cs + G::WIN + G::PERIOD - 1;
+        spare -= (spare >= G::W) ? G::W : 0;
         float* dst = lds + spare * G::P;
 #pragma unroll
         for (int q = 0; q < G::FILL; ++q)
             if (tid + kLThreads * q < G::NV) store2(dst, q, PRE[K & 1][q][0], PRE[K & 1][q][1]);
         cs = (cs + 1 == G::W) ? 0 : cs + 1;
-        __syncthreads();
+        if constexpr (K % G::PERIOD == G::PERIOD - 1) __syncthreads();
     };
     using K0 = std::integral_constant<int, 0>;
     using K1 = std::integral_constant<int, 1>;
