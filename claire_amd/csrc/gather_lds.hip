@@ -851,8 +851,9 @@ void dispatch(int dbound, const GatherDims& g, const uint32_t* P, const float* f
     CLR_REQUIRE(g.nl < (int64_t(1) << 28), kErrInvalidArgument, "grid too large for 32-bit buffer byte offsets");
     CLR_REQUIRE(dbound >= 0 && dbound <= 4, kErrInvalidArgument, "displacement bound too large for the LDS gather");
     // The pair layout is faster for one field per launch (the solves' steps:
-    // 110 µs against 115 at 256³) and slower for three (313 µs against 303:
-    // three times the blocks at two per CU instead of three).
+    // 110 µs against 115 at 256³).  For three fields it gains 2% on plain
+    // gathers and loses 6-12% on the trajectory stages (two blocks per CU
+    // instead of three), so they keep the one-value layout.
     if (dbound <= 2 && nz == 1 && pair_layout())
         launch_lds<2, true>(g, P, f, fzstride, nz, epi, st);
     else if (dbound <= 2)
