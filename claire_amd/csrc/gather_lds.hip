@@ -208,6 +208,9 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
     constexpr int P4 = 4 * G::P;  // bytes per ring slot
     const char* ldsb = reinterpret_cast<const char*>(lds);
     int cs = 0;  // ring slot of plane x1 − HLO
+    // periodic index of plane x1 + HHI + 1, advanced by one per step (a
+    // scalar modulo per step costs about 20 scalar instructions)
+    int xf = pmod(x1s + G::HHI + 1, g.n1);
     auto step = [&](int x1, const PlanPt (&cur)[G::PTS], const float (&caux)[G::PTS], PlanPt (&nxt)[G::PTS],
                     float (&naux)[G::PTS]) {
         // Prefetch for the next step, unconditionally (past the chunk end it
@@ -215,7 +218,8 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
         const int xn = (x1 + 1 < x1e) ? x1 + 1 : x1;
         float pre[G::FILL];
         {
-            const uint32_t src = static_cast<uint32_t>(pmod(x1 + G::HHI + 1, g.n1)) * plane4;
+            const uint32_t src = static_cast<uint32_t>(xf) * plane4;
+            xf = (xf + 1 == g.n1) ? 0 : xf + 1;
 #pragma unroll
             for (int q = 0; q < G::FILL; ++q) pre[q] = bload(rf, foff[q], src);
         }
@@ -420,8 +424,8 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
         q = bload_plan(rp, 3u * off, 3u * pb);
         a = Epi::kHasAux ? bload_once(ra, off, pb) : 0.0f;
     };
-    auto load_plane = [&](int x, float (&pre)[G::FILL][2]) {
-        const uint32_t src = static_cast<uint32_t>(pmod(x, g.n1)) * plane4;
+    auto load_plane = [&](int xw, float (&pre)[G::FILL][2]) {  // xw: plane index in [0, n1)
+        const uint32_t src = static_cast<uint32_t>(xw) * plane4;
 #pragma unroll
         for (int q = 0; q < G::FILL; ++q) {
             pre[q][0] = bload(rf, foff[q][0], src);
@@ -430,7 +434,10 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
     };
     load_point(x1s, Q[0], A[0]);
     load_point(x1s + 1, Q[1], A[1]);
-    load_plane(x1s + G::HHI + G::PERIOD, PRE[0]);
+    load_plane(pmod(x1s + G::HHI + G::PERIOD, g.n1), PRE[0]);
+    // periodic index of the input plane the next step loads, advanced by one
+    // per step (a scalar modulo per step costs about 20 scalar instructions)
+    int xf = pmod(x1s + G::HHI + G::PERIOD + 1, g.n1);
     __syncthreads();
 
     constexpr int P4 = 4 * G::P;
@@ -453,7 +460,8 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
         const PlanPt& cur = Q[K & 3];
         const float caux = A[K & 3];
         load_point(x1 + 2, Q[(K + 2) & 3], A[(K + 2) & 3]);
-        load_plane(x1 + G::HHI + G::PERIOD + 1, PRE[(K + 1) & 1]);
+        load_plane(xf, PRE[(K + 1) & 1]);  // plane x1 + HHI + PERIOD + 1
+        xf = (xf + 1 == g.n1) ? 0 : xf + 1;
         float w1[4], w2[4], w3[4];
         lagrange_weights(plan_t(cur.w[0]), w1);
         lagrange_weights(plan_t(cur.w[1]), w2);
