@@ -409,15 +409,20 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
         for (int q = 0; q < G::FILL; ++q)
             if (tid + kLThreads * q < G::NV) store2(dst, q, bload(rf, foff[q][0], src), bload(rf, foff[q][1], src));
     }
-    // Loads run two planes ahead of their use: the plan words (and the
-    // epilogue operand) of plane x1 + 2 and the input plane
-    // x1 + HHI + PERIOD + 1 are issued while plane x1 is evaluated, so each
-    // load has two steps of evaluation to arrive.  Q[k & 3] / A[k & 3] hold
-    // the plan words and operand of step k; PRE[k & 1] the input plane that
-    // step k stores into a free ring slot.
+    // Loads run ahead of their use: while plane x1 is evaluated, the plan
+    // words (and the epilogue operand) of plane x1 + LA_PT and the input
+    // plane x1 + HHI + PERIOD + LA_PL are issued.  Q[k & 3] / A[k & 3] hold
+    // the plan words and operand of step k; PRE[k % NPRE] the input plane
+    // that step k lands in a free ring slot.  Measured at 256³ (outputs
+    // identical): plane loads two steps ahead instead of one take the steps
+    // from 104.5 to 103.7 µs for 8 more VGPRs (123); plan words three steps
+    // ahead instead of two are slower (105 µs).
+    constexpr int LA_PT = 2;  // steps between a plan word load and its use
+    constexpr int LA_PL = 2;  // steps between an input plane load and its landing
+    constexpr int NPRE = (LA_PL == 1) ? 2 : 4;
     PlanPt Q[4];
     float A[4];
-    float PRE[2][G::FILL][2];
+    float PRE[NPRE][G::FILL][2];
     auto load_point = [&](int x, PlanPt& q, float& a) {
         const int xc = min(x, x1e - 1);  // past the chunk end: a plane of this chunk, never used
         const uint32_t pb = static_cast<uint32_t>(xc) * plane4;
@@ -432,12 +437,11 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
             pre[q][1] = bload(rf, foff[q][1], src);
         }
     };
-    load_point(x1s, Q[0], A[0]);
-    load_point(x1s + 1, Q[1], A[1]);
-    load_plane(pmod(x1s + G::HHI + G::PERIOD, g.n1), PRE[0]);
+    for (int k = 0; k < LA_PT; ++k) load_point(x1s + k, Q[k], A[k]);
+    for (int k = 0; k < LA_PL; ++k) load_plane(pmod(x1s + G::HHI + G::PERIOD + k, g.n1), PRE[k]);
     // periodic index of the input plane the next step loads, advanced by one
     // per step (a scalar modulo per step costs about 20 scalar instructions)
-    int xf = pmod(x1s + G::HHI + G::PERIOD + 1, g.n1);
+    int xf = pmod(x1s + G::HHI + G::PERIOD + LA_PL, g.n1);
     __syncthreads();
 
     constexpr int P4 = 4 * G::P;
@@ -459,8 +463,8 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
         constexpr int K = decltype(kconst)::value;
         const PlanPt& cur = Q[K & 3];
         const float caux = A[K & 3];
-        load_point(x1 + 2, Q[(K + 2) & 3], A[(K + 2) & 3]);
-        load_plane(xf, PRE[(K + 1) & 1]);  // plane x1 + HHI + PERIOD + 1
+        load_point(x1 + LA_PT, Q[(K + LA_PT) & 3], A[(K + LA_PT) & 3]);
+        load_plane(xf, PRE[(K + LA_PL) % NPRE]);  // plane x1 + HHI + PERIOD + LA_PL
         xf = (xf + 1 == g.n1) ? 0 : xf + 1;
         float w1[4], w2[4], w3[4];
         lagrange_weights(plan_t(cur.w[0]), w1);
@@ -525,7 +529,7 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
         float* dst = lds + spare * G::P;
 #pragma unroll
         for (int q = 0; q < G::FILL; ++q)
-            if (tid + kLThreads * q < G::NV) store2(dst, q, PRE[K & 1][q][0], PRE[K & 1][q][1]);
+            if (tid + kLThreads * q < G::NV) store2(dst, q, PRE[K % NPRE][q][0], PRE[K % NPRE][q][1]);
         cs = (cs + 1 == G::W) ? 0 : cs + 1;
         if constexpr (K % G::PERIOD == G::PERIOD - 1) __syncthreads();
     };
