@@ -261,11 +261,15 @@ struct EpiTrajRk2 {
     const float* v;
     float* out;
     int64_t zstride;
-    float hx[3];
+    float hx0, hx1, hx2;  // grid spacings
     float shth;  // σ·ht/2
     int z;
+    float hz;  // spacing of axis z, set by select
     __device__ void select(int zz) {
         z = zz;
+        // a blend, not a conditional: the compiler turns a conditional choice of
+        // members into an indexed load, which keeps the struct in scratch memory
+        hz = hx0 * static_cast<float>(zz == 0) + hx1 * static_cast<float>(zz == 1) + hx2 * static_cast<float>(zz == 2);
         v += zz * zstride;
         out += zz * zstride;
     }
@@ -273,7 +277,7 @@ struct EpiTrajRk2 {
     __device__ float* outp() const { return out; }
     __device__ float apply_at(float vx, float vl, int i1, int i2, int i3) const {
         const int idx = z == 0 ? i1 : (z == 1 ? i2 : i3);
-        const float x = hx[z] * static_cast<float>(idx);
+        const float x = hz * static_cast<float>(idx);
         const float t = x - shth * (vx + vl);
         return static_cast<float>(static_cast<double>(t) / (2.0 * 3.14159265358979323846));
     }
@@ -296,12 +300,16 @@ struct EpiTrajRk4 {
     float* out;
     float* f;
     int64_t zstride;
-    float hx[3];
+    float hx0, hx1, hx2;  // grid spacings
     float step;  // σ·ht/2 (stage 1), σ·ht (stage 2)
     double s6;   // σ·(ht/6) (stage 3)
     int z;
+    float hz;  // spacing of axis z, set by select
     __device__ void select(int zz) {
         z = zz;
+        // a blend, not a conditional: the compiler turns a conditional choice of
+        // members into an indexed load, which keeps the struct in scratch memory
+        hz = hx0 * static_cast<float>(zz == 0) + hx1 * static_cast<float>(zz == 1) + hx2 * static_cast<float>(zz == 2);
         a += zz * zstride;
         out += zz * zstride;
         f += zz * zstride;
@@ -311,7 +319,7 @@ struct EpiTrajRk4 {
     __device__ float* outp2() const { return f; }
     __device__ float apply_at(float vx, float av, int i1, int i2, int i3) const {
         const int idx = z == 0 ? i1 : (z == 1 ? i2 : i3);
-        const float x = hx[z] * static_cast<float>(idx);
+        const float x = hz * static_cast<float>(idx);
         constexpr double twopi = 2.0 * 3.14159265358979323846;
         if constexpr (STAGE < 3) {
             const float t = x - step * vx;

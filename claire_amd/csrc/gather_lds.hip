@@ -208,20 +208,28 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
     constexpr int P4 = 4 * G::P;  // bytes per ring slot
     const char* ldsb = reinterpret_cast<const char*>(lds);
     int cs = 0;  // ring slot of plane x1 − HLO
-    // periodic index of plane x1 + HHI + 1, advanced by one per step (a
+    // Input planes are loaded two steps before they are landed: step x1
+    // lands plane x1 + HHI + 1 (in pa or pb, loaded by step x1 − 1) and
+    // loads plane x1 + HHI + 2 into the other set.
+    float pa[G::FILL], pb2[G::FILL];
+    {
+        const uint32_t src = static_cast<uint32_t>(pmod(x1s + G::HHI + 1, g.n1)) * plane4;
+#pragma unroll
+        for (int q = 0; q < G::FILL; ++q) pa[q] = bload(rf, foff[q], src);
+    }
+    // periodic index of plane x1 + HHI + 2, advanced by one per step (a
     // scalar modulo per step costs about 20 scalar instructions)
-    int xf = pmod(x1s + G::HHI + 1, g.n1);
+    int xf = pmod(x1s + G::HHI + 2, g.n1);
     auto step = [&](int x1, const PlanPt (&cur)[G::PTS], const float (&caux)[G::PTS], PlanPt (&nxt)[G::PTS],
-                    float (&naux)[G::PTS]) {
-        // Prefetch for the next step, unconditionally (past the chunk end it
+                    float (&naux)[G::PTS], const float (&pre)[G::FILL], float (&pnext)[G::FILL]) {
+        // Prefetch for the next steps, unconditionally (past the chunk end it
         // reloads a plane of this chunk; the values are never used).
         const int xn = (x1 + 1 < x1e) ? x1 + 1 : x1;
-        float pre[G::FILL];
         {
             const uint32_t src = static_cast<uint32_t>(xf) * plane4;
             xf = (xf + 1 == g.n1) ? 0 : xf + 1;
 #pragma unroll
-            for (int q = 0; q < G::FILL; ++q) pre[q] = bload(rf, foff[q], src);
+            for (int q = 0; q < G::FILL; ++q) pnext[q] = bload(rf, foff[q], src);
         }
         {
             const uint32_t pb = static_cast<uint32_t>(xn) * plane4;
@@ -301,10 +309,10 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
 
     int x1 = x1s;
     for (; x1 + 1 < x1e; x1 += 2) {
-        step(x1, qa, aa, qb, ab);
-        step(x1 + 1, qb, ab, qa, aa);
+        step(x1, qa, aa, qb, ab, pa, pb2);
+        step(x1 + 1, qb, ab, qa, aa, pb2, pa);
     }
-    if (x1 < x1e) step(x1, qa, aa, qb, ab);
+    if (x1 < x1e) step(x1, qa, aa, qb, ab, pa, pb2);
 }
 
 // Pair layout for D <= 2.  Entry c of an LDS row holds the two consecutive
@@ -917,7 +925,7 @@ void launch_lds_traj_rk2_corrector(const Dims& d, const uint32_t* P, int dbound,
     const GatherDims g = gather_dims(d);
     const int64_t vz = v[1] - v[0];
     CLR_REQUIRE(v[2] - v[1] == vz && vz == g.nl, kErrInvalidArgument, "velocity components must be contiguous");
-    EpiTrajRk2 e{v[0], W, g.nl, {d.hx[0], d.hx[1], d.hx[2]}, sigma * (0.5f * ht), 0};
+    EpiTrajRk2 e{v[0], W, g.nl, d.hx[0], d.hx[1], d.hx[2], sigma * (0.5f * ht), 0};
     dispatch(dbound, g, P, v[0], vz, 3, e, st);
 }
 
@@ -935,15 +943,15 @@ void launch_lds_traj_rk4_stage(int stage, const Dims& d, const uint32_t* P, int 
     const double s6 = static_cast<double>(sigma) * (static_cast<double>(ht) / 6.0);
     switch (stage) {
         case 1: {
-            EpiTrajRk4<1> e{v[0], W, Fout, g.nl, {hx[0], hx[1], hx[2]}, sigma * hthalf, s6, 0};
+            EpiTrajRk4<1> e{v[0], W, Fout, g.nl, hx[0], hx[1], hx[2], sigma * hthalf, s6, 0};
             return dispatch(dbound, g, P, v[0], vz, 3, e, st);
         }
         case 2: {
-            EpiTrajRk4<2> e{Fin, W, Fout, g.nl, {hx[0], hx[1], hx[2]}, sigma * ht, s6, 0};
+            EpiTrajRk4<2> e{Fin, W, Fout, g.nl, hx[0], hx[1], hx[2], sigma * ht, s6, 0};
             return dispatch(dbound, g, P, v[0], vz, 3, e, st);
         }
         case 3: {
-            EpiTrajRk4<3> e{Fin, W, W, g.nl, {hx[0], hx[1], hx[2]}, 0.0f, s6, 0};  // no second output
+            EpiTrajRk4<3> e{Fin, W, W, g.nl, hx[0], hx[1], hx[2], 0.0f, s6, 0};  // no second output
             return dispatch(dbound, g, P, v[0], vz, 3, e, st);
         }
         default:
