@@ -730,11 +730,9 @@ int num_cus() {
 
 // CLAIRE_AMD_GATHER_CHUNK (planes, >= 4) overrides the choice, for tuning.
 int chunk_override() {
-    static const int v = [] {
-        const char* e = std::getenv("CLAIRE_AMD_GATHER_CHUNK");
-        return e ? std::atoi(e) : 0;
-    }();
-    return v;
+    // read at every launch (not cached) so that a process can switch it
+    const char* e = std::getenv("CLAIRE_AMD_GATHER_CHUNK");
+    return e ? std::atoi(e) : 0;
 }
 
 // CLAIRE_AMD_GATHER_PAIR=0 selects the one-value-per-entry layout for single-field D <= 2 gathers

@@ -443,3 +443,36 @@ def test_hessian_matvec_graph_replay():
             d.free()
         eager.close()
         graph.close()
+
+
+@pytest.mark.parametrize("cfl", [1.5, 4.5], ids=["pair-layout", "bound-4"])
+def test_gather_chunk_lengths_bitwise(cfl, monkeypatch):
+    """The chunk length a gather block marches along x1 only splits the planes
+    among blocks: every length, including ones that leave 1-3 planes after
+    the unrolled steps of four, must reproduce the tuned launch bit for bit
+    (scalar and vector gathers along both trajectories, the state solve and
+    the Hessian).  CFL 1.5 keeps the foot points within a cell per step
+    (bound D <= 2, pair layout); 4.5 moves them up to about 2.7 cells (bound
+    3-4, the one-value D <= 4 kernel)."""
+    shape = (48, 48, 64)
+    v, mT, mR, vt = _data(shape, seed=11, cfl=cfl)
+
+    def outputs():
+        c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA)
+        try:
+            c.set_velocity(v)
+            c.set_template_image(mT)
+            c.set_reference_image(mR)
+            return [c.interp_scalar(mT, STATE), c.interp_scalar(mT, ADJOINT), c.interp_vector(vt, STATE),
+                    c.solve_state(), c.hessian_matvec(vt)]
+        finally:
+            c.close()
+
+    monkeypatch.delenv("CLAIRE_AMD_GATHER_CHUNK", raising=False)
+    want = outputs()
+    for chunk in (4, 5, 6, 7, 9, 48):
+        print(f"[gpu case] cfl={cfl} chunk={chunk}", flush=True)
+        monkeypatch.setenv("CLAIRE_AMD_GATHER_CHUNK", str(chunk))
+        got = outputs()
+        for name, a, b in zip(("interp state", "interp adjoint", "interp vector", "state", "hessian"), got, want):
+            assert np.array_equal(a, b), (chunk, name, float(np.abs(a - b).max()))
