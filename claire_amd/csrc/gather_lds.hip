@@ -615,28 +615,57 @@ struct RhsP {
 
 // Src = SrcF32 (plain) or SrcBfp (mixed form: the Newton-invariant gradient
 // fields stored as 16-bit block floating point, accumulated in FP32).
+// Each wave of the streaming kernels covers kStreamK consecutive runs of 64
+// float4 (1 KB of FP32) of every stream, lane l taking float4 l of each run,
+// so each of the 30-odd streams is read in 2 KB bursts.  Measured at 256³
+// (outputs identical): k_inc_state_rhs 399 → 367 µs and k_body_force 256 →
+// 243 µs with two runs per wave; four runs (227 VGPRs for the right-hand
+// sides) were slower than two.  A microbenchmark of 30 read and 4 write
+// streams gave 5.6, 6.2 and 6.6 TB/s for one, two and four runs.
+constexpr int kStreamK = 2;
+__device__ __forceinline__ int64_t stream_base(int k) {
+    return static_cast<int64_t>(blockIdx.x) * 256 * k + (threadIdx.x >> 6) * 64 * k + (threadIdx.x & 63);
+}
+
 template <class Src>
 __global__ __launch_bounds__(256) void k_inc_state_rhs(int64_t nl4, int nt, float hthalf, RhsP<Src> p) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= nl4) return;
+    const int64_t i0 = stream_base(kStreamK);
     const int64_t nl = nl4 * 4;
-    const float4 x0 = ld4(p.vx[0], i), x1 = ld4(p.vx[1], i), x2 = ld4(p.vx[2], i);
-    const float4 v0 = ld4(p.vt[0], i), v1 = ld4(p.vt[1], i), v2 = ld4(p.vt[2], i);
+    float4 x0[kStreamK], x1[kStreamK], x2[kStreamK], v0[kStreamK], v1[kStreamK], v2[kStreamK];
+#pragma unroll
+    for (int k = 0; k < kStreamK; ++k) {
+        const int64_t i = min(i0 + 64 * k, nl4 - 1);  // past the end: reload the last group, never stored
+        x0[k] = ld4(p.vx[0], i); x1[k] = ld4(p.vx[1], i); x2[k] = ld4(p.vx[2], i);
+        v0[k] = ld4(p.vt[0], i); v1[k] = ld4(p.vt[1], i); v2[k] = ld4(p.vt[2], i);
+    }
     for (int j = 0; j < nt; ++j) {
-        const float4 a0 = p.gX.load(3 * j, nl, i), a1 = p.gX.load(3 * j + 1, nl, i),
-                     a2 = p.gX.load(3 * j + 2, nl, i);
-        const float4 c0 = p.gm.load(3 * (j + 1), nl, i), c1 = p.gm.load(3 * (j + 1) + 1, nl, i),
-                     c2 = p.gm.load(3 * (j + 1) + 2, nl, i);
-        float4 s;
-        s.x = hthalf * (a0.x * x0.x + a1.x * x1.x + a2.x * x2.x) + hthalf * (c0.x * v0.x + c1.x * v1.x + c2.x * v2.x);
-        s.y = hthalf * (a0.y * x0.y + a1.y * x1.y + a2.y * x2.y) + hthalf * (c0.y * v0.y + c1.y * v1.y + c2.y * v2.y);
-        s.z = hthalf * (a0.z * x0.z + a1.z * x1.z + a2.z * x2.z) + hthalf * (c0.z * v0.z + c1.z * v1.z + c2.z * v2.z);
-        s.w = hthalf * (a0.w * x0.w + a1.w * x1.w + a2.w * x2.w) + hthalf * (c0.w * v0.w + c1.w * v1.w + c2.w * v2.w);
-        if (j == 0) {
-            if (nt != 1) s = make_float4(-s.x, -s.y, -s.z, -s.w);
-            reinterpret_cast<float4*>(p.mt1)[i] = s;
-        } else {
-            reinterpret_cast<float4*>(p.s + static_cast<int64_t>(j) * nl)[i] = s;
+        float4 a0[kStreamK], a1[kStreamK], a2[kStreamK], c0[kStreamK], c1[kStreamK], c2[kStreamK];
+#pragma unroll
+        for (int k = 0; k < kStreamK; ++k) {
+            const int64_t i = min(i0 + 64 * k, nl4 - 1);
+            a0[k] = p.gX.load(3 * j, nl, i); a1[k] = p.gX.load(3 * j + 1, nl, i); a2[k] = p.gX.load(3 * j + 2, nl, i);
+            c0[k] = p.gm.load(3 * (j + 1), nl, i); c1[k] = p.gm.load(3 * (j + 1) + 1, nl, i);
+            c2[k] = p.gm.load(3 * (j + 1) + 2, nl, i);
+        }
+#pragma unroll
+        for (int k = 0; k < kStreamK; ++k) {
+            const int64_t i = i0 + 64 * k;
+            if (i >= nl4) continue;
+            float4 s;
+            s.x = hthalf * (a0[k].x * x0[k].x + a1[k].x * x1[k].x + a2[k].x * x2[k].x) +
+                  hthalf * (c0[k].x * v0[k].x + c1[k].x * v1[k].x + c2[k].x * v2[k].x);
+            s.y = hthalf * (a0[k].y * x0[k].y + a1[k].y * x1[k].y + a2[k].y * x2[k].y) +
+                  hthalf * (c0[k].y * v0[k].y + c1[k].y * v1[k].y + c2[k].y * v2[k].y);
+            s.z = hthalf * (a0[k].z * x0[k].z + a1[k].z * x1[k].z + a2[k].z * x2[k].z) +
+                  hthalf * (c0[k].z * v0[k].z + c1[k].z * v1[k].z + c2[k].z * v2[k].z);
+            s.w = hthalf * (a0[k].w * x0[k].w + a1[k].w * x1[k].w + a2[k].w * x2[k].w) +
+                  hthalf * (c0[k].w * v0[k].w + c1[k].w * v1[k].w + c2[k].w * v2[k].w);
+            if (j == 0) {
+                if (nt != 1) s = make_float4(-s.x, -s.y, -s.z, -s.w);
+                reinterpret_cast<float4*>(p.mt1)[i] = s;
+            } else {
+                reinterpret_cast<float4*>(p.s + static_cast<int64_t>(j) * nl)[i] = s;
+            }
         }
     }
 }
@@ -654,25 +683,42 @@ struct BodyP {
 
 template <class Src>
 __global__ __launch_bounds__(256) void k_body_force(int64_t nl4, int nt, float ht, float hd, BodyP<Src> p) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= nl4) return;
+    const int64_t i0 = stream_base(kStreamK);
     const int64_t nl = nl4 * 4;
-    float4 s[3] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+    float4 s[kStreamK][3];
+#pragma unroll
+    for (int k = 0; k < kStreamK; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s[k][d] = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int j = 0; j <= nt; ++j) {
         const float w = (j == 0 || j == nt) ? 0.5f * ht : ht;
-        const float4 l = ld4(p.lam + static_cast<int64_t>(j) * nl, i);
+        float4 l[kStreamK], g[kStreamK][3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const float4 gd = p.gm.load(3 * (nt - j) + d, nl, i);
-            s[d].x += w * gd.x * l.x;
-            s[d].y += w * gd.y * l.y;
-            s[d].z += w * gd.z * l.z;
-            s[d].w += w * gd.w * l.w;
+        for (int k = 0; k < kStreamK; ++k) {
+            const int64_t i = min(i0 + 64 * k, nl4 - 1);  // past the end: reload the last group, never stored
+            l[k] = ld4(p.lam + static_cast<int64_t>(j) * nl, i);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) g[k][d] = p.gm.load(3 * (nt - j) + d, nl, i);
         }
+#pragma unroll
+        for (int k = 0; k < kStreamK; ++k)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                s[k][d].x += w * g[k][d].x * l[k].x;
+                s[k][d].y += w * g[k][d].y * l[k].y;
+                s[k][d].z += w * g[k][d].z * l[k].z;
+                s[k][d].w += w * g[k][d].w * l[k].w;
+            }
     }
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-        reinterpret_cast<float4*>(p.b[d])[i] = make_float4(s[d].x * hd, s[d].y * hd, s[d].z * hd, s[d].w * hd);
+    for (int k = 0; k < kStreamK; ++k) {
+        const int64_t i = i0 + 64 * k;
+        if (i >= nl4) continue;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            reinterpret_cast<float4*>(p.b[d])[i] =
+                make_float4(s[k][d].x * hd, s[k][d].y * hd, s[k][d].z * hd, s[k][d].w * hd);
+    }
 }
 
 // FP32 → 16-bit block floating point, field blockIdx.y, 4 values per thread.
@@ -964,7 +1010,7 @@ void body_force(int64_t nl, int nt, float ht, float hd, const float* lam, const 
     CLR_REQUIRE(nl % 4 == 0, kErrInvalidArgument, "grid size must be a multiple of 4 points");
     const int64_t nl4 = nl / 4;
     BodyP<Src> p{lam, gm, {b[0], b[1], b[2]}};
-    const int nb = static_cast<int>((nl4 + 255) / 256);
+    const int nb = static_cast<int>((nl4 + 256 * kStreamK - 1) / (256 * kStreamK));
     hipLaunchKernelGGL((k_body_force<Src>), dim3(nb), dim3(256), 0, st, nl4, nt, ht, hd, p);
 }
 
@@ -975,7 +1021,7 @@ void inc_state_rhs(int64_t nl, int nt, float ht, const float* const vx[3], const
     CLR_REQUIRE(nt >= 1, kErrInvalidArgument, "nt must be >= 1");
     const int64_t nl4 = nl / 4;
     RhsP<Src> p{{vx[0], vx[1], vx[2]}, {vt[0], vt[1], vt[2]}, gX, gm, s, mt1};
-    const int nb = static_cast<int>((nl4 + 255) / 256);
+    const int nb = static_cast<int>((nl4 + 256 * kStreamK - 1) / (256 * kStreamK));
     hipLaunchKernelGGL((k_inc_state_rhs<Src>), dim3(nb), dim3(256), 0, st, nl4, nt, 0.5f * ht, p);
 }
 
