@@ -507,6 +507,27 @@ __device__ __forceinline__ int64_t col_base(const ColGeom& g, int64_t cg) {
     return static_cast<int64_t>(q) * g.outer + static_cast<int64_t>(c - q * in);
 }
 
+// Raw buffer access to one field of a complex spectrum of `elems` values: a
+// per-lane byte offset (the lane's column and row) plus a uniform one (the
+// element's stride multiple), both 32-bit, so a column pass keeps one address
+// register per lane instead of a 64-bit address per element.  The launchers
+// require fields under 2 GiB.
+using spec_rsrc_t = __amdgpu_buffer_rsrc_t;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ spec_rsrc_t spec_rsrc(const c32* p, int64_t elems) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<c32*>(p), 0, static_cast<int>(elems * 8), 0x00020000);
+}
+__device__ __forceinline__ c32 bload_c(spec_rsrc_t r, uint32_t voff, uint32_t soff) {
+    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    return {__uint_as_float(w.x), __uint_as_float(w.y)};
+}
+__device__ __forceinline__ void bstore_c(spec_rsrc_t r, c32 v, uint32_t voff, uint32_t soff) {
+    u32x2 w;
+    w.x = __float_as_uint(v.x);
+    w.y = __float_as_uint(v.y);
+    __builtin_amdgcn_raw_buffer_store_b64(w, r, voff, soff, 0);
+}
+
 template <int N, int DIR>
 __global__ __launch_bounds__(ColGeo<N>::THREADS) void k_col(ColPtrs ptr, ColGeom geo, const c32* __restrict__ tw) {
     const c32* in = ptr.in[blockIdx.y];  // may equal out (in place): no __restrict__
@@ -526,21 +547,23 @@ __global__ __launch_bounds__(ColGeo<N>::THREADS) void k_col(ColPtrs ptr, ColGeom
     const bool valid = cg < geo.ncols;
     const int64_t base = valid ? col_base<N>(geo, cg) : 0;
     c32 v[V];
+    const spec_rsrc_t rin = spec_rsrc(in, geo.ncols * N), rout = spec_rsrc(out, geo.ncols * N);
+    const uint32_t es8 = static_cast<uint32_t>(geo.estride) * 8u;
+    const uint32_t lane = static_cast<uint32_t>(base) * 8u + static_cast<uint32_t>(t) * es8;
 #pragma unroll
-    for (int q = 0; q < V / R0; ++q) {
-        const int j = t + T * q;
+    for (int q = 0; q < V / R0; ++q)
 #pragma unroll
-        for (int r = 0; r < R0; ++r)
-            v[q * R0 + r] = valid ? in[base + static_cast<int64_t>(j + r * (N / R0)) * geo.estride] : c32{0.f, 0.f};
-    }
+        for (int r = 0; r < R0; ++r) {
+            const c32 x = bload_c(rin, lane, static_cast<uint32_t>(T * q + r * (N / R0)) * es8);
+            v[q * R0 + r] = valid ? x : c32{0.f, 0.f};
+        }
     fft_regs<N, DIR>(v, s, ColLayout{G::LD}, c, t, tw);
     if (valid) {
 #pragma unroll
-        for (int q = 0; q < V / RL; ++q) {
-            const int j = t + T * q;
+        for (int q = 0; q < V / RL; ++q)
 #pragma unroll
-            for (int r = 0; r < RL; ++r) out[base + static_cast<int64_t>(j + r * (N / RL)) * geo.estride] = v[q * RL + r];
-        }
+            for (int r = 0; r < RL; ++r)
+                bstore_c(rout, v[q * RL + r], lane, static_cast<uint32_t>(T * q + r * (N / RL)) * es8);
     }
 }
 
@@ -596,14 +619,17 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_x1_fused(Ptr3 ptr, ColGeom 
     const int k3 = static_cast<int>(cg) - k2 * nh;
     const int w2 = wavenum(k2, n2), w3 = wavenum(k3, n3);
 
+    const uint32_t es8 = static_cast<uint32_t>(geo.estride) * 8u;
+    const uint32_t lane = static_cast<uint32_t>(base) * 8u + static_cast<uint32_t>(t) * es8;
     auto load_fwd = [&](const c32* src, c32 (&v)[V]) {
+        const spec_rsrc_t r_in = spec_rsrc(src, geo.ncols * N);
 #pragma unroll
-        for (int q = 0; q < V / R0; ++q) {
-            const int j = t + T * q;
+        for (int q = 0; q < V / R0; ++q)
 #pragma unroll
-            for (int r = 0; r < R0; ++r)
-                v[q * R0 + r] = valid ? src[base + static_cast<int64_t>(j + r * (N / R0)) * geo.estride] : c32{0.f, 0.f};
-        }
+            for (int r = 0; r < R0; ++r) {
+                const c32 x = bload_c(r_in, lane, static_cast<uint32_t>(T * q + r * (N / R0)) * es8);
+                v[q * R0 + r] = valid ? x : c32{0.f, 0.f};
+            }
         fft_regs<N, -1>(v, s, ColLayout{G::LD}, c, t, tw);
     };
     // After the forward pass v[q·RL + r] = X[k1], k1 = j + r·N/RL; the
@@ -612,12 +638,12 @@ __global__ __launch_bounds__(Geo<N>::THREADS) void k_x1_fused(Ptr3 ptr, ColGeom 
         fft_regs<N, +1>(v, s, ColLayout{G::LD}, c, t, tw);
         constexpr int RI = Last<typename Plan<N>::Inv>::value;
         if (valid) {
+            const spec_rsrc_t r_out = spec_rsrc(dst, geo.ncols * N);
 #pragma unroll
-            for (int q = 0; q < V / RI; ++q) {
-                const int j = t + T * q;
+            for (int q = 0; q < V / RI; ++q)
 #pragma unroll
-                for (int r = 0; r < RI; ++r) dst[base + static_cast<int64_t>(j + r * (N / RI)) * geo.estride] = v[q * RI + r];
-            }
+                for (int r = 0; r < RI; ++r)
+                    bstore_c(r_out, v[q * RI + r], lane, static_cast<uint32_t>(T * q + r * (N / RI)) * es8);
         }
     };
 
@@ -761,6 +787,8 @@ template <int N>
 struct ColPass {
     static void go(const ColPtrs& p, int ncomp, ColGeom geo, int dir, const c32* tw, hipStream_t st) {
         using G = ColGeo<N>;
+        CLR_REQUIRE(geo.ncols * N * 8 < (int64_t(1) << 31), kErrInvalidArgument,
+                    "spectrum field too large for 32-bit buffer offsets");
         const dim3 grid(static_cast<unsigned>((geo.ncols + G::B - 1) / G::B), ncomp);
         if (dir < 0)
             hipLaunchKernelGGL((k_col<N, -1>), grid, dim3(G::THREADS), 0, st, p, geo, tw);
@@ -781,6 +809,8 @@ struct X1Fused {
     static void go(int mode, Ptr3 p, ColGeom geo, const c32* tw, SpecParams sp, int n2, int n3, int nh, int ncomp,
                    hipStream_t st) {
         using G = Geo<N>;
+        CLR_REQUIRE(geo.ncols * N * 8 < (int64_t(1) << 31), kErrInvalidArgument,
+                    "spectrum field too large for 32-bit buffer offsets");
         const int64_t blocks = (geo.ncols + G::B - 1) / G::B;
         dim3 grid(static_cast<unsigned>(blocks), mode == kFuseMult ? ncomp : 1);
         switch (mode) {
