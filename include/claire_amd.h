@@ -6,7 +6,10 @@
  * are DEVICE pointers to FP32 data; vector fields are flat [v1|v2|v3] with
  * each component stored x3-fastest (the layout of the PETSc vectors TAO/KSP
  * hand to CLAIRE, VecField::SetComponents/GetComponents,
- * src/VecField.cpp:396-476).  Work is ordered on the stream given to
+ * src/VecField.cpp:396-476).  Field pointers must be 16-byte aligned
+ * (hipMalloc returns 256-byte aligned memory, and a component of a vector
+ * field starts 4·N1·N2·N3 bytes after the previous one): the streaming
+ * kernels and the FFT's real passes move 16 bytes per lane.  Work is ordered on the stream given to
  * clr_create (0 = the null stream).  Every function returns 0 on success or
  * a nonzero status (CLR_ERR_*) with a message from clr_last_error(); this
  * maps one-to-one onto PetscErrorCode at the binding.  A context is not
