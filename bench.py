@@ -35,8 +35,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--n", type=int, default=256, help="grid size per axis")
     p.add_argument("--nt", type=int, default=4)
     p.add_argument("--no-cpu-baseline", action="store_true")
