@@ -476,3 +476,33 @@ def test_gather_chunk_lengths_bitwise(cfl, monkeypatch):
         got = outputs()
         for name, a, b in zip(("interp state", "interp adjoint", "interp vector", "state", "hessian"), got, want):
             assert np.array_equal(a, b), (chunk, name, float(np.abs(a - b).max()))
+
+
+def test_grid_thinner_than_the_gather_ring(oracle):
+    """A grid of 8 planes along x1, fewer than the 11 input planes the
+    pair-layout ring holds (its slots then hold periodic copies of the same
+    planes): gathers along both trajectories, the state solve and the
+    Hessian against the oracle, at the bars of the other parity tests."""
+    shape = (8, 32, 48)
+    print(f"\n[gpu case] thin grid shape={shape}", flush=True)
+    v, mT, mR, vt = _data(shape, seed=21, cfl=1.5)
+    c = Claire(shape, nt=NT, regnorm="h1sn", beta=BETA)
+    try:
+        c.set_velocity(v)
+        c.set_template_image(mT)
+        c.set_reference_image(mR)
+        Xs = oracle.trajectory(v, NT, +1, prec="f32")
+        Xa = oracle.trajectory(v, NT, -1, prec="f32")
+        assert relerr(c.interp_scalar(mT, STATE), oracle.interp(mT, Xs, prec="f32")) < 1e-5
+        vv = c.interp_vector(vt, ADJOINT)
+        for d in range(3):
+            assert relerr(vv[d], oracle.interp(vt[d], Xa, prec="f32")) < 1e-5
+        m = c.solve_state()
+        m32 = oracle.solve_state(v, mT, NT, prec="f32")
+        assert relerr(m, m32) < 1e-5
+        H = c.hessian_matvec(vt)
+        Ho = oracle.hessian_matvec(v, m32, vt, NT, "h1sn", BETA, prec="f32")
+        H64 = oracle.hessian_matvec(v, oracle.solve_state(v, mT, NT), vt, NT, "h1sn", BETA)
+        assert_fp32_parity(H, Ho, H64)
+    finally:
+        c.close()
