@@ -88,7 +88,17 @@ void Profiler::reset() {
 // ---------------------------------------------------------------------------
 // Reducer (FP64)
 // ---------------------------------------------------------------------------
-Reducer::Reducer(const RegOpt* opt) : m_Opt(opt), m_Partial(kReduceBlocks), m_Result(kSlots) {}
+Reducer::Reducer(const RegOpt* opt) : m_Opt(opt), m_Partial(2 * kReduceBlocks), m_Result(kSlots) {}
+
+void Reducer::DotAlpha(const float* p, const float* w, IntType n, claire_amd::PcgState* s) {
+    KernelScope k(m_Opt->prof, "reduce_dot", 8.0 * n);
+    launch_dot_pcg_alpha(p, w, n, m_Partial.get(), m_Result.get(), s, m_Opt->stream);
+}
+
+void Reducer::DotsBeta(const float* r, const float* z, IntType n, claire_amd::PcgState* s) {
+    KernelScope k(m_Opt->prof, "reduce_dot2", 8.0 * n);
+    launch_dots_pcg_beta(r, z, n, m_Partial.get(), m_Result.get(), s, m_Opt->stream);
+}
 
 void Reducer::DotTo(const float* a, const float* b, IntType n, int slot) {
     CLR_REQUIRE(slot >= 0 && slot < kSlots, kErrInvalidArgument, "reduction slot out of range");

@@ -168,6 +168,10 @@ class Reducer {
     // once per product.
     static constexpr int kSlots = 16;
     void DotTo(const float* a, const float* b, IntType n, int slot);
+    // The CG products with the CG scalar step fused into the final sum:
+    // slot 0 = pᵀw then α; slots 1, 2 = rᵀr, rᵀz (one pass) then β.
+    void DotAlpha(const float* p, const float* w, IntType n, claire_amd::PcgState* s);
+    void DotsBeta(const float* r, const float* z, IntType n, claire_amd::PcgState* s);
     void Fetch(double* out, int nslots);
     // Device address of the result slots (read by kernels that consume the
     // products without a host round trip).

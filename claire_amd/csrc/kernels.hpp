@@ -239,6 +239,16 @@ void launch_pcg_beta(PcgState* s, const double* slots, hipStream_t st);
 void launch_axpy_pcg(const PcgState* s, int sign, const float* x, float* y, int64_t n, hipStream_t st);
 // p = β·p + z unless stopped
 void launch_xpay_pcg(const PcgState* s, float* p, const float* z, int64_t n, hipStream_t st);
+// d += α·p and r −= α·w in one launch unless stopped
+void launch_axpy2_pcg(const PcgState* s, const float* p, float* d, const float* w, float* r, int64_t n,
+                      hipStream_t st);
+// slots[0] = pᵀw (partial: kReduceBlocks doubles), then the α step
+void launch_dot_pcg_alpha(const float* p, const float* w, int64_t n, double* partial, double* slots, PcgState* s,
+                          hipStream_t st);
+// slots[1] = rᵀr and slots[2] = rᵀz in one pass (partial: 2·kReduceBlocks
+// doubles), then the stopping test and the β step
+void launch_dots_pcg_beta(const float* r, const float* z, int64_t n, double* partial, double* slots, PcgState* s,
+                          hipStream_t st);
 void launch_dot(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st);
 void launch_absmax(const float* a, int64_t n, double* partial, double* result, hipStream_t st);
 void launch_sqdist(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st);

@@ -59,10 +59,9 @@ __global__ void k_pcg_init(PcgState* s, const double* __restrict__ slots, double
     s->kmax = kmax;
     s->done = 0;
 }
-__global__ void k_pcg_alpha(PcgState* s, const double* __restrict__ slots) {
+__device__ __forceinline__ void pcg_alpha(PcgState* s, double pw) {
     if (s->done) return;
     s->k += 1;
-    const double pw = slots[0];
     if (!(pw > 0.0)) {  // indefinite operator or breakdown: stop before the update
         s->done = 1;
         return;
@@ -71,18 +70,19 @@ __global__ void k_pcg_alpha(PcgState* s, const double* __restrict__ slots) {
     s->a = static_cast<float>(a);
     s->na = static_cast<float>(-a);
 }
-__global__ void k_pcg_beta(PcgState* s, const double* __restrict__ slots) {
+__device__ __forceinline__ void pcg_beta(PcgState* s, double rr, double rz_new) {
     if (s->done) return;
-    const double rnorm = sqrt(slots[1]);
+    const double rnorm = sqrt(rr);
     s->rnorm = rnorm;
     if (rnorm <= s->tol || !(rnorm < 1e6 * s->bnorm) || s->k >= s->kmax) {
         s->done = 1;
         return;
     }
-    const double rz_new = slots[2];
     s->beta = static_cast<float>(rz_new / s->rz);
     s->rz = rz_new;
 }
+__global__ void k_pcg_alpha(PcgState* s, const double* __restrict__ slots) { pcg_alpha(s, slots[0]); }
+__global__ void k_pcg_beta(PcgState* s, const double* __restrict__ slots) { pcg_beta(s, slots[1], slots[2]); }
 __global__ __launch_bounds__(kThreads) void k_axpy_pcg(const PcgState* __restrict__ s, int sign,
                                                        const float* __restrict__ x, float* __restrict__ y,
                                                        int64_t n) {
@@ -90,6 +90,18 @@ __global__ __launch_bounds__(kThreads) void k_axpy_pcg(const PcgState* __restric
     const float a = sign > 0 ? s->a : s->na;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (i < n) y[i] = a * x[i] + y[i];
+}
+// d += α·p and r −= α·w in one pass (the two updates of a CG step)
+__global__ __launch_bounds__(kThreads) void k_axpy2_pcg(const PcgState* __restrict__ s, const float* __restrict__ p,
+                                                        float* __restrict__ d, const float* __restrict__ w,
+                                                        float* __restrict__ r, int64_t n) {
+    if (s->done) return;
+    const float a = s->a, na = s->na;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) {
+        d[i] = a * p[i] + d[i];
+        r[i] = na * w[i] + r[i];
+    }
 }
 __global__ __launch_bounds__(kThreads) void k_xpay_pcg(const PcgState* __restrict__ s, float* __restrict__ p,
                                                        const float* __restrict__ z, int64_t n) {
@@ -197,6 +209,87 @@ __global__ __launch_bounds__(kThreads) void k_reduce_final(const double* __restr
     if (threadIdx.x == 0) result[0] = r;
 }
 
+// Two dot products a·b and a·c in one pass, each folded exactly as
+// k_reduce_partial<kRedDot> folds it (same grid, same element order): the
+// partials of a·b go to partial[0, nblocks), those of a·c after them.
+template <bool VEC>
+__global__ __launch_bounds__(kThreads) void k_reduce_partial_dot2(const float* __restrict__ a,
+                                                                  const float* __restrict__ b,
+                                                                  const float* __restrict__ c, int64_t n,
+                                                                  double* __restrict__ partial) {
+    __shared__ double sh[kThreads];
+    double ab = 0.0, ac = 0.0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if constexpr (VEC) {
+        const int64_t n4 = n >> 2;
+        const float4* a4 = reinterpret_cast<const float4*>(a);
+        const float4* b4 = reinterpret_cast<const float4*>(b);
+        const float4* c4 = reinterpret_cast<const float4*>(c);
+        auto fold4 = [&](const float4& x, const float4& y, const float4& z) {
+            ab = red_step<kRedDot>(ab, x.x, y.x);
+            ab = red_step<kRedDot>(ab, x.y, y.y);
+            ab = red_step<kRedDot>(ab, x.z, y.z);
+            ab = red_step<kRedDot>(ab, x.w, y.w);
+            ac = red_step<kRedDot>(ac, x.x, z.x);
+            ac = red_step<kRedDot>(ac, x.y, z.y);
+            ac = red_step<kRedDot>(ac, x.z, z.z);
+            ac = red_step<kRedDot>(ac, x.w, z.w);
+        };
+        int64_t i = first;
+        for (; i + stride < n4; i += 2 * stride) {
+            const float4 x0 = a4[i], x1 = a4[i + stride];
+            const float4 y0 = b4[i], y1 = b4[i + stride];
+            const float4 z0 = c4[i], z1 = c4[i + stride];
+            fold4(x0, y0, z0);
+            fold4(x1, y1, z1);
+        }
+        if (i < n4) fold4(a4[i], b4[i], c4[i]);
+        const int64_t tail = 4 * n4 + threadIdx.x;
+        if (blockIdx.x == 0 && tail < n) {
+            ab = red_step<kRedDot>(ab, a[tail], b[tail]);
+            ac = red_step<kRedDot>(ac, a[tail], c[tail]);
+        }
+    } else {
+        for (int64_t i = first; i < n; i += stride) {
+            ab = red_step<kRedDot>(ab, a[i], b[i]);
+            ac = red_step<kRedDot>(ac, a[i], c[i]);
+        }
+    }
+    const double rab = block_reduce<kRedDot>(ab, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = rab;
+    __syncthreads();
+    const double rac = block_reduce<kRedDot>(ac, sh);
+    if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = rac;
+}
+
+// Final sums of the CG products, then the CG scalar step on thread 0:
+//   NSUM = 1: slots[0] = pᵀw, then α (as k_pcg_alpha);
+//   NSUM = 2: slots[1] = rᵀr, slots[2] = rᵀz, then the stopping test and β
+//             (as k_pcg_beta).
+template <int NSUM>
+__global__ __launch_bounds__(kThreads) void k_reduce_final_pcg(const double* __restrict__ partial, int np,
+                                                               double* __restrict__ slots, PcgState* s) {
+    __shared__ double sh[kThreads];
+    double sums[NSUM];
+#pragma unroll
+    for (int k = 0; k < NSUM; ++k) {
+        double acc = 0.0;
+        for (int i = threadIdx.x; i < np; i += kThreads) acc = red_combine<kRedDot>(acc, partial[k * np + i]);
+        sums[k] = block_reduce<kRedDot>(acc, sh);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if constexpr (NSUM == 1) {
+        slots[0] = sums[0];
+        pcg_alpha(s, sums[0]);
+    } else {
+        slots[1] = sums[0];
+        slots[2] = sums[1];
+        pcg_beta(s, sums[0], sums[1]);
+    }
+}
+
 template <int OP>
 void reduce(const float* a, const float* b, int64_t n, double* partial, double* result, hipStream_t st) {
     const uintptr_t align = reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b ? b : a);
@@ -239,6 +332,32 @@ void launch_pcg_beta(PcgState* s, const double* slots, hipStream_t st) {
 }
 void launch_axpy_pcg(const PcgState* s, int sign, const float* x, float* y, int64_t n, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_axpy_pcg, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, sign, x, y, n);
+}
+void launch_axpy2_pcg(const PcgState* s, const float* p, float* d, const float* w, float* r, int64_t n,
+                      hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_axpy2_pcg, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, p, d, w, r, n);
+}
+void launch_dot_pcg_alpha(const float* p, const float* w, int64_t n, double* partial, double* slots, PcgState* s,
+                          hipStream_t st) {
+    const uintptr_t align = reinterpret_cast<uintptr_t>(p) | reinterpret_cast<uintptr_t>(w);
+    if ((align & 15u) == 0)
+        hipLaunchKernelGGL((k_reduce_partial<kRedDot, true>), dim3(kReduceBlocks), dim3(kThreads), 0, st, p, w, n,
+                           partial);
+    else
+        hipLaunchKernelGGL((k_reduce_partial<kRedDot, false>), dim3(kReduceBlocks), dim3(kThreads), 0, st, p, w, n,
+                           partial);
+    hipLaunchKernelGGL(k_reduce_final_pcg<1>, dim3(1), dim3(kThreads), 0, st, partial, kReduceBlocks, slots, s);
+}
+void launch_dots_pcg_beta(const float* r, const float* z, int64_t n, double* partial, double* slots, PcgState* s,
+                          hipStream_t st) {
+    const uintptr_t align = reinterpret_cast<uintptr_t>(r) | reinterpret_cast<uintptr_t>(z);
+    if ((align & 15u) == 0)
+        hipLaunchKernelGGL(k_reduce_partial_dot2<true>, dim3(kReduceBlocks), dim3(kThreads), 0, st, r, r, z, n,
+                           partial);
+    else
+        hipLaunchKernelGGL(k_reduce_partial_dot2<false>, dim3(kReduceBlocks), dim3(kThreads), 0, st, r, r, z, n,
+                           partial);
+    hipLaunchKernelGGL(k_reduce_final_pcg<2>, dim3(1), dim3(kThreads), 0, st, partial, kReduceBlocks, slots, s);
 }
 void launch_xpay_pcg(const PcgState* s, float* p, const float* z, int64_t n, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_xpay_pcg, dim3(blocks_for(n)), dim3(kThreads), 0, st, s, p, z, n);

@@ -90,19 +90,17 @@ void Optimizer::PcgIteration(bool inline_matvec, const GNOptions& opt) {
     float* w = m_W.get();
     PcgState* s = m_State.get();
     Reducer* red = m_Prob->Red();
-    const double* slots = red->Slots();
     if (inline_matvec)
         m_Prob->HessianMatVecInline(w, p, true);
     else
         m_Prob->HessianMatVec(w, p, true);
-    red->DotTo(p, w, m_N, 0);
-    launch_pcg_alpha(s, slots, Stream());
-    launch_axpy_pcg(s, +1, p, d, m_N, Stream());
-    launch_axpy_pcg(s, -1, w, r, m_N, Stream());
+    // pᵀw and α in one reduction; both updates in one pass; rᵀr and rᵀz
+    // in one pass with the stopping test and β (the same sums and scalars
+    // as separate launches, bit for bit)
+    red->DotAlpha(p, w, m_N, s);
+    launch_axpy2_pcg(s, p, d, w, r, m_N, Stream());
     Precond(r, z, opt);
-    red->DotTo(r, r, m_N, 1);
-    red->DotTo(r, z, m_N, 2);
-    launch_pcg_beta(s, slots, Stream());
+    red->DotsBeta(r, z, m_N, s);
     launch_xpay_pcg(s, p, z, m_N, Stream());
 }
 
