@@ -48,6 +48,8 @@ def parse():
     p.add_argument("--no-batched", action="store_true",
                    help="skip the 48-registration batched leg")
     p.add_argument("--batch-workers", type=int, default=8)
+    p.add_argument("--batch-graphs", type=int, choices=(0, 1), default=1,
+                   help="batched leg: replay the Krylov iterations from captured graphs (1) or launch them one by one (0)")
     p.add_argument("--no-variants", action="store_true",
                    help="skip the option variants (matvec types, models, RK2/RK4, continuity) at the same size")
     p.add_argument("--no-large", action="store_true",
@@ -182,7 +184,7 @@ def batched_leg(args):
         shape = (n, n, n)
         mT = synth.ellipsoid_phantom(shape, count=30, seed=1000 + i)
         v = synth.bandlimited_velocity(shape, kmax=8, cfl=2.0, nt=nt, seed=2000 + i)
-        c = Claire(shape, nt=nt, regnorm="h1", beta=beta, graphs=True)
+        c = Claire(shape, nt=nt, regnorm="h1", beta=beta, graphs=bool(args.batch_graphs))
         c.set_velocity(v)
         c.set_template_image(mT)
         mR = c.solve_state()[nt]
@@ -211,7 +213,7 @@ def batched_leg(args):
             mism.append(float(np.linalg.norm(mR - m1) / np.linalg.norm(mR - mT)))
         its = [rep["iterations"] for _, rep in got]
         reasons = sorted({rep["reason"] for _, rep in got})
-        res.update({"workers": args.batch_workers, "seconds": round(dt_batch, 3),
+        res.update({"workers": args.batch_workers, "graphs": bool(args.batch_graphs), "seconds": round(dt_batch, 3),
                     "registrations_per_s": round(len(sizes) / dt_batch, 2),
                     "sequential_seconds": round(dt_seq, 3),
                     "speedup_vs_sequential": round(dt_seq / dt_batch, 2),
@@ -273,7 +275,7 @@ def golden_pairs_check(nt, beta):
         mT = synth.ellipsoid_phantom(shape, count=30, seed=1000 + i)
         v = synth.bandlimited_velocity(shape, kmax=8, cfl=2.0, nt=nt, seed=2000 + i)
         mR = oracle_py.solve_state(v.astype(np.float64), mT.astype(np.float64), nt)[nt].astype(np.float32)
-        c = Claire(shape, nt=nt, regnorm="h1", beta=beta, graphs=True)
+        c = Claire(shape, nt=nt, regnorm="h1", beta=beta, graphs=bool(args.batch_graphs))
         try:
             c.set_template_image(mT)
             c.set_reference_image(mR)
