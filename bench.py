@@ -275,7 +275,7 @@ def golden_pairs_check(nt, beta):
         mT = synth.ellipsoid_phantom(shape, count=30, seed=1000 + i)
         v = synth.bandlimited_velocity(shape, kmax=8, cfl=2.0, nt=nt, seed=2000 + i)
         mR = oracle_py.solve_state(v.astype(np.float64), mT.astype(np.float64), nt)[nt].astype(np.float32)
-        c = Claire(shape, nt=nt, regnorm="h1", beta=beta, graphs=bool(args.batch_graphs))
+        c = Claire(shape, nt=nt, regnorm="h1", beta=beta, graphs=True)
         try:
             c.set_template_image(mT)
             c.set_reference_image(mR)
