@@ -168,8 +168,17 @@ __device__ __forceinline__ bool plan_encode(float u, int idx, int n, uint32_t& w
     if (!(u >= 0.0f && u < static_cast<float>(n + 8))) return false;
     const int fu = __float2int_rd(u);
     const float t = u - static_cast<float>(fu - 1);
+    // o ∈ [−n − 3, n + 3] (0 <= fu <= n + 7, 0 <= idx < n): two rounds of a
+    // conditional shift by n bring it into [−⌊n/2⌋, n − ⌊n/2⌋) for every
+    // n >= 4, exactly as ((o + ⌊n/2⌋) mod n) − ⌊n/2⌋, without an integer
+    // division
     int o = (fu - 4) - idx;
-    o = ((o + n / 2) % n + n) % n - n / 2;
+    const int h = n / 2;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (o < -h) o += n;
+        if (o >= n - h) o -= n;
+    }
     off = o;
     w = (__float_as_uint(t) & 0x7FFFFFu) | (static_cast<uint32_t>(o) << 23);
     return true;

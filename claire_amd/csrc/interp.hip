@@ -40,6 +40,27 @@ __device__ __forceinline__ void grid_index(const GatherDims& g, int64_t i, int i
     idx[2] = static_cast<int>(r - i2 * static_cast<uint32_t>(g.n3));
 }
 
+// Grid indices of a point index that advances by a fixed stride (the
+// bounded-grid loops of the plan kernels): the divisions happen once per
+// thread, and each step adds the stride's (i1, i2, i3) parts with carries.
+struct GridWalk {
+    int idx[3];
+    int s[3];
+    __device__ GridWalk(const GatherDims& g, int64_t i0, int64_t stride) {
+        grid_index(g, i0, idx);
+        grid_index(g, stride, s);  // stride = s0·n2·n3 + s1·n3 + s2, s1 < n2, s2 < n3
+    }
+    __device__ void advance(const GatherDims& g) {
+        idx[2] += s[2];
+        int c = idx[2] >= g.n3;
+        if (c) idx[2] -= g.n3;
+        idx[1] += s[1] + c;
+        c = idx[1] >= g.n2;
+        if (c) idx[1] -= g.n2;
+        idx[0] += s[0] + c;
+    }
+};
+
 __device__ __forceinline__ void load_q3(const float* X, int64_t i, float q[3]) {
     q[0] = X[3 * i + 0];
     q[1] = X[3 * i + 1];
@@ -224,13 +245,13 @@ __global__ __launch_bounds__(kThreads) void k_make_plan(GatherDims g, HH hh, con
     __shared__ int sh[kThreads];
     int need = 0;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < g.nl; i += stride) {
-        int idx[3];
-        grid_index(g, i, idx);
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    GridWalk gw(g, i0, stride);
+    for (int64_t i = i0; i < g.nl; i += stride, gw.advance(g)) {
         float q[3];
         load_q3(X, i, q);
         PlanPt pt;
-        need = max(need, encode_point(g, hh, q, idx, pt));
+        need = max(need, encode_point(g, hh, q, gw.idx, pt));
         P[i] = pt;
     }
     block_max_to(need, sh, dmax);
@@ -246,18 +267,18 @@ __global__ __launch_bounds__(kThreads) void k_predictor_plan(GatherDims g, HH hh
     int need = 0;
     const double twopi = 2.0 * 3.14159265358979323846;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < g.nl; i += stride) {
-        int idx[3];
-        grid_index(g, i, idx);
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    GridWalk gw(g, i0, stride);
+    for (int64_t i = i0; i < g.nl; i += stride, gw.advance(g)) {
         float q[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const float x = tp.hx[d] * static_cast<float>(idx[d]);
+            const float x = tp.hx[d] * static_cast<float>(gw.idx[d]);
             const float t = x - tp.sht * v.in[d][i];
             q[d] = static_cast<float>(static_cast<double>(t) / twopi);
         }
         PlanPt pt;
-        need = max(need, encode_point(g, hh, q, idx, pt));
+        need = max(need, encode_point(g, hh, q, gw.idx, pt));
         P[i] = pt;
     }
     block_max_to(need, sh, dmax);
@@ -271,9 +292,9 @@ __global__ __launch_bounds__(kThreads) void k_make_plan_soa(GatherDims g, HH hh,
     __shared__ int sh[kThreads];
     int need = 0;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < g.nl; i += stride) {
-        int idx[3];
-        grid_index(g, i, idx);
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    GridWalk gw(g, i0, stride);
+    for (int64_t i = i0; i < g.nl; i += stride, gw.advance(g)) {
         const float q[3] = {W[i], W[g.nl + i], W[2 * g.nl + i]};
         if (X) {
             X[3 * i + 0] = q[0];
@@ -281,7 +302,7 @@ __global__ __launch_bounds__(kThreads) void k_make_plan_soa(GatherDims g, HH hh,
             X[3 * i + 2] = q[2];
         }
         PlanPt pt;
-        need = max(need, encode_point(g, hh, q, idx, pt));
+        need = max(need, encode_point(g, hh, q, gw.idx, pt));
         P[i] = pt;
     }
     block_max_to(need, sh, dmax);
