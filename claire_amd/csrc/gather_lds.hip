@@ -474,6 +474,20 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
         load_point(x1 + LA_PT, Q[(K + LA_PT) & 3], A[(K + LA_PT) & 3]);
         load_plane(xf, PRE[(K + LA_PL) % NPRE]);  // plane x1 + HHI + PERIOD + LA_PL
         xf = (xf + 1 == g.n1) ? 0 : xf + 1;
+        // land input plane x1 + HHI + PERIOD (loaded LA_PL steps earlier) in
+        // the slot of plane x1 + HHI + PERIOD − W, which no step between
+        // this period's barriers reads.  Landing it before the point's
+        // stencil is read, not after the point is stored, lets the writes
+        // drain while the point is evaluated, and a wave reaches the
+        // period's barrier with no landing pending (measured at 256³,
+        // outputs identical: adjoint step 103.9 -> 103.1 µs, three-field
+        // gather 289.4 -> 287.3 µs).
+        int spare = cs + G::WIN + G::PERIOD - 1;
+        spare -= (spare >= G::W) ? G::W : 0;
+        float* dst = lds + spare * G::P;
+#pragma unroll
+        for (int q = 0; q < G::FILL; ++q)
+            if (tid + kLThreads * q < G::NV) store2(dst, q, PRE[K % NPRE][q][0], PRE[K % NPRE][q][1]);
         float w1[4], w2[4], w3[4];
         lagrange_weights(plan_t(cur.w[0]), w1);
         lagrange_weights(plan_t(cur.w[1]), w2);
@@ -529,15 +543,6 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
                 bstore(ro, epi.apply(v, caux), off, pb);
             if constexpr (Epi::kOut2) bstore(ro2, epi.apply2(v, caux), off, pb);
         }
-        // land input plane x1 + HHI + PERIOD (loaded one step earlier) in
-        // the slot of plane x1 + HHI + PERIOD − W, which no step between
-        // this period's barriers reads
-        int spare = cs + G::WIN + G::PERIOD - 1;
-        spare -= (spare >= G::W) ? G::W : 0;
-        float* dst = lds + spare * G::P;
-#pragma unroll
-        for (int q = 0; q < G::FILL; ++q)
-            if (tid + kLThreads * q < G::NV) store2(dst, q, PRE[K % NPRE][q][0], PRE[K % NPRE][q][1]);
         cs = (cs + 1 == G::W) ? 0 : cs + 1;
         if constexpr (K % G::PERIOD == G::PERIOD - 1) __syncthreads();
     };
