@@ -361,12 +361,19 @@ k_gather_lds_pair(GatherDims g, const PlanPt* __restrict__ plan, const float* __
     const int tid = threadIdx.x;
     const int nt3 = (g.n3 + G::LT3 - 1) / G::LT3;
     const int nt2 = (g.n2 + kLT2 - 1) / kLT2;
-    const int L = xcd_remap(blockIdx.x, nblocks);
+    // One block per (tile, chunk, field) on a 1D grid.  The fields of a tile
+    // are neighbours in each XCD's range, so the plan words one of them loads
+    // are L2 hits for the others (three-field gather at 256³ 288.0 -> 285.4
+    // µs, outputs identical).
+    const int nz = static_cast<int>(gridDim.x) / nblocks;
+    const int Lz = xcd_remap(blockIdx.x, nblocks * nz);
+    const int z = Lz % nz;
+    const int L = Lz / nz;
     const int t3 = L % nt3;
     const int t2 = (L / nt3) % nt2;
     const int ch = L / (nt3 * nt2);
-    const float* f = fbase + static_cast<int64_t>(blockIdx.y) * fzstride;
-    epi.select(blockIdx.y);
+    const float* f = fbase + static_cast<int64_t>(z) * fzstride;
+    epi.select(z);
     const uint32_t fbytes = static_cast<uint32_t>(g.nl) * 4u;
     const rsrc_t rf = make_rsrc(f, fbytes);
     const rsrc_t rp = make_rsrc(plan, 3u * fbytes);
@@ -909,7 +916,7 @@ void launch_lds(const GatherDims& g, const uint32_t* P, const float* f, int64_t 
         const int nch = (g.n1 + chunk - 1) / chunk;
         const int nb = nt3 * nt2 * nch;
         if constexpr (PAIR)
-            hipLaunchKernelGGL((k_gather_lds_pair<Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
+            hipLaunchKernelGGL((k_gather_lds_pair<Epi>), dim3(nb * nz), dim3(kLThreads), 0, st, g,
                                reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
         else
             hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
