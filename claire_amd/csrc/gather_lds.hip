@@ -127,12 +127,17 @@ __global__ __launch_bounds__(kLThreads) __attribute__((amdgpu_waves_per_eu(LdsGe
     const int tid = threadIdx.x;
     const int nt3 = (g.n3 + G::LT3 - 1) / G::LT3;
     const int nt2 = (g.n2 + kLT2 - 1) / kLT2;
-    const int L = xcd_remap(blockIdx.x, nblocks);
+    // one block per (tile, chunk, field), a tile's fields adjacent (as in
+    // k_gather_lds_pair)
+    const int nz = static_cast<int>(gridDim.x) / nblocks;
+    const int Lz = xcd_remap(blockIdx.x, nblocks * nz);
+    const int z = Lz % nz;
+    const int L = Lz / nz;
     const int t3 = L % nt3;
     const int t2 = (L / nt3) % nt2;
     const int ch = L / (nt3 * nt2);
-    const float* f = fbase + static_cast<int64_t>(blockIdx.y) * fzstride;
-    epi.select(blockIdx.y);
+    const float* f = fbase + static_cast<int64_t>(z) * fzstride;
+    epi.select(z);
     // Byte sizes fit 32 bits: the host requires nl < 2^28 (12 B per point).
     const uint32_t fbytes = static_cast<uint32_t>(g.nl) * 4u;
     const rsrc_t rf = make_rsrc(f, fbytes);
@@ -919,7 +924,7 @@ void launch_lds(const GatherDims& g, const uint32_t* P, const float* f, int64_t 
             hipLaunchKernelGGL((k_gather_lds_pair<Epi>), dim3(nb * nz), dim3(kLThreads), 0, st, g,
                                reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
         else
-            hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb, nz), dim3(kLThreads), 0, st, g,
+            hipLaunchKernelGGL((k_gather_lds<D, Epi>), dim3(nb * nz), dim3(kLThreads), 0, st, g,
                                reinterpret_cast<const PlanPt*>(P), f, fzstride, epi, nb, chunk);
     };
     const ShapeKey key{PAIR ? 100 + D : D, g.n1, g.n2, g.n3, nz};
