@@ -940,9 +940,10 @@ void dispatch(int dbound, const GatherDims& g, const uint32_t* P, const float* f
     CLR_REQUIRE(g.nl < (int64_t(1) << 28), kErrInvalidArgument, "grid too large for 32-bit buffer byte offsets");
     CLR_REQUIRE(dbound >= 0 && dbound <= 4, kErrInvalidArgument, "displacement bound too large for the LDS gather");
     // The pair layout is faster for one field per launch (the solves' steps:
-    // 110 µs against 115 at 256³) and for plain three-field gathers (295
-    // against 300 µs); the trajectory stages lose 6-12% with it (two blocks
-    // per CU instead of three), so they keep the one-value layout.
+    // 110 µs against 115 at 256³, before later changes to both kernels) and
+    // for plain three-field gathers (295 against 300 µs); the trajectory
+    // stages were 6-12% slower with it (two blocks per CU instead of three)
+    // and now only match the one-value layout, so they keep that one.
     constexpr bool kPlain = std::is_same<Epi, EpiPlain>::value;
     if (dbound <= 2 && (nz == 1 || kPlain) && pair_layout())
         launch_lds<2, true>(g, P, f, fzstride, nz, epi, st);
